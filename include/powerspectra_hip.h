@@ -1,0 +1,166 @@
+/*
+ * powerspectra_hip.h -- C ABI of the MI355X (gfx950) mode-coupling and
+ * covariance engine that replaces the inner loops of PowerSpectra.jl.
+ *
+ * Every entry point here is what a Julia `ccall` (or Python ctypes) binds:
+ * plain pointers, sizes and an int status.  No exceptions, no C++ or torch
+ * types cross the boundary.
+ *
+ * Conventions shared by all entry points
+ *   - FP64 everywhere on the boundary.
+ *   - Matrices are column-major.  Element (l1, l2) of an lmin:lmax block is at
+ *     M[(l1 - lmin) + (l2 - lmin) * ldM]  (parent(SpectralArray) layout,
+ *     reference src/spectralarray.jl:5-45, src/modecoupling.jl:90-91).
+ *   - Spectra (W, signal, noise ratios) are 0-based: W[l] for l = 0 .. LW-1
+ *     (SpectralVector, reference src/spectralarray.jl:17).
+ *   - Host-pointer entry points copy in and out; no pointer is retained.
+ *     `_d` entry points take device pointers and enqueue on the context's
+ *     stream; they return after the work is enqueued (call
+ *     psx_ctx_synchronize to wait).
+ *   - Calls on one context are serialised by the context.  One context owns
+ *     one HIP stream on one device.
+ */
+#ifndef POWERSPECTRA_HIP_H
+#define POWERSPECTRA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+enum psx_status {
+  PSX_OK = 0,
+  PSX_ERR_ARGUMENT = 1,   /* bad spec / kind / size  (Julia ArgumentError)       */
+  PSX_ERR_AXES = 2,       /* axis mismatch           (Julia AssertionError)      */
+  PSX_ERR_SINGULAR = 3,   /* zero pivot in LU        (LinearAlgebra.SingularException) */
+  PSX_ERR_DOMAIN = 4,     /* sqrt of a negative spectrum product (DomainError)   */
+  PSX_ERR_BUDGET = 5,     /* request exceeds the context's HBM budget            */
+  PSX_ERR_DEVICE = 6,     /* HIP / rocSOLVER failure                             */
+  PSX_ERR_NOT_READY = 7   /* serving: ticket not finished yet                    */
+};
+
+/* ---- mode-coupling kinds (reference src/modecoupling.jl:78-159) -------- */
+enum psx_mcm_kind {
+  PSX_MCM_00 = 0,  /* inner_mcm⁰⁰!  (TT)                     :78-95   */
+  PSX_MCM_02 = 1,  /* inner_mcm⁰²!  (TE, ET, TB, BT)         :99-119  */
+  PSX_MCM_PP = 2,  /* inner_mcm⁺⁺!  (spin-2, even l1+l2+l3)  :123-139 */
+  PSX_MCM_MM = 3   /* inner_mcm⁻⁻!  (spin-2, odd  l1+l2+l3)  :143-159 */
+};
+
+/* ---- covariance kinds (reference src/covariance.jl:92-446) ------------- */
+enum psx_cov_kind {
+  PSX_COV_TTTT = 0,        /* loop_covTTTT!         :92-122  K=8 */
+  PSX_COV_EEEE = 1,        /* loop_covEEEE!         :153-183 K=8 */
+  PSX_COV_TTTE = 2,        /* loop_covTTTE!         :208-235 K=4 */
+  PSX_COV_TETE = 3,        /* loop_covTETE!         :261-302 K=5 */
+  PSX_COV_TEEE = 4,        /* loop_covTEEE!         :337-372 K=4 (planck=false) */
+  PSX_COV_TEEE_PLANCK = 5, /* loop_covTEEE_planck!  :376-402 K=4 (default)      */
+  PSX_COV_TTEE = 6         /* loop_covTTEE!         :422-446 K=2 */
+};
+
+/* Number of W spectra, signal spectra and noise-ratio spectra each covariance
+ * kind takes, in the argument order of the reference loop (see
+ * psx_cov).  Returns PSX_ERR_ARGUMENT for an unknown kind. */
+int psx_cov_arity(int kind, int* nW, int* nspectra, int* nratios);
+
+enum psx_precision {
+  PSX_FP64 = 0,   /* all arithmetic in FP64                                   */
+  PSX_MIXED = 1   /* covariance contraction on MFMA, 3j terms and epilogue FP64 */
+};
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct psx_ctx psx_ctx;
+
+/* Create a context on HIP device `device`.  `hbm_budget` bounds the bytes the
+ * context may hold for workspaces and resident spectra (0 = 90% of free HBM). */
+int psx_ctx_create(int device, size_t hbm_budget, psx_ctx** out);
+int psx_ctx_destroy(psx_ctx* ctx);
+int psx_ctx_synchronize(psx_ctx* ctx);
+/* Human-readable message for the last error on this context (never NULL). */
+const char* psx_ctx_last_error(psx_ctx* ctx);
+/* Bytes of device memory the context holds right now. */
+size_t psx_ctx_bytes_in_use(psx_ctx* ctx);
+
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
+ * on the context stream and its duration accumulated under the kernel's name. */
+int psx_ctx_set_timing(psx_ctx* ctx, int enable);
+/* Fetch accumulated time (ms) and launch count for kernel `name`; resets it
+ * when `reset` != 0. Unknown names report zero launches. */
+int psx_ctx_kernel_time(psx_ctx* ctx, const char* name, int reset,
+                        double* total_ms, int64_t* launches);
+
+/* Device memory owned by the context (for resident inputs / outputs). */
+int psx_dev_alloc(psx_ctx* ctx, size_t bytes, void** dptr);
+int psx_dev_free(psx_ctx* ctx, void* dptr);
+int psx_memcpy_h2d(psx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int psx_memcpy_d2h(psx_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- mode-coupling matrices ------------------------------------------- */
+/* One block.  Replaces inner_mcm⁰⁰!/⁰²!/⁺⁺!/⁻⁻!(M, V) (reference
+ * src/modecoupling.jl:78-159), i.e. fills M[l1,l2] = (2l2+1) Ξ and
+ * M[l2,l1] = (2l1+1) Ξ for lmin <= l1 <= l2 <= lmax, with Ξ the projector of
+ * src/modecoupling.jl:3-66 summed over l3 <= min(l1+l2, LW-1).
+ * The reference passes V = alm2cl(...)[0:lmax], i.e. LW = lmax+1
+ * (src/modecoupling.jl:197); LW is a free parameter here. */
+int psx_mcm(psx_ctx* ctx, int kind, int lmin, int lmax,
+            const double* W, int64_t LW, double* M, int64_t ldM);
+int psx_mcm_d(psx_ctx* ctx, int kind, int lmin, int lmax,
+              const double* dW, int64_t LW, double* dM, int64_t ldM);
+
+/* Fused spin-2 pair: one pass fills M⁺⁺ and M⁻⁻ from the same W (the two
+ * reference passes inner_mcm⁺⁺! and inner_mcm⁻⁻! of src/modecoupling.jl:216-217
+ * or :238-239 share their (0 -2 2) family). */
+int psx_mcm_pm(psx_ctx* ctx, int lmin, int lmax, const double* W, int64_t LW,
+               double* Mpp, double* Mmm, int64_t ldM);
+int psx_mcm_pm_d(psx_ctx* ctx, int lmin, int lmax, const double* dW, int64_t LW,
+                 double* dMpp, double* dMmm, int64_t ldM);
+
+/* Full polarised set in one pass over (l1, l2): TT from W_TT, TE from W_TE,
+ * ET from W_ET, ++ and -- from W_PP -- the five mcm() calls made by
+ * maskedalm2spectra (reference src/modecoupling.jl:348-377).  Any W pointer
+ * may be NULL to skip that block (its M pointer is then ignored). */
+int psx_mcm_set(psx_ctx* ctx, int lmin, int lmax, int64_t LW,
+                const double* W_TT, const double* W_TE, const double* W_ET,
+                const double* W_PP,
+                double* M_TT, double* M_TE, double* M_ET, double* M_PP,
+                double* M_MM, int64_t ldM);
+int psx_mcm_set_d(psx_ctx* ctx, int lmin, int lmax, int64_t LW,
+                  const double* dW_TT, const double* dW_TE, const double* dW_ET,
+                  const double* dW_PP,
+                  double* dM_TT, double* dM_TE, double* dM_ET, double* dM_PP,
+                  double* dM_MM, int64_t ldM);
+
+/* ---- coupled covariance ------------------------------------------------ */
+/* Replaces loop_cov*!(C, spectra..., ratios..., W1..WK) (reference
+ * src/covariance.jl:92-446).  `spectra`, `ratios` and `W` are arrays of
+ * pointers in exactly the reference loop's argument order; psx_cov_arity
+ * gives their counts.  Spectra and ratios are 0-based with length >= lmax+1,
+ * W has length LW (reference: workspace.lmax+1).  Fills C[l1,l2] = C[l2,l1]
+ * for lmin <= l1 <= l2 <= lmax.  `precision` is PSX_FP64 or PSX_MIXED. */
+int psx_cov(psx_ctx* ctx, int kind, int lmin, int lmax,
+            const double* const* spectra, const double* const* ratios,
+            const double* const* W, int64_t LW, int precision,
+            double* C, int64_t ldC);
+int psx_cov_d(psx_ctx* ctx, int kind, int lmin, int lmax,
+              const double* const* dspectra, const double* const* dratios,
+              const double* const* dW, int64_t LW, int precision,
+              double* dC, int64_t ldC);
+
+/* ---- FP64 dense linear algebra (rocSOLVER) ----------------------------- */
+/* decouple_covmat (reference src/covariance.jl:8-14): out = B1^{-1} Y B2^{-T},
+ * all n x n column-major with leading dimension n.  Y is not modified. */
+int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
+                        const double* B2, double* out);
+/* `A \ B` (reference src/blockspectralmatrix.jl:101-129): LU with partial
+ * pivoting of the n x n matrix A, then solve for nrhs right-hand sides in B
+ * (n x nrhs, overwritten with the solution).  A is not modified. */
+int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
+                 double* B, int64_t ldb, int nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POWERSPECTRA_HIP_H */

@@ -1,0 +1,311 @@
+"""ctypes binding of the C ABI in include/powerspectra_hip.h (libpsx_hip.so).
+
+This is the only way the Python mirror reaches the device: every array goes
+through the HIP library.  If the shared library is missing the import fails
+loudly -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsx_hip.so")
+
+# status codes (enum psx_status)
+PSX_OK = 0
+PSX_ERR_ARGUMENT = 1
+PSX_ERR_AXES = 2
+PSX_ERR_SINGULAR = 3
+PSX_ERR_DOMAIN = 4
+PSX_ERR_BUDGET = 5
+PSX_ERR_DEVICE = 6
+PSX_ERR_NOT_READY = 7
+
+# enum psx_mcm_kind
+MCM_00, MCM_02, MCM_PP, MCM_MM = 0, 1, 2, 3
+# enum psx_cov_kind
+COV_TTTT, COV_EEEE, COV_TTTE, COV_TETE, COV_TEEE, COV_TEEE_PLANCK, COV_TTEE = range(7)
+# enum psx_precision
+FP64, MIXED = 0, 1
+
+
+class PowerSpectraError(RuntimeError):
+    """Base class of errors reported by the HIP library."""
+
+
+class ArgumentError(PowerSpectraError, ValueError):
+    """Julia ArgumentError: bad spectrum name, kind or size."""
+
+
+class AxisError(PowerSpectraError, AssertionError):
+    """Julia AssertionError on mismatched SpectralArray axes."""
+
+
+class SingularException(PowerSpectraError, ArithmeticError):
+    """LinearAlgebra.SingularException: exact zero pivot in LU."""
+
+
+class DomainError(PowerSpectraError, ArithmeticError):
+    """Julia DomainError: sqrt of a negative spectrum product."""
+
+
+class BudgetError(PowerSpectraError, MemoryError):
+    """Request exceeds the context's HBM budget."""
+
+
+class DeviceError(PowerSpectraError):
+    """HIP / rocSOLVER failure."""
+
+
+_ERRORS = {
+    PSX_ERR_ARGUMENT: ArgumentError,
+    PSX_ERR_AXES: AxisError,
+    PSX_ERR_SINGULAR: SingularException,
+    PSX_ERR_DOMAIN: DomainError,
+    PSX_ERR_BUDGET: BudgetError,
+    PSX_ERR_DEVICE: DeviceError,
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+
+# name -> (restype, argtypes); mirrors include/powerspectra_hip.h
+SIGNATURES = {
+    "psx_cov_arity": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "psx_ctx_create": (_int, [_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "psx_ctx_destroy": (_int, [_vp]),
+    "psx_ctx_synchronize": (_int, [_vp]),
+    "psx_ctx_last_error": (ctypes.c_char_p, [_vp]),
+    "psx_ctx_bytes_in_use": (ctypes.c_size_t, [_vp]),
+    "psx_ctx_set_timing": (_int, [_vp, _int]),
+    "psx_ctx_kernel_time": (_int, [_vp, ctypes.c_char_p, _int, _dp, ctypes.POINTER(_i64)]),
+    "psx_dev_alloc": (_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "psx_dev_free": (_int, [_vp, _vp]),
+    "psx_memcpy_h2d": (_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "psx_memcpy_d2h": (_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "psx_mcm": (_int, [_vp, _int, _int, _int, _dp, _i64, _dp, _i64]),
+    "psx_mcm_d": (_int, [_vp, _int, _int, _int, _vp, _i64, _vp, _i64]),
+    "psx_mcm_pm": (_int, [_vp, _int, _int, _dp, _i64, _dp, _dp, _i64]),
+    "psx_mcm_pm_d": (_int, [_vp, _int, _int, _vp, _i64, _vp, _vp, _i64]),
+    "psx_mcm_set": (_int, [_vp, _int, _int, _i64] + [_dp] * 4 + [_dp] * 5 + [_i64]),
+    "psx_mcm_set_d": (_int, [_vp, _int, _int, _i64] + [_vp] * 4 + [_vp] * 5 + [_i64]),
+    "psx_cov": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                       ctypes.POINTER(_dp), _i64, _int, _dp, _i64]),
+    "psx_cov_d": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                         ctypes.POINTER(_vp), _i64, _int, _vp, _i64]),
+    "psx_decouple_covmat": (_int, [_vp, _int, _dp, _dp, _dp, _dp]),
+    "psx_lu_solve": (_int, [_vp, _int, _dp, _i64, _dp, _i64, _int]),
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """Load libpsx_hip.so (raises if it has not been built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with powerspectra.jl_amd/build.py "
+                    "(the HIP library is the only compute path)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _block_size(lmin: int, lmax: int) -> int:
+    if lmin < 0 or lmax < lmin:
+        raise ArgumentError(f"need 0 <= lmin <= lmax (got lmin={lmin}, lmax={lmax})")
+    return lmax - lmin + 1
+
+
+class Context:
+    """One HIP device context (psx_ctx): a stream, an HBM budget, caches."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = lib()
+        h = _vp()
+        st = L.psx_ctx_create(device, hbm_budget, ctypes.byref(h))
+        if st != PSX_OK:
+            raise _ERRORS.get(st, PowerSpectraError)(f"psx_ctx_create failed with status {st}")
+        self._h = h
+        self.device = device
+        self.lib = L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, st: int, what: str = ""):
+        if st != PSX_OK:
+            msg = self.lib.psx_ctx_last_error(self._h)
+            msg = msg.decode() if msg else ""
+            raise _ERRORS.get(st, PowerSpectraError)(f"{what}: {msg} (status {st})")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.psx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- utilities -------------------------------------------------------
+    def synchronize(self):
+        self.check(self.lib.psx_ctx_synchronize(self._h), "synchronize")
+
+    def bytes_in_use(self) -> int:
+        return int(self.lib.psx_ctx_bytes_in_use(self._h))
+
+    def set_timing(self, on: bool):
+        self.check(self.lib.psx_ctx_set_timing(self._h, int(bool(on))), "set_timing")
+
+    def kernel_time(self, name: str, reset: bool = False):
+        ms = ctypes.c_double(0)
+        n = _i64(0)
+        self.check(self.lib.psx_ctx_kernel_time(self._h, name.encode(), int(reset),
+                                                ctypes.byref(ms), ctypes.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def alloc(self, nbytes: int) -> int:
+        p = _vp()
+        self.check(self.lib.psx_dev_alloc(self._h, nbytes, ctypes.byref(p)), "dev_alloc")
+        return p.value
+
+    def free(self, p: int):
+        self.check(self.lib.psx_dev_free(self._h, _vp(p)), "dev_free")
+
+    def h2d(self, dst: int, src: np.ndarray):
+        src = np.ascontiguousarray(src)
+        self.check(self.lib.psx_memcpy_h2d(self._h, _vp(dst), src.ctypes.data_as(_vp), src.nbytes),
+                   "memcpy_h2d")
+
+    def d2h(self, dst: np.ndarray, src: int):
+        assert dst.flags.c_contiguous or dst.flags.f_contiguous
+        self.check(self.lib.psx_memcpy_d2h(self._h, dst.ctypes.data_as(_vp), _vp(src), dst.nbytes),
+                   "memcpy_d2h")
+
+    # --- compute ---------------------------------------------------------
+    def mcm(self, kind: int, lmin: int, lmax: int, W: np.ndarray, out: np.ndarray | None = None):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        n = _block_size(lmin, lmax)
+        if out is None:
+            out = np.empty((n, n), order="F")
+        self.check(self.lib.psx_mcm(self._h, kind, lmin, lmax, dptr(W), W.size, dptr(out), n),
+                   "psx_mcm")
+        return out
+
+    def mcm_pm(self, lmin: int, lmax: int, W: np.ndarray):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        n = _block_size(lmin, lmax)
+        P = np.empty((n, n), order="F")
+        M = np.empty((n, n), order="F")
+        self.check(self.lib.psx_mcm_pm(self._h, lmin, lmax, dptr(W), W.size, dptr(P), dptr(M), n),
+                   "psx_mcm_pm")
+        return P, M
+
+    def mcm_set(self, lmin: int, lmax: int, W_TT, W_TE, W_ET, W_PP):
+        Ws = [None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+              for w in (W_TT, W_TE, W_ET, W_PP)]
+        LW = next(w.size for w in Ws if w is not None)
+        if any(w is not None and w.size != LW for w in Ws):
+            raise ArgumentError("all W spectra must have the same length")
+        n = _block_size(lmin, lmax)
+        want = [Ws[0] is not None, Ws[1] is not None, Ws[2] is not None,
+                Ws[3] is not None, Ws[3] is not None]
+        outs = [np.empty((n, n), order="F") if w else None for w in want]
+        nullp = ctypes.cast(None, _dp)
+        args = [nullp if w is None else dptr(w) for w in Ws]
+        oargs = [nullp if o is None else dptr(o) for o in outs]
+        self.check(self.lib.psx_mcm_set(self._h, lmin, lmax, LW, *args, *oargs, n), "psx_mcm_set")
+        return outs
+
+    def cov(self, kind: int, lmin: int, lmax: int, spectra, ratios, W, precision: int = FP64):
+        nW, nS, nR = cov_arity(kind)
+        _block_size(lmin, lmax)
+        if len(W) != nW or len(spectra) != nS or len(ratios) != nR:
+            raise ArgumentError(f"covariance kind {kind} takes {nW} W, {nS} spectra, {nR} ratios")
+        spectra = [np.ascontiguousarray(s, dtype=np.float64) for s in spectra]
+        ratios = [np.ascontiguousarray(r, dtype=np.float64) for r in ratios]
+        W = [np.ascontiguousarray(w, dtype=np.float64) for w in W]
+        for s in spectra + ratios:
+            if s.size < lmax + 1:
+                raise ArgumentError("spectra and noise ratios need at least lmax+1 entries")
+        LW = W[0].size
+        if any(w.size != LW for w in W):
+            raise ArgumentError("all W spectra must have the same length")
+        n = lmax - lmin + 1
+        out = np.empty((n, n), order="F")
+        sp = (_dp * 4)(*[dptr(s) for s in spectra])
+        rp = (_dp * 4)(*([dptr(r) for r in ratios] + [ctypes.cast(None, _dp)] * (4 - nR)))
+        wp = (_dp * 8)(*([dptr(w) for w in W] + [ctypes.cast(None, _dp)] * (8 - nW)))
+        self.check(self.lib.psx_cov(self._h, kind, lmin, lmax, sp, rp, wp, LW, precision,
+                                    dptr(out), n), "psx_cov")
+        return out
+
+    def decouple_covmat(self, Y, B1, B2):
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        B1 = np.asfortranarray(B1, dtype=np.float64)
+        B2 = np.asfortranarray(B2, dtype=np.float64)
+        n = Y.shape[0]
+        if Y.shape != (n, n) or B1.shape != (n, n) or B2.shape != (n, n):
+            raise AxisError("decouple_covmat needs three n x n matrices")
+        out = np.empty((n, n), order="F")
+        self.check(self.lib.psx_decouple_covmat(self._h, n, dptr(Y), dptr(B1), dptr(B2),
+                                                dptr(out)), "psx_decouple_covmat")
+        return out
+
+    def lu_solve(self, A, B):
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        if A.shape != (n, n):
+            raise AxisError("lu_solve needs a square matrix")
+        vec = np.ndim(B) == 1
+        X = np.array(B, dtype=np.float64, order="F", copy=True)
+        if vec:
+            X = X.reshape(n, 1, order="F")
+        if X.shape[0] != n:
+            raise AxisError("right-hand side length does not match the matrix")
+        self.check(self.lib.psx_lu_solve(self._h, n, dptr(A), n, dptr(X), n, X.shape[1]),
+                   "psx_lu_solve")
+        return X[:, 0] if vec else X
+
+
+def cov_arity(kind: int):
+    nW, nS, nR = _int(0), _int(0), _int(0)
+    st = lib().psx_cov_arity(kind, ctypes.byref(nW), ctypes.byref(nS), ctypes.byref(nR))
+    if st != PSX_OK:
+        raise ArgumentError(f"unknown covariance kind {kind}")
+    return nW.value, nS.value, nR.value
+
+
+_default_ctx = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    """Process-wide context on device 0 (created on first use)."""
+    global _default_ctx
+    with _default_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(int(os.environ.get("PSX_DEVICE", "0")))
+    return _default_ctx

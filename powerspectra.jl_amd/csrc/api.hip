@@ -1,0 +1,333 @@
+// extern "C" entry points for mode-coupling matrices and covariance blocks.
+#include "psx_internal.h"
+
+#include <cstring>
+
+using namespace psx;
+
+namespace {
+
+Ctx* C(psx_ctx* h) { return reinterpret_cast<Ctx*>(h); }
+
+void check_block(int lmin, int lmax, int64_t LW, int64_t ld) {
+  PSX_REQUIRE(lmin >= 0 && lmax >= lmin, PSX_ERR_ARGUMENT, "need 0 <= lmin <= lmax");
+  PSX_REQUIRE(LW >= 1, PSX_ERR_ARGUMENT, "W must have at least one entry");
+  PSX_REQUIRE(ld >= lmax - lmin + 1, PSX_ERR_AXES,
+              "leading dimension smaller than lmax-lmin+1");
+}
+
+// Scoped device buffer owned by the context.
+struct DevBuf {
+  Ctx* c;
+  void* p = nullptr;
+  DevBuf(Ctx* cc, size_t bytes) : c(cc) { p = ctx_alloc(c, bytes); }
+  ~DevBuf() {
+    if (p) {
+      (void)hipStreamSynchronize(c->stream);
+      try {
+        ctx_free(c, p);
+      } catch (...) {
+      }
+    }
+  }
+  double* d() const { return static_cast<double*>(p); }
+};
+
+void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
+  PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
+// copy an n x n device block (ld n) into a host block with leading dim ld
+void d2h_block(Ctx* c, double* host, int64_t ld, const double* dev, int n) {
+  PSX_HIP(hipMemcpy2DAsync(host, ld * sizeof(double), dev, n * sizeof(double),
+                           n * sizeof(double), n, hipMemcpyDeviceToHost, c->stream));
+}
+
+int mcm_family(int kind) {
+  switch (kind) {
+    case PSX_MCM_00: return FAM_00;
+    case PSX_MCM_02: return FAM_02;
+    case PSX_MCM_PP: return FAM_PE;
+    case PSX_MCM_MM: return FAM_PO;
+  }
+  throw Error{PSX_ERR_ARGUMENT, "unknown mcm kind (valid: 00, 02, PP, MM)"};
+}
+
+// W -> V table order for each covariance kind, and the family of each table
+// (reference argument order W1..WK of loop_cov*!).
+void cov_layout(int kind, ContractRequest& rq, const double* const* W) {
+  int nW = 0;
+  PSX_REQUIRE(psx_cov_arity(kind, &nW, nullptr, nullptr) == PSX_OK, PSX_ERR_ARGUMENT,
+              "unknown covariance kind");
+  auto put = [&](int fam, std::initializer_list<int> idx) {
+    int k = rq.nfam[0] + rq.nfam[1] + rq.nfam[2] + rq.nfam[3];
+    for (int i : idx) rq.dW[k++] = W[i];
+    rq.nfam[fam] += (int)idx.size();
+  };
+  switch (kind) {
+    case PSX_COV_TTTT: put(FAM_00, {0, 1, 2, 3, 4, 5, 6, 7}); break;
+    case PSX_COV_EEEE: put(FAM_PE, {0, 1, 2, 3, 4, 5, 6, 7}); break;
+    case PSX_COV_TTTE: put(FAM_00, {0, 1, 2, 3}); break;
+    case PSX_COV_TETE:
+      put(FAM_00, {1});
+      put(FAM_02, {0, 2, 3, 4});
+      break;
+    case PSX_COV_TEEE: put(FAM_02, {0, 1, 2, 3}); break;
+    case PSX_COV_TEEE_PLANCK: put(FAM_PE, {0, 1, 2, 3}); break;
+    case PSX_COV_TTEE: put(FAM_00, {0, 1}); break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int psx_cov_arity(int kind, int* nW, int* nspectra, int* nratios) {
+  static const int tab[7][3] = {{8, 4, 4}, {8, 4, 4}, {4, 4, 2}, {5, 4, 2},
+                                {4, 4, 2}, {4, 4, 2}, {2, 4, 0}};
+  if (kind < 0 || kind > 6) return PSX_ERR_ARGUMENT;
+  if (nW) *nW = tab[kind][0];
+  if (nspectra) *nspectra = tab[kind][1];
+  if (nratios) *nratios = tab[kind][2];
+  return PSX_OK;
+}
+
+int psx_mcm_d(psx_ctx* h, int kind, int lmin, int lmax, const double* dW,
+              int64_t LW, double* dM, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_REQUIRE(dW && dM, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    ContractRequest rq;
+    rq.lmin = lmin;
+    rq.lmax = lmax;
+    rq.LW = LW;
+    rq.nfam[mcm_family(kind)] = 1;
+    rq.dW[0] = dW;
+    rq.epi = EPI_MCM;
+    rq.dout[0] = dM;
+    rq.ldo = ldM;
+    run_contraction(c, rq);
+  });
+}
+
+int psx_mcm(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t LW,
+            double* M, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_REQUIRE(W && M, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    DevBuf dW(c, sizeof(double) * LW), dM(c, sizeof(double) * n * (size_t)n);
+    h2d(c, dW.p, W, sizeof(double) * LW);
+    int st = psx_mcm_d(h, kind, lmin, lmax, dW.d(), LW, dM.d(), n);
+    if (st != PSX_OK) throw Error{st, c->last_error};
+    d2h_block(c, M, ldM, dM.d(), n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int psx_mcm_pm_d(psx_ctx* h, int lmin, int lmax, const double* dW, int64_t LW,
+                 double* dMpp, double* dMmm, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_REQUIRE(dW && dMpp && dMmm, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    ContractRequest rq;
+    rq.lmin = lmin;
+    rq.lmax = lmax;
+    rq.LW = LW;
+    rq.nfam[FAM_PE] = 1;
+    rq.nfam[FAM_PO] = 1;
+    rq.dW[0] = dW;
+    rq.dW[1] = dW;
+    rq.epi = EPI_MCM;
+    rq.dout[0] = dMpp;
+    rq.dout[1] = dMmm;
+    rq.ldo = ldM;
+    run_contraction(c, rq);
+  });
+}
+
+int psx_mcm_pm(psx_ctx* h, int lmin, int lmax, const double* W, int64_t LW,
+               double* Mpp, double* Mmm, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_REQUIRE(W && Mpp && Mmm, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    DevBuf dW(c, sizeof(double) * LW), dP(c, sizeof(double) * n * (size_t)n),
+        dQ(c, sizeof(double) * n * (size_t)n);
+    h2d(c, dW.p, W, sizeof(double) * LW);
+    int st = psx_mcm_pm_d(h, lmin, lmax, dW.d(), LW, dP.d(), dQ.d(), n);
+    if (st != PSX_OK) throw Error{st, c->last_error};
+    d2h_block(c, Mpp, ldM, dP.d(), n);
+    d2h_block(c, Mmm, ldM, dQ.d(), n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int psx_mcm_set_d(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* dW_TT,
+                  const double* dW_TE, const double* dW_ET, const double* dW_PP,
+                  double* dM_TT, double* dM_TE, double* dM_ET, double* dM_PP,
+                  double* dM_MM, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_HIP(hipSetDevice(c->device));
+    const bool all = dW_TT && dW_TE && dW_ET && dW_PP;
+    if (all) {
+      PSX_REQUIRE(dM_TT && dM_TE && dM_ET && dM_PP && dM_MM, PSX_ERR_ARGUMENT,
+                  "null output pointer");
+      ContractRequest rq;
+      rq.lmin = lmin;
+      rq.lmax = lmax;
+      rq.LW = LW;
+      rq.nfam[FAM_00] = 1;
+      rq.nfam[FAM_02] = 2;
+      rq.nfam[FAM_PE] = 1;
+      rq.nfam[FAM_PO] = 1;
+      const double* Ws[5] = {dW_TT, dW_TE, dW_ET, dW_PP, dW_PP};
+      double* Ms[5] = {dM_TT, dM_TE, dM_ET, dM_PP, dM_MM};
+      for (int k = 0; k < 5; ++k) {
+        rq.dW[k] = Ws[k];
+        rq.dout[k] = Ms[k];
+      }
+      rq.epi = EPI_MCM;
+      rq.ldo = ldM;
+      run_contraction(c, rq);
+      return;
+    }
+    // partial set: run the blocks that were asked for
+    int st = PSX_OK;
+    if (dW_TT && st == PSX_OK) st = psx_mcm_d(h, PSX_MCM_00, lmin, lmax, dW_TT, LW, dM_TT, ldM);
+    if (dW_TE && st == PSX_OK) st = psx_mcm_d(h, PSX_MCM_02, lmin, lmax, dW_TE, LW, dM_TE, ldM);
+    if (dW_ET && st == PSX_OK) st = psx_mcm_d(h, PSX_MCM_02, lmin, lmax, dW_ET, LW, dM_ET, ldM);
+    if (dW_PP && st == PSX_OK)
+      st = psx_mcm_pm_d(h, lmin, lmax, dW_PP, LW, dM_PP, dM_MM, ldM);
+    if (st != PSX_OK) throw Error{st, c->last_error};
+  });
+}
+
+int psx_mcm_set(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
+                const double* W_TE, const double* W_ET, const double* W_PP,
+                double* M_TT, double* M_TE, double* M_ET, double* M_PP, double* M_MM,
+                int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
+    const double* Wh[4] = {W_TT, W_TE, W_ET, W_PP};
+    double* Mh[5] = {M_TT, M_TE, M_ET, M_PP, M_MM};
+    DevBuf dw(c, 4 * wb), dm(c, 5 * mb);
+    const double* dWp[4];
+    double* dMp[5];
+    for (int k = 0; k < 4; ++k) {
+      dWp[k] = nullptr;
+      if (Wh[k]) {
+        double* p = dw.d() + k * LW;
+        h2d(c, p, Wh[k], wb);
+        dWp[k] = p;
+      }
+    }
+    for (int k = 0; k < 5; ++k) dMp[k] = dm.d() + (size_t)k * n * n;
+    const bool want[5] = {W_TT != nullptr, W_TE != nullptr, W_ET != nullptr,
+                          W_PP != nullptr, W_PP != nullptr};
+    for (int k = 0; k < 5; ++k)
+      if (want[k]) PSX_REQUIRE(Mh[k] != nullptr, PSX_ERR_ARGUMENT, "null output pointer");
+    int st = psx_mcm_set_d(h, lmin, lmax, LW, dWp[0], dWp[1], dWp[2], dWp[3], dMp[0],
+                           dMp[1], dMp[2], dMp[3], dMp[4], n);
+    if (st != PSX_OK) throw Error{st, c->last_error};
+    for (int k = 0; k < 5; ++k)
+      if (want[k]) d2h_block(c, Mh[k], ldM, dMp[k], n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int psx_cov_d(psx_ctx* h, int kind, int lmin, int lmax, const double* const* dspectra,
+              const double* const* dratios, const double* const* dW, int64_t LW,
+              int precision, double* dC, int64_t ldC) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldC);
+    int nW = 0, nS = 0, nR = 0;
+    PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
+                "unknown covariance kind");
+    PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED, PSX_ERR_ARGUMENT,
+                "precision must be PSX_FP64 or PSX_MIXED");
+    PSX_REQUIRE(precision == PSX_FP64, PSX_ERR_ARGUMENT,
+                "PSX_MIXED covariance contraction is not available in this build");
+    PSX_REQUIRE(dW && dC && dspectra, PSX_ERR_ARGUMENT, "null pointer");
+    for (int i = 0; i < nW; ++i) PSX_REQUIRE(dW[i], PSX_ERR_ARGUMENT, "null W spectrum");
+    for (int i = 0; i < nS; ++i) PSX_REQUIRE(dspectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+    for (int i = 0; i < nR; ++i)
+      PSX_REQUIRE(dratios && dratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+    PSX_HIP(hipSetDevice(c->device));
+    ContractRequest rq;
+    rq.lmin = lmin;
+    rq.lmax = lmax;
+    rq.LW = LW;
+    cov_layout(kind, rq, dW);
+    rq.epi = EPI_COV;
+    rq.cov_kind = kind;
+    rq.dout[0] = dC;
+    rq.ldo = ldC;
+    for (int i = 0; i < 4; ++i) {
+      rq.dspec[i] = i < nS ? dspectra[i] : nullptr;
+      rq.dratio[i] = i < nR ? dratios[i] : nullptr;
+    }
+    rq.precision = precision;
+    run_contraction(c, rq);
+  });
+}
+
+int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spectra,
+            const double* const* ratios, const double* const* W, int64_t LW,
+            int precision, double* Cm, int64_t ldC) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldC);
+    int nW = 0, nS = 0, nR = 0;
+    PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
+                "unknown covariance kind");
+    PSX_REQUIRE(W && Cm && spectra, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    const size_t sb = sizeof(double) * (lmax + 1), wb = sizeof(double) * LW;
+    DevBuf buf(c, (nS + nR) * sb + nW * wb), dC(c, sizeof(double) * n * (size_t)n);
+    const double* ds[4] = {nullptr, nullptr, nullptr, nullptr};
+    const double* dr[4] = {nullptr, nullptr, nullptr, nullptr};
+    const double* dw[8] = {nullptr};
+    char* p = (char*)buf.p;
+    for (int i = 0; i < nS; ++i) {
+      PSX_REQUIRE(spectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+      h2d(c, p, spectra[i], sb);
+      ds[i] = (const double*)p;
+      p += sb;
+    }
+    for (int i = 0; i < nR; ++i) {
+      PSX_REQUIRE(ratios && ratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+      h2d(c, p, ratios[i], sb);
+      dr[i] = (const double*)p;
+      p += sb;
+    }
+    for (int i = 0; i < nW; ++i) {
+      PSX_REQUIRE(W[i], PSX_ERR_ARGUMENT, "null W spectrum");
+      h2d(c, p, W[i], wb);
+      dw[i] = (const double*)p;
+      p += wb;
+    }
+    int st = psx_cov_d(h, kind, lmin, lmax, ds, dr, dw, LW, precision, dC.d(), n);
+    if (st != PSX_OK) throw Error{st, c->last_error};
+    d2h_block(c, Cm, ldC, dC.d(), n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

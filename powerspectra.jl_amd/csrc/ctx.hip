@@ -1,0 +1,240 @@
+// Context, device memory, kernel timing and the Wigner tables.
+#include "psx_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace psx {
+
+void* ctx_alloc(Ctx* c, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  PSX_REQUIRE(c->in_use + bytes <= c->budget, PSX_ERR_BUDGET,
+              "allocation of " + std::to_string(bytes) +
+                  " bytes exceeds the context HBM budget");
+  void* p = nullptr;
+  PSX_HIP(hipMalloc(&p, bytes));
+  c->allocations[p] = bytes;
+  c->in_use += bytes;
+  return p;
+}
+
+void ctx_free(Ctx* c, void* p) {
+  if (!p) return;
+  auto it = c->allocations.find(p);
+  PSX_REQUIRE(it != c->allocations.end(), PSX_ERR_ARGUMENT,
+              "pointer not owned by this context");
+  c->in_use -= it->second;
+  c->allocations.erase(it);
+  PSX_HIP(hipFree(p));
+}
+
+void* ctx_scratch(Ctx* c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return c->scratch;
+  if (c->scratch) {
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    ctx_free(c, c->scratch);
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+  }
+  size_t want = bytes + bytes / 4;
+  c->scratch = ctx_alloc(c, want);
+  c->scratch_bytes = want;
+  return c->scratch;
+}
+
+TimedLaunch::TimedLaunch(Ctx* ctx, const char* kname) : c(ctx), name(kname) {
+  if (c->timing) {
+    PSX_HIP(hipEventCreate(&e0));
+    PSX_HIP(hipEventCreate(&e1));
+    PSX_HIP(hipEventRecord(e0, c->stream));
+  }
+}
+
+TimedLaunch::~TimedLaunch() noexcept(false) {
+  hipError_t le = hipGetLastError();
+  if (e0) {
+    hipEventRecord(e1, c->stream);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    auto& t = c->timers[name];
+    t.total_ms += ms;
+    t.launches += 1;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  if (le != hipSuccess)
+    throw Error{PSX_ERR_DEVICE, std::string("launch of ") + name + ": " +
+                                    hipGetErrorString(le)};
+}
+
+// gamma_m = prod_{k<=m} (2k-1)/(2k) accumulated in long double (64-bit
+// mantissa); the rounding to double of each entry is then the dominant error.
+void ensure_tables(Ctx* c, int nmax_index) {
+  Tables& t = c->tables;
+  if (t.n >= nmax_index + 2 * kTabOff) return;
+  int n = nmax_index + 2 * kTabOff;
+  n = (n + 1023) & ~1023;
+  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0), T(n, 0.0);
+  long double g = 1.0L;
+  for (int m = 0; m + kTabOff < n; ++m) {
+    if (m > 0) g = g * (long double)(2 * m - 1) / (long double)(2 * m);
+    int i = m + kTabOff;
+    G[i] = (double)g;
+    Z[i] = (double)((long double)(2 * m + 1) * g);
+    D[i] = (double)(1.0L / ((long double)(2 * m + 1) * g));
+    IG[i] = (double)(1.0L / g);
+  }
+  for (int i = 0; i < n; ++i) {
+    long l = (long)i - kTabOff;
+    T[i] = (double)(l * (l + 1));
+  }
+  PSX_HIP(hipStreamSynchronize(c->stream));
+  if (t.dG) {
+    ctx_free(c, t.dG);
+    ctx_free(c, t.dZ);
+    ctx_free(c, t.dD);
+    ctx_free(c, t.dIG);
+    ctx_free(c, t.dT);
+  }
+  size_t b = sizeof(double) * n;
+  t.dG = (double*)ctx_alloc(c, b);
+  t.dZ = (double*)ctx_alloc(c, b);
+  t.dD = (double*)ctx_alloc(c, b);
+  t.dIG = (double*)ctx_alloc(c, b);
+  t.dT = (double*)ctx_alloc(c, b);
+  PSX_HIP(hipMemcpy(t.dG, G.data(), b, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(t.dZ, Z.data(), b, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(t.dD, D.data(), b, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(t.dIG, IG.data(), b, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(t.dT, T.data(), b, hipMemcpyHostToDevice));
+  t.n = n;
+}
+
+}  // namespace psx
+
+using namespace psx;
+
+extern "C" {
+
+int psx_ctx_create(int device, size_t hbm_budget, psx_ctx** out) {
+  if (!out) return PSX_ERR_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PSX_ERR_DEVICE;
+  if (device < 0 || device >= ndev) return PSX_ERR_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return PSX_ERR_DEVICE;
+  Ctx* c = new Ctx();
+  c->device = device;
+  size_t freeb = 0, total = 0;
+  if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+    delete c;
+    return PSX_ERR_DEVICE;
+  }
+  c->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)freeb);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PSX_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    c->cu_count = prop.multiProcessorCount;
+  *out = reinterpret_cast<psx_ctx*>(c);
+  return PSX_OK;
+}
+
+int psx_ctx_destroy(psx_ctx* h) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return PSX_ERR_ARGUMENT;
+  {
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->allocations) hipFree(kv.first);
+    c->allocations.clear();
+    release_blas(c);
+    hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return PSX_OK;
+}
+
+int psx_ctx_synchronize(psx_ctx* h) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    hipSetDevice(c->device);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+const char* psx_ctx_last_error(psx_ctx* h) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return "null context";
+  return c->last_error.c_str();
+}
+
+size_t psx_ctx_bytes_in_use(psx_ctx* h) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return c ? c->in_use : 0;
+}
+
+int psx_ctx_set_timing(psx_ctx* h, int enable) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] { c->timing = enable != 0; });
+}
+
+int psx_ctx_kernel_time(psx_ctx* h, const char* name, int reset,
+                        double* total_ms, int64_t* launches) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(name != nullptr, PSX_ERR_ARGUMENT, "null kernel name");
+    auto it = c->timers.find(name);
+    double ms = 0;
+    int64_t n = 0;
+    if (it != c->timers.end()) {
+      ms = it->second.total_ms;
+      n = it->second.launches;
+      if (reset) c->timers.erase(it);
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = n;
+  });
+}
+
+int psx_dev_alloc(psx_ctx* h, size_t bytes, void** dptr) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(dptr != nullptr, PSX_ERR_ARGUMENT, "null output pointer");
+    hipSetDevice(c->device);
+    *dptr = ctx_alloc(c, bytes);
+  });
+}
+
+int psx_dev_free(psx_ctx* h, void* dptr) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    hipSetDevice(c->device);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    ctx_free(c, dptr);
+  });
+}
+
+int psx_memcpy_h2d(psx_ctx* h, void* dst, const void* src, size_t bytes) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    hipSetDevice(c->device);
+    PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int psx_memcpy_d2h(psx_ctx* h, void* dst, const void* src, size_t bytes) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    hipSetDevice(c->device);
+    PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Internal declarations of the gfx950 engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/powerspectra_hip.h"
+
+namespace psx {
+
+// ---- error plumbing ------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define PSX_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      throw ::psx::Error{PSX_ERR_DEVICE, std::string(#call) + ": " +        \
+                                             hipGetErrorString(e_)};         \
+  } while (0)
+
+#define PSX_REQUIRE(cond, code, text)                 \
+  do {                                                \
+    if (!(cond)) throw ::psx::Error{(code), (text)}; \
+  } while (0)
+
+// ---- Wigner tables -------------------------------------------------------
+// Host-built (long double) tables of the central-binomial ratio
+//   gamma_m = C(2m, m) / 4^m = prod_{k=1..m} (2k-1)/(2k)
+// and the derived tables the closed-form 3j products need:
+//   G[m]  = gamma_m                     (0 for m < 0)
+//   Z[m]  = (2m+1) gamma_m              (0 for m < 0)
+//   D[g]  = 1 / ((2g+1) gamma_g)
+//   IG[g] = 1 / gamma_g
+// Index m is stored at offset m + kTabOff.
+constexpr int kTabOff = 256;
+
+struct Tables {
+  int n = 0;            // entries per table (indices -kTabOff .. n-kTabOff-1)
+  double* dG = nullptr;
+  double* dZ = nullptr;
+  double* dD = nullptr;
+  double* dIG = nullptr;
+  double* dT = nullptr;  // T[l] = l (l+1) for l >= -kTabOff (exact)
+};
+
+// ---- context -------------------------------------------------------------
+struct KernelTimer {
+  double total_ms = 0;
+  int64_t launches = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  size_t budget = 0;
+  size_t in_use = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  std::string last_error;
+  bool timing = false;
+  std::map<std::string, KernelTimer> timers;
+  std::map<void*, size_t> allocations;
+  Tables tables;
+  int cu_count = 0;
+  // reusable device scratch (grown on demand)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // rocSOLVER/rocBLAS handle (created lazily)
+  void* blas = nullptr;
+};
+
+void* ctx_alloc(Ctx* c, size_t bytes);
+void ctx_free(Ctx* c, void* p);
+void* ctx_scratch(Ctx* c, size_t bytes);
+void ensure_tables(Ctx* c, int nmax_index);
+void release_blas(Ctx* c);
+
+// Launch helper: records HIP events around the launch when timing is on.
+struct TimedLaunch {
+  Ctx* c;
+  const char* name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TimedLaunch(Ctx* ctx, const char* kname);
+  ~TimedLaunch() noexcept(false);
+};
+
+// ---- contraction engine --------------------------------------------------
+// Family products a V table is contracted against (closed forms, see
+// contract.hip):
+enum Family : int {
+  FAM_00 = 0,  // (l3 l1 l2; 0 0 0)^2                          even l1+l2+l3
+  FAM_02 = 1,  // (l3 l1 l2; 0 0 0)(l3 l1 l2; 0 -2 2)          even
+  FAM_PE = 2,  // (l3 l1 l2; 0 -2 2)^2                         even
+  FAM_PO = 3   // (l3 l1 l2; 0 -2 2)^2                         odd
+};
+
+enum EpiMode : int {
+  EPI_MCM = 0,   // out_k[l1,l2] = (2l2+1) kappa Ξ_k, out_k[l2,l1] = (2l1+1) kappa Ξ_k
+  EPI_COV = 1    // out[l1,l2] = out[l2,l1] = sum_k coef_k(l1,l2) Ξ_k
+};
+
+struct ContractRequest {
+  int lmin = 0, lmax = 0;
+  int64_t LW = 0;
+  // V tables in family order: all FAM_00, then FAM_02, FAM_PE, FAM_PO.
+  int nfam[4] = {0, 0, 0, 0};
+  const double* dW[16];      // source W (device) for each V table, in order
+  EpiMode epi = EPI_MCM;
+  // EPI_MCM: one output per V table, ld = ldo
+  double* dout[16];
+  int64_t ldo = 0;
+  // EPI_COV
+  int cov_kind = 0;
+  const double* dspec[4];
+  const double* dratio[4];
+  int precision = PSX_FP64;
+};
+
+void run_contraction(Ctx* c, const ContractRequest& rq);
+void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
+                        void* scratch, size_t scratch_bytes);
+size_t contraction_scratch_bytes(const ContractRequest& rq);
+
+// ---- linear algebra ------------------------------------------------------
+void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB,
+                  int64_t ldb, int nrhs);  // dA overwritten by LU
+void decouple_dev(Ctx* c, int n, const double* dY, double* dB1, double* dB2,
+                  double* dOut);
+
+// status wrapper used by every extern "C" entry point
+template <class F>
+int guarded(Ctx* c, F&& f) {
+  if (!c) return PSX_ERR_ARGUMENT;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  try {
+    f();
+    return PSX_OK;
+  } catch (const Error& e) {
+    c->last_error = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    c->last_error = e.what();
+    return PSX_ERR_DEVICE;
+  }
+}
+
+}  // namespace psx

@@ -1,0 +1,68 @@
+"""GPU parity: FP64 LU decoupling (rocSOLVER) through the C ABI.
+
+Reference: decouple_covmat (src/covariance.jl:8-14) and `\\`
+(src/blockspectralmatrix.jl:101-129).
+"""
+import numpy as np
+import pytest
+from helpers import mask_like_W, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_decouple_reference_kat(ctx):
+    """test/test_covmat.jl:11-17 of the reference, matrices inline there."""
+    A = np.array([[1.0, 0.2, 0.3], [0.4, 2.0, 0.15], [0.3, 0.1, 1.44]])
+    B1 = np.array([[1.2, 0.6, 0.1], [0.3, 1.4, 0.5], [0.44, 0.2, 1.3]])
+    B2 = np.array([[1.6, 0.4, 0.1], [0.3, 1.4, 0.9], [0.45, 0.8, 1.7]])
+    Cref = np.linalg.inv(B1) @ A @ np.linalg.inv(B2).T
+    C = ctx.decouple_covmat(A, B1, B2)
+    assert np.allclose(C, Cref, rtol=1.5e-8, atol=0)
+    assert rel_err(C, Cref) < 1e-13
+
+
+def test_decouple_matches_oracle_and_numpy(ctx, oracle):
+    rng = np.random.default_rng(2)
+    n = 400
+    Y = rng.standard_normal((n, n))
+    Y = Y @ Y.T
+    B1 = np.eye(n) + 0.1 * rng.standard_normal((n, n))
+    B2 = np.eye(n) + 0.1 * rng.standard_normal((n, n))
+    C = ctx.decouple_covmat(Y, B1, B2)
+    R = oracle.decouple_covmat(Y, B1, B2)
+    N = np.linalg.solve(B1, Y) @ np.linalg.inv(B2).T
+    assert np.linalg.norm(C - R) / np.linalg.norm(R) < 1e-11
+    assert np.linalg.norm(C - N) / np.linalg.norm(N) < 1e-11
+
+
+def test_decouple_invariant_with_gpu_mcm(ctx):
+    """test/test_covmat.jl:115-127: C_coupled = M C_dec M^T for a TT MCM M."""
+    lmax = 300
+    M = ctx.mcm(0, 0, lmax, mask_like_W(lmax + 1, seed=9))
+    rng = np.random.default_rng(0)
+    X = rng.uniform(size=M.shape)
+    Cc = X.T @ X
+    Cd = ctx.decouple_covmat(Cc, M, M)
+    back = M @ Cd @ M.T
+    assert np.linalg.norm(back - Cc) / np.linalg.norm(Cc) < 1e-9
+
+
+def test_lu_solve(ctx, oracle):
+    rng = np.random.default_rng(5)
+    n = 513
+    A = np.eye(n) * 3 + rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    x = ctx.lu_solve(A, b)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-12
+    B = rng.standard_normal((n, 3))
+    X = ctx.lu_solve(A, B)
+    R = oracle.lu_solve(A, B)
+    assert np.linalg.norm(X - R) / np.linalg.norm(R) < 1e-11
+
+
+def test_singular(ctx, pkg):
+    A = np.ones((4, 4))
+    with pytest.raises(pkg.SingularException):
+        ctx.lu_solve(A, np.ones(4))
+    with pytest.raises(pkg.SingularException):
+        ctx.decouple_covmat(np.eye(4), np.zeros((4, 4)), np.eye(4))
