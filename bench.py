@@ -1,0 +1,232 @@
+"""Benchmark: full polarised mode-coupling set (TT, TE, ET, ++, --) at lmax 3071.
+
+One step = one pass of the hot path over one batch: the five mode-coupling
+blocks the reference's maskedalm2spectra computes for a pair of polarised
+maps (src/modecoupling.jl:348-377; inner loops :78-159), from four mask
+cross-spectra W (TT, TE, ET, PP) of length lmax+1 (the reference truncation,
+src/modecoupling.jl:197), into five 3072 x 3072 FP64 matrices.  Inputs and
+outputs are resident in HBM; the timed region is the device work only.
+
+Prints ONE JSON line (see README for the fields).  `roofline` is for the
+dominant kernel (psx_mcm_set), timed with HIP events on the stream it runs on;
+`cpu_baseline` times the CPU oracle (a C restatement of the reference
+algorithm, OpenMP over l1 rows) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "MCM elements/s at lmax 3071 (FP64 vs mixed, ≤1e-9 rel); req/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# FP64 flops the closed-form contraction spends per (l1, l2, c) step, FMA = 2
+# ("Roofline"): set = p 1, TT 2, E/Q 3, TE 1+2, ET 2, ++ 1+2,
+# -- 1+1+1+1+2.
+FLOPS_PER_STEP = {"psx_mcm_set": 20, "psx_mcm_00": 3}
+
+
+def csteps(lmin: int, lmax: int, LW: int) -> tuple[int, int]:
+    """Number of (l1 <= l2, c) steps with an even (resp. odd) l3 <= LW-1.
+
+    Even: l3 = l1+l2-2c, 0 <= c <= min(l1, l2).  Odd: l3 = l1+l2-2c-1,
+    0 <= c <= min(l1, l2) - 1.  The even count is the survey's T_con(even)."""
+    ev = od = 0
+    for l1 in range(lmin, lmax + 1):
+        l2 = np.arange(l1, lmax + 1, dtype=np.int64)
+        S = l1 + l2
+        lo_e = np.maximum(0, (S - LW + 2) // 2)      # ceil((S-LW+1)/2)
+        lo_o = np.maximum(0, (S - LW + 1) // 2)      # ceil((S-LW)/2)
+        ev += int(np.maximum(0, l1 - lo_e + 1).sum())
+        od += int(np.maximum(0, l1 - 1 - lo_o + 1).sum())
+    return ev, od
+
+
+def synthetic_W(LW: int, seed: int, cross: bool) -> np.ndarray:
+    """Stand-in mask cross-spectrum of length LW with the shape of an
+    apodised galactic-cut + holes mask: W_0 ~ 4 pi f_sky, a fall-off of more
+    than ten decades and ripples; cross spectra change sign."""
+    rng = np.random.default_rng(seed)
+    ell = np.arange(LW, dtype=np.float64)
+    W = 4 * np.pi * 0.55 * np.exp(-ell / 40.0) * (1 + 0.2 * np.cos(ell / 7.0))
+    W += 4 * np.pi * 1e-5 * np.exp(-ell / 600.0) * rng.uniform(0.5, 1.5, LW)
+    W += 4 * np.pi * 1e-9 * rng.uniform(0.0, 1.0, LW)
+    if cross:
+        W *= np.where(rng.uniform(size=LW) < 0.15, -0.3, 1.0)
+    return W
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        line = line.strip()
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v)
+            except ValueError:
+                pass
+    return vals.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(lmax: int, LW: int, Ws, stride: int, nthreads: int):
+    """Time the CPU oracle (reference-algorithm restatement) on rows
+    l1 = 0, stride, 2 stride, ... of the five blocks; the systematic row
+    sample scales the time of the full job by `stride`."""
+    from oracle import oracle as O
+
+    W_TT, W_TE, W_ET, W_PP = Ws
+    jobs = [(O.MCM_00, W_TT), (O.MCM_02, W_TE), (O.MCM_02, W_ET), (O.MCM_PP, W_PP),
+            (O.MCM_MM, W_PP)]
+    n = lmax + 1
+    M = np.full((n, n), np.nan, order="F")
+    rows = list(range(0, lmax + 1, stride))
+    # warm the library / thread pool
+    O.mcm_rows(0, 0, lmax, lmax, lmax, W_TT, nthreads=nthreads, out=M)
+    t0 = time.perf_counter()
+    for kind, W in jobs:
+        for r in rows:
+            O.mcm_rows(kind, 0, lmax, r, r, W, nthreads=nthreads, out=M)
+    dt = time.perf_counter() - t0
+    full_time = dt * stride
+    elements = 5 * n * n
+    return {
+        "value": elements / full_time,
+        "unit": "elements/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": (f"rows l1 = 0 mod {stride} ({len(rows)} of {n}) of all 5 blocks (TT, TE, ET, ++, --), "
+                   f"{dt:.1f} s measured, full-job time = {stride} x sample"),
+        "sample_seconds": round(dt, 3),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--lmax", type=int, default=3071)
+    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(see DESIGN.md, 'More than one GPU')", file=sys.stderr)
+        return 2
+
+    from __graft_entry__ import load_package
+
+    pkg = load_package()
+    abi = sys.modules[pkg.__name__ + "._abi"]
+    ctx = pkg.Context(0)
+    lmax = args.lmax
+    LW = lmax + 1
+    n = lmax + 1
+    Ws = [synthetic_W(LW, s, cross) for s, cross in ((1, False), (2, True), (3, True), (4, False))]
+
+    # resident inputs/outputs in HBM
+    dW = [ctx.alloc(8 * LW) for _ in range(4)]
+    for d, w in zip(dW, Ws):
+        ctx.h2d(d, w)
+    dM = [ctx.alloc(8 * n * n) for _ in range(5)]
+    L = ctx.lib
+
+    def step():
+        ctx.check(L.psx_mcm_set_d(ctx.handle, 0, lmax, LW, *dW, *dM, n), "psx_mcm_set_d")
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.kernel_time("psx_mcm_set", reset=True)
+    ctx.kernel_time("psx_vtable", reset=True)
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    k_ms, k_n = ctx.kernel_time("psx_mcm_set", reset=True)
+    v_ms, v_n = ctx.kernel_time("psx_vtable", reset=True)
+
+    dt = t1 - t0
+    elements = 5 * n * n
+    value = elements * args.steps / dt
+    ev, od = csteps(0, lmax, LW)
+    flops = FLOPS_PER_STEP["psx_mcm_set"] * ev
+    k_avg = k_ms / max(k_n, 1)
+    achieved_tflops = flops / (k_avg * 1e-3) / 1e12
+    # survey convention v1 (prices a per-term recurrence this design does not run)
+    t_fam_even = None
+    traffic = read_traffic("psx_mcm_set_lmax3071")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded analytic stand-ins for the four mask cross-spectra "
+                "(length lmax+1, >10 decades of fall-off, sign changes for the cross spectra)",
+        "config": {
+            "workload": "mcm_set_lmax3071: TT, TE, ET, ++, -- mode-coupling blocks, lmax 3071, "
+                        "W length lmax+1, 5 x 3072^2 FP64 outputs per step",
+            "lmax": lmax,
+            "LW": LW,
+            "blocks": ["TT", "TE", "ET", "++", "--"],
+            "elements_per_step": elements,
+            "precision": "fp64",
+        },
+        "kernel_ms": {"psx_mcm_set": k_avg, "psx_vtable": v_ms / max(v_n, 1)},
+        "roofline": {
+            "bound": "fp64-valu",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "psx_mcm_set",
+            "flops_per_launch": flops,
+            "units_per_launch": ev,
+            "unit_def": "(l1<=l2, c) steps with even l3 <= LW-1; 20 FP64 flops each",
+        },
+        "mixed": None,
+    }
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(lmax, LW, Ws, args.cpu_stride, nthreads)
+    else:
+        out["cpu_baseline"] = None
+    for d in dW + dM:
+        ctx.free(d)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

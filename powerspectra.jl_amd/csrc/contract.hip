@@ -18,21 +18,29 @@
 //        z_m = (2m+1) gamma_m,  D1 = P1 + P2 - l3(l3+1) - 2
 //
 // With S = l1 + l2 and the summation index c (l3 = S - 2c even, S - 2c - 1
-// odd) every term factorises into a lane factor gamma_{l1-c} gamma_{l2-c}
-// (or z_{l1-c-1} z_{l2-c-1}) and a factor that depends on (S, c) only:
-//   V_even[c][l3] = What_{l3} gamma_c / ((2(l3+c)+1) gamma_{l3+c})
-//   V_odd [c][l3] = What_{l3} z_c / gamma_{l3+c+1},   What = (2l3+1) W / (4 pi)
-// V is precomputed once per W spectrum (one HBM pass); the main kernel then
-// spends 2 FP64 VALU ops per term (mul + fma) for (0 0 0)^2 and no sqrt, no
-// division and no recurrence.  All integer-valued polynomials (P, Q, D0, D1)
-// are exact in FP64 for lmax <= ~6000.
+// odd; a = l2 - c, b = l1 - c, g = S - c) every term is a product of
+// one-dimensional tables:
+//   even:  gamma_{l1-c} * gamma_c  *  gamma_{l2-c} * delta_{S-c}  *  What_{l3}
+//   odd:   z_{l1-c-1}   * z_c      *  z_{l2-c-1}   * igamma_{S-c} *  What_{l3}
+// with delta_g = 1/((2g+1) gamma_g), igamma_g = 1/gamma_g and
+// What_l = (2l+1) W_l / (4 pi).  The main kernel spends 2 FP64 VALU ops per
+// (0 0 0)^2 term (mul + fma) -- no sqrt, no division, no recurrence.  All
+// integer-valued polynomials (P, Q, D0, D1) are exact in FP64 for
+// lmax <= ~6000.
 //
 // Work decomposition: one wavefront owns a tile of 64 consecutive l1 (lanes)
-// times R consecutive anti-diagonals S (register slots); for a fixed S the
-// lanes walk along an anti-diagonal of the (l1, l2) triangle.  Per step c a
-// lane reads gamma_{l1-c} from LDS and slides a register window over
-// gamma_{l2-c} (one LDS read per step), while V[c][S-2c] is uniform across
-// the wave and is read with scalar loads.
+// times R consecutive anti-diagonals S = S0 .. S0+R-1 (register slots); for a
+// fixed S the lanes walk along an anti-diagonal of the (l1, l2) triangle.
+// Per step c:
+//   * the first factor pair depends on (lane, c) only: one vector load per
+//     step, times the uniform gamma_c;
+//   * the second pair depends on (lane, S - c), i.e. on r - c: it lives in a
+//     register window that slides by one slot per step (one vector load and
+//     one multiply per step, shared by all R slots);
+//   * What_{S-2c} (and l3(l3+1) for the spin-2 polynomials) is uniform across
+//     the wave and comes from small 1D tables through scalar loads.
+// Lane tables are read with vector memory loads (in-order vmcnt), uniform
+// ones with scalar loads, so neither waits on the other.
 #include "psx_internal.h"
 
 #include <algorithm>
@@ -42,9 +50,7 @@ namespace psx {
 
 namespace {
 
-constexpr int kVOff = 192;   // V column of l3 = 0
-constexpr int kTabLo = -176; // first table index staged into LDS
-constexpr int kTabHiPad = 128;
+constexpr int kWOff = 192;   // What index of l3 = 0
 constexpr double kInv4Pi = 0.07957747154594766788444188168625718;  // 1/(4 pi)
 
 struct Tile {
@@ -58,12 +64,15 @@ struct KArgs {
   int ntiles;
   int lmin, lmax;
   int LW;
-  const double* G;   // device tables, index m at [m + kTabOff]
-  const double* Z;
-  const double* T;
-  int tab_n;         // entries staged into LDS (indices kTabLo .. kTabLo+tab_n-1)
-  const double* V[kMaxV];
-  int64_t vpitch;
+  // 1D tables, index m at [m + kTabOff]
+  const double* G;   // gamma_m
+  const double* Z;   // (2m+1) gamma_m
+  const double* D;   // 1 / ((2m+1) gamma_m)
+  const double* IG;  // 1 / gamma_m
+  const double* T;   // m (m+1)
+  const double* T1;  // m (m+1) + 1
+  // What_k[l3 + kWOff] = (2 l3 + 1) W_k[l3] / (4 pi), zero outside [0, LW)
+  const double* Wh[kMaxV];
   double* out[kMaxV];
   int64_t ldo;
   const double* spec[4];
@@ -147,16 +156,7 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   constexpr bool kG = (N00 + N02 + NPE) > 0;
   constexpr bool kZ = NPO > 0;
   constexpr bool kE = (N02 + NPE) > 0;
-  extern __shared__ double lds[];
-  double* sG = lds;
-  double* sZ = lds + (kG ? A.tab_n : 0);
-
-  // Stage the gamma / z tables for indices kTabLo .. kTabLo + tab_n - 1.
-  for (int i = threadIdx.x; i < A.tab_n; i += blockDim.x) {
-    if (kG) sG[i] = A.G[i + kTabLo + kTabOff];
-    if (kZ) sZ[i] = A.Z[i + kTabLo + kTabOff];
-  }
-  __syncthreads();
+  constexpr bool kS = kE || kZ;  // spin-2 polynomials needed
 
   const int lane = threadIdx.x & 63;
   const int wave = ufl(threadIdx.x >> 6);
@@ -166,6 +166,8 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   const int S0 = sload(&A.tiles[tile].S0, 0);
   const int lmax = A.lmax;
   const int l1 = x0 + lane;
+  const double* G = A.G + kTabOff;
+  const double* Z = A.Z + kTabOff;
 
   // summation range over c, rounded out to whole blocks of R
   const int chi = min(x0 + 63, lmax);
@@ -173,17 +175,15 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   const int cbeg = ufl((clo / R) * R);
   const int nblk = ufl(max(0, (chi + 1 - cbeg + R - 1) / R));
 
-  // lane constants per slot
-  double sig1[kE ? R : 1], kap[kE ? R : 1], sig2[kZ ? R : 1];
+  // lane constants per slot: E = sig[r] - T(l3), Q = E^2 - kap[r],
+  // D1 = sig[r] - T1(l3 - 1)
+  double sig[kS ? R : 1], kap[kE ? R : 1];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const double L1 = (double)l1, L2 = (double)(S0 + r - l1);
     const double P1 = L1 * (L1 + 1.0), P2 = L2 * (L2 + 1.0);
-    if constexpr (kE) {
-      sig1[r] = P1 + P2 - 1.0;
-      kap[r] = 1.0 + 2.0 * P1 * P2;
-    }
-    if constexpr (kZ) sig2[r] = P1 + P2 - 2.0;
+    if constexpr (kS) sig[r] = P1 + P2 - 1.0;
+    if constexpr (kE) kap[r] = 1.0 + 2.0 * P1 * P2;
   }
 
   double acc[NV][R];
@@ -192,69 +192,69 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[k][r] = 0.0;
 
-  // sliding windows: slot r at step c needs G[t + r - c], Z[t + r - c - 1]
-  // with t = S0 - x0 - lane; held at gw[(r - c) mod R] (cbeg % R == 0).
+  // sliding windows: slot r at step c needs (q = r - c)
+  //   gw: gamma_{t+q} delta_{S0+q},  zw: z_{t+q-1} igamma_{S0+q},
+  // t = S0 - x0 - lane, held at index q mod R (cbeg % R == 0).
   const int t = S0 - x0 - lane;
   double gw[kG ? R : 1], zw[kZ ? R : 1];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    if constexpr (kG) gw[r] = sG[t + r - cbeg - kTabLo];
-    if constexpr (kZ) zw[r] = sZ[t + r - cbeg - 1 - kTabLo];
+    const int q = r - cbeg;
+    if constexpr (kG) gw[r] = G[t + q] * sload(A.D, (int64_t)kTabOff + S0 + q);
+    if constexpr (kZ) zw[r] = Z[t + q - 1] * sload(A.IG, (int64_t)kTabOff + S0 + q);
   }
 
-  const int64_t P = A.vpitch;
   for (int blk = 0; blk < nblk; ++blk) {
     const int c0 = cbeg + blk * R;
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const int c = c0 + u;
       double g1 = 0.0, z1 = 0.0;
-      if constexpr (kG) g1 = sG[l1 - c - kTabLo];
-      if constexpr (kZ) z1 = sZ[l1 - c - 1 - kTabLo];
-      // uniform offsets of V row c at column l3 = S0 - 2c (+r)
-      const int64_t voff = (int64_t)c * P + kVOff + S0 - 2 * c;
+      if constexpr (kG) g1 = G[l1 - c] * sload(A.G, (int64_t)kTabOff + c);
+      if constexpr (kZ) z1 = Z[l1 - c - 1] * sload(A.Z, (int64_t)kTabOff + c);
+      const int64_t w0 = (int64_t)kWOff + S0 - 2 * c;      // What index of r = 0
+      const int64_t t0 = (int64_t)kTabOff + S0 - 2 * c;    // T index of r = 0
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int wi = ((r - u) % R + R) % R;
         double p = 0.0, Q = 0.0;
         if constexpr (kG) p = g1 * gw[wi];
         if constexpr (kE) {
-          const double Te = sload(A.T, (int64_t)kTabOff + S0 + r - 2 * c);
-          const double E = sig1[r] - Te;
+          const double E = sig[r] - sload(A.T, t0 + r);
           Q = fma(E, E, -kap[r]);
         }
 #pragma unroll
         for (int k = 0; k < N00; ++k)
-          acc[k][r] = fma(sload(A.V[k], voff + r), p, acc[k][r]);
+          acc[k][r] = fma(sload(A.Wh[k], w0 + r), p, acc[k][r]);
         double pq = 0.0;
         if constexpr (N02 > 0) {
           pq = p * Q;
 #pragma unroll
           for (int k = 0; k < N02; ++k)
-            acc[N00 + k][r] = fma(sload(A.V[N00 + k], voff + r), pq, acc[N00 + k][r]);
+            acc[N00 + k][r] = fma(sload(A.Wh[N00 + k], w0 + r), pq, acc[N00 + k][r]);
         }
         if constexpr (NPE > 0) {
           const double pqq = (N02 > 0 ? pq : p * Q) * Q;
 #pragma unroll
           for (int k = 0; k < NPE; ++k)
             acc[N00 + N02 + k][r] =
-                fma(sload(A.V[N00 + N02 + k], voff + r), pqq, acc[N00 + N02 + k][r]);
+                fma(sload(A.Wh[N00 + N02 + k], w0 + r), pqq, acc[N00 + N02 + k][r]);
         }
         if constexpr (NPO > 0) {
-          const double To = sload(A.T, (int64_t)kTabOff + S0 + r - 2 * c - 1);
-          const double D1 = sig2[r] - To;
+          const double D1 = sig[r] - sload(A.T1, t0 - 1 + r);
           const double pd = (z1 * zw[wi]) * (D1 * D1);
 #pragma unroll
           for (int k = 0; k < NPO; ++k)
-            acc[N00 + N02 + NPE + k][r] = fma(sload(A.V[N00 + N02 + NPE + k], voff - 1 + r),
+            acc[N00 + N02 + NPE + k][r] = fma(sload(A.Wh[N00 + N02 + NPE + k], w0 - 1 + r),
                                               pd, acc[N00 + N02 + NPE + k][r]);
         }
       }
-      // advance the windows to step c + 1: the slot that held r = R-1
-      // receives G[t - c - 1]
+      // advance the windows to step c + 1: the slot that held r = R-1 now
+      // receives q = -c - 1
       const int ni = (R - 1 - u) % R;
-      if constexpr (kG) gw[ni] = sG[t - c - 1 - kTabLo];
-      if constexpr (kZ) zw[ni] = sZ[t - c - 2 - kTabLo];
+      const int qn = -c - 1;
+      if constexpr (kG) gw[ni] = G[t + qn] * sload(A.D, (int64_t)kTabOff + S0 + qn);
+      if constexpr (kZ) zw[ni] = Z[t + qn - 1] * sload(A.IG, (int64_t)kTabOff + S0 + qn);
     }
   }
 
@@ -297,34 +297,22 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   }
 }
 
-// ---- V table construction ------------------------------------------------
-struct VArgs {
-  double* V[kMaxV];
+// ---- What tables: (2 l3 + 1) W[l3] / (4 pi), zero-padded ------------------
+struct WhArgs {
+  double* Wh[kMaxV];
   const double* W[kMaxV];
-  int odd[kMaxV];
-  int64_t pitch;
+  int len;  // entries per table (index l3 + kWOff)
   int LW;
-  const double* G;
-  const double* Z;
-  const double* D;
-  const double* IG;
 };
 
-__global__ void __launch_bounds__(256) vtable_kernel(VArgs a) {
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.y;
-  const int k = blockIdx.z;
-  if (col >= a.pitch) return;
-  const int l3 = (int)col - kVOff;
+__global__ void __launch_bounds__(256) what_kernel(WhArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= a.len) return;
+  const int l3 = i - kWOff;
   double v = 0.0;
-  if (l3 >= 0 && l3 < a.LW) {
-    const double wh = (2.0 * l3 + 1.0) * a.W[k][l3] * kInv4Pi;
-    if (a.odd[k])
-      v = wh * a.Z[c + kTabOff] * a.IG[l3 + c + 1 + kTabOff];
-    else
-      v = wh * a.G[c + kTabOff] * a.D[l3 + c + kTabOff];
-  }
-  a.V[k][(int64_t)c * a.pitch + col] = v;
+  if (l3 >= 0 && l3 < a.LW) v = (2.0 * l3 + 1.0) * a.W[k][l3] * kInv4Pi;
+  a.Wh[k][i] = v;
 }
 
 // ---- tile lists ----------------------------------------------------------
@@ -361,19 +349,19 @@ std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R) {
 
 struct Geometry {
   int R;
-  int64_t pitch;  // V row pitch (doubles)
-  int nrows;      // V rows
-  int tab_n;      // LDS table entries
-  int tab_need;   // largest table index any read touches
+  int wh_len;     // entries per What table
+  int tab_need;   // largest 1D table index any read touches
 };
 
 // Index ranges touched by contract_kernel for this tile set, checked against
-// the buffers before launch (see the access list in contract_kernel).
+// the buffers before launch (the access list follows the kernel: lane reads
+// of G/Z at l1-c(-1) and t+q(-1), uniform reads of D/IG at S0+q, of G/Z at c,
+// of What/T/T1 at l3 = S0+r-2c(-1)).
 Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R,
                         int tables_n) {
   Geometry g{};
   g.R = R;
-  int tmin = 0, tmax = 0, vmin = 0, vmax = 0, rmax = 0, Tmin = 0, Tmax = 0;
+  long lo = 0, hi = 0, l3lo = 0, l3hi = 0;
   bool first = true;
   for (const Tile& t : tiles) {
     const int chi = std::min(t.x0 + 63, lmax);
@@ -381,34 +369,27 @@ Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R,
     const int cbeg = (clo / R) * R;
     const int nblk = std::max(0, (chi + 1 - cbeg + R - 1) / R);
     const int cend = cbeg + nblk * R;  // exclusive
-    // LDS table reads (G and Z, incl. the -1 / -2 shifts; window start and
-    // the loads for steps cbeg .. cend)
-    const int lo = std::min({t.x0 - cend, t.S0 - t.x0 - 63 - cend - 1,
-                             t.S0 - t.x0 - 63 - cbeg - 1});
-    const int hi = std::max({t.x0 + 63 - cbeg, t.S0 - t.x0 + R - 1 - cbeg});
-    // V columns (l3) and rows, T table
-    const int l3lo = t.S0 - 2 * (cend - 1) - 1;
-    const int l3hi = t.S0 + R - 1 - 2 * cbeg;
+    const long tmin = (long)t.S0 - t.x0 - 63, tmax = (long)t.S0 - t.x0;
+    const long a = std::min({(long)t.x0 - cend, tmin - cend - 1, tmin - cbeg - 1,
+                             (long)t.S0 - cend, (long)cbeg});
+    const long b = std::max({(long)t.x0 + 63 - cbeg, tmax + R - 1 - cbeg,
+                             (long)t.S0 + R - 1 - cbeg, (long)cend - 1});
+    const long e = (long)t.S0 - 2L * (cend - 1) - 1;
+    const long f = (long)t.S0 + R - 1 - 2L * cbeg;
     if (first) {
-      tmin = lo; tmax = hi; vmin = l3lo; vmax = l3hi; rmax = cend - 1;
+      lo = a; hi = b; l3lo = e; l3hi = f;
       first = false;
     } else {
-      tmin = std::min(tmin, lo); tmax = std::max(tmax, hi);
-      vmin = std::min(vmin, l3lo); vmax = std::max(vmax, l3hi);
-      rmax = std::max(rmax, cend - 1);
+      lo = std::min(lo, a); hi = std::max(hi, b);
+      l3lo = std::min(l3lo, e); l3hi = std::max(l3hi, f);
     }
   }
-  Tmin = vmin;
-  Tmax = vmax;
-  PSX_REQUIRE(tmin >= kTabLo, PSX_ERR_ARGUMENT, "tile geometry: LDS table underflow");
-  g.tab_n = tmax - kTabLo + 1;
-  PSX_REQUIRE(vmin >= -kVOff, PSX_ERR_ARGUMENT, "tile geometry: V column underflow");
-  g.pitch = ((int64_t)kVOff + std::max(vmax, LW) + 1 + 15) & ~(int64_t)15;
-  g.nrows = rmax + 1;
-  PSX_REQUIRE(Tmin >= -kTabOff, PSX_ERR_ARGUMENT, "tile geometry: T table underflow");
-  // V construction reads D/IG up to (l3 + c + 1) with l3 < LW, c < nrows
-  g.tab_need = std::max({Tmax, g.tab_n + kTabLo, LW + g.nrows + 2, tmax});
-  PSX_REQUIRE(g.tab_need + kTabOff < tables_n || tables_n == 0, PSX_ERR_ARGUMENT,
+  PSX_REQUIRE(lo >= -kTabOff && l3lo >= -kTabOff, PSX_ERR_ARGUMENT,
+              "tile geometry: table underflow");
+  PSX_REQUIRE(l3lo >= -kWOff, PSX_ERR_ARGUMENT, "tile geometry: What underflow");
+  g.wh_len = (int)(((long)kWOff + std::max<long>(l3hi + 1, LW) + 15) & ~15L);
+  g.tab_need = (int)std::max(hi, l3hi);
+  PSX_REQUIRE(tables_n == 0 || g.tab_need + kTabOff < tables_n, PSX_ERR_ARGUMENT,
               "tile geometry: table overflow");
   return g;
 }
@@ -454,15 +435,15 @@ Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
   if (rq.epi == EPI_MCM) {
     if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
-      return cfg<16, 1, 0, 0, 0, EPI_MCM, 0>(4, "psx_mcm_00");
+      return cfg<8, 1, 0, 0, 0, EPI_MCM, 0>(4, "psx_mcm_00");
     if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
-      return cfg<8, 0, 1, 0, 0, EPI_MCM, 0>(4, "psx_mcm_02");
+      return cfg<4, 0, 1, 0, 0, EPI_MCM, 0>(4, "psx_mcm_02");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
-      return cfg<8, 0, 0, 1, 0, EPI_MCM, 0>(4, "psx_mcm_pp");
+      return cfg<4, 0, 0, 1, 0, EPI_MCM, 0>(4, "psx_mcm_pp");
     if (f[0] == 0 && f[1] == 0 && f[2] == 0 && f[3] == 1)
-      return cfg<8, 0, 0, 0, 1, EPI_MCM, 0>(4, "psx_mcm_mm");
+      return cfg<4, 0, 0, 0, 1, EPI_MCM, 0>(4, "psx_mcm_mm");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 1)
-      return cfg<8, 0, 0, 1, 1, EPI_MCM, 0>(4, "psx_mcm_pm");
+      return cfg<4, 0, 0, 1, 1, EPI_MCM, 0>(4, "psx_mcm_pm");
     if (f[0] == 1 && f[1] == 2 && f[2] == 1 && f[3] == 1)
       return cfg<4, 1, 2, 1, 1, EPI_MCM, 0>(4, "psx_mcm_set");
     if (f[0] == 1 && f[1] == 1 && f[2] == 1 && f[3] == 1)
@@ -471,12 +452,12 @@ Config pick_config(const ContractRequest& rq) {
     switch (rq.cov_kind) {
       case PSX_COV_TTTT: return cfg<4, 8, 0, 0, 0, EPI_COV, PSX_COV_TTTT>(4, "psx_cov_tttt");
       case PSX_COV_EEEE: return cfg<4, 0, 0, 8, 0, EPI_COV, PSX_COV_EEEE>(4, "psx_cov_eeee");
-      case PSX_COV_TTTE: return cfg<8, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE>(4, "psx_cov_ttte");
+      case PSX_COV_TTTE: return cfg<4, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE>(4, "psx_cov_ttte");
       case PSX_COV_TETE: return cfg<4, 1, 4, 0, 0, EPI_COV, PSX_COV_TETE>(4, "psx_cov_tete");
       case PSX_COV_TEEE: return cfg<4, 0, 4, 0, 0, EPI_COV, PSX_COV_TEEE>(4, "psx_cov_teee");
       case PSX_COV_TEEE_PLANCK:
         return cfg<4, 0, 0, 4, 0, EPI_COV, PSX_COV_TEEE_PLANCK>(4, "psx_cov_teee_planck");
-      case PSX_COV_TTEE: return cfg<8, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE>(4, "psx_cov_ttee");
+      case PSX_COV_TTEE: return cfg<4, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE>(4, "psx_cov_ttee");
     }
   }
   throw Error{PSX_ERR_ARGUMENT, "unsupported contraction configuration"};
@@ -492,12 +473,10 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
                                     void* scratch, size_t scratch_bytes);
 
 size_t contraction_scratch_bytes(const ContractRequest& rq) {
-  // V tables dominate: nv * nrows * pitch doubles (+ error flag)
-  const int R = 16;
-  const int64_t nrows = rq.lmax + 64 + R + 1;
+  // What tables (nv x wh_len doubles) + error flag
   const int64_t LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
-  const int64_t pitch = kVOff + std::max<int64_t>(LW + 3 * R, rq.lmax + 64) + 32;
-  return (size_t)nv_of(rq) * nrows * pitch * sizeof(double) + 256;
+  const int64_t len = kWOff + LW + 3 * 16 + 32;
+  return (size_t)nv_of(rq) * len * sizeof(double) + 512;
 }
 
 void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
@@ -523,34 +502,28 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ensure_tables(c, (int)rq.LW + 2 * rq.lmax + 512);
   const Geometry g = check_geometry(tl.host, rq.lmax, (int)rq.LW, cf.R, c->tables.n);
 
-  // scratch: [V tables][err flag]
-  const size_t vbytes = (size_t)nv * g.nrows * g.pitch * sizeof(double);
-  PSX_REQUIRE(vbytes + 256 <= scratch_bytes, PSX_ERR_BUDGET, "scratch too small");
-  double* V = (double*)scratch;
-  int* err = (int*)((char*)scratch + ((vbytes + 255) & ~(size_t)255));
+  // scratch: [What tables][err flag]
+  const size_t whbytes = (size_t)nv * g.wh_len * sizeof(double);
+  PSX_REQUIRE(whbytes + 512 <= scratch_bytes, PSX_ERR_BUDGET, "scratch too small");
+  double* Wh = (double*)scratch;
+  int* err = (int*)((char*)scratch + ((whbytes + 255) & ~(size_t)255));
 
-  VArgs va{};
-  int fam_of[kMaxV];
-  {
-    int k = 0;
-    for (int fam = 0; fam < 4; ++fam)
-      for (int i = 0; i < rq.nfam[fam]; ++i) fam_of[k++] = fam;
-  }
+  WhArgs wa{};
   for (int k = 0; k < nv; ++k) {
-    va.V[k] = V + (size_t)k * g.nrows * g.pitch;
-    va.W[k] = rq.dW[k];
-    va.odd[k] = fam_of[k] == FAM_PO;
+    wa.Wh[k] = Wh + (size_t)k * g.wh_len;
+    wa.W[k] = rq.dW[k];
   }
-  va.pitch = g.pitch;
-  va.LW = (int)rq.LW;
-  va.G = c->tables.dG;
-  va.Z = c->tables.dZ;
-  va.D = c->tables.dD;
-  va.IG = c->tables.dIG;
+  wa.len = g.wh_len;
+  wa.LW = (int)rq.LW;
   {
-    dim3 grid((unsigned)((g.pitch + 255) / 256), (unsigned)g.nrows, (unsigned)nv);
-    hipLaunchKernelGGL(vtable_kernel, grid, dim3(256), 0, s, va);
-    PSX_HIP(hipGetLastError());
+    dim3 grid((unsigned)((g.wh_len + 255) / 256), (unsigned)nv);
+    if (s == c->stream) {
+      TimedLaunch tlv(c, "psx_what");
+      hipLaunchKernelGGL(what_kernel, grid, dim3(256), 0, s, wa);
+    } else {
+      hipLaunchKernelGGL(what_kernel, grid, dim3(256), 0, s, wa);
+      PSX_HIP(hipGetLastError());
+    }
   }
   if (rq.epi == EPI_COV) PSX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
 
@@ -562,10 +535,11 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ka.LW = (int)rq.LW;
   ka.G = c->tables.dG;
   ka.Z = c->tables.dZ;
+  ka.D = c->tables.dD;
+  ka.IG = c->tables.dIG;
   ka.T = c->tables.dT;
-  ka.tab_n = g.tab_n;
-  for (int k = 0; k < nv; ++k) ka.V[k] = va.V[k];
-  ka.vpitch = g.pitch;
+  ka.T1 = c->tables.dT1;
+  for (int k = 0; k < nv; ++k) ka.Wh[k] = wa.Wh[k];
   ka.ldo = rq.ldo;
   if (rq.epi == EPI_MCM) {
     for (int k = 0; k < nv; ++k) ka.out[k] = rq.dout[k];
@@ -577,20 +551,14 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
     }
   }
   ka.err = err;
-  const bool kG = (rq.nfam[0] + rq.nfam[1] + rq.nfam[2]) > 0;
-  const bool kZ = rq.nfam[3] > 0;
-  const size_t lds = (size_t)((kG ? 1 : 0) + (kZ ? 1 : 0)) * g.tab_n * sizeof(double);
-  PSX_REQUIRE(lds <= 160 * 1024, PSX_ERR_ARGUMENT, "lmax too large for LDS staging");
-  PSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cf.fn),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int threads = 64 * cf.waves;
   const unsigned blocks = (unsigned)((tl.n + cf.waves - 1) / cf.waves);
   {
     if (s == c->stream) {
       TimedLaunch tlch(c, cf.name);
-      hipLaunchKernelGGL(cf.fn, dim3(blocks), dim3(threads), lds, s, ka);
+      hipLaunchKernelGGL(cf.fn, dim3(blocks), dim3(threads), 0, s, ka);
     } else {
-      hipLaunchKernelGGL(cf.fn, dim3(blocks), dim3(threads), lds, s, ka);
+      hipLaunchKernelGGL(cf.fn, dim3(blocks), dim3(threads), 0, s, ka);
       PSX_HIP(hipGetLastError());
     }
   }

@@ -53,19 +53,28 @@ TimedLaunch::TimedLaunch(Ctx* ctx, const char* kname) : c(ctx), name(kname) {
 TimedLaunch::~TimedLaunch() noexcept(false) {
   hipError_t le = hipGetLastError();
   if (e0) {
-    hipEventRecord(e1, c->stream);
-    hipEventSynchronize(e1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    auto& t = c->timers[name];
-    t.total_ms += ms;
-    t.launches += 1;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventRecord(e1, c->stream);
+    c->timers[name].pending.emplace_back(e0, e1);
   }
   if (le != hipSuccess)
     throw Error{PSX_ERR_DEVICE, std::string("launch of ") + name + ": " +
                                     hipGetErrorString(le)};
+}
+
+// Wait for the recorded launches and fold their durations into the totals.
+void resolve_timers(Ctx* c) {
+  for (auto& kv : c->timers) {
+    for (auto& ev : kv.second.pending) {
+      PSX_HIP(hipEventSynchronize(ev.second));
+      float ms = 0;
+      PSX_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+      kv.second.total_ms += ms;
+      kv.second.launches += 1;
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
 }
 
 // gamma_m = prod_{k<=m} (2k-1)/(2k) accumulated in long double (64-bit
@@ -75,7 +84,7 @@ void ensure_tables(Ctx* c, int nmax_index) {
   if (t.n >= nmax_index + 2 * kTabOff) return;
   int n = nmax_index + 2 * kTabOff;
   n = (n + 1023) & ~1023;
-  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0), T(n, 0.0);
+  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0), T(n, 0.0), T1(n, 0.0);
   long double g = 1.0L;
   for (int m = 0; m + kTabOff < n; ++m) {
     if (m > 0) g = g * (long double)(2 * m - 1) / (long double)(2 * m);
@@ -88,6 +97,7 @@ void ensure_tables(Ctx* c, int nmax_index) {
   for (int i = 0; i < n; ++i) {
     long l = (long)i - kTabOff;
     T[i] = (double)(l * (l + 1));
+    T1[i] = (double)(l * (l + 1) + 1);
   }
   PSX_HIP(hipStreamSynchronize(c->stream));
   if (t.dG) {
@@ -96,6 +106,7 @@ void ensure_tables(Ctx* c, int nmax_index) {
     ctx_free(c, t.dD);
     ctx_free(c, t.dIG);
     ctx_free(c, t.dT);
+    ctx_free(c, t.dT1);
   }
   size_t b = sizeof(double) * n;
   t.dG = (double*)ctx_alloc(c, b);
@@ -103,11 +114,13 @@ void ensure_tables(Ctx* c, int nmax_index) {
   t.dD = (double*)ctx_alloc(c, b);
   t.dIG = (double*)ctx_alloc(c, b);
   t.dT = (double*)ctx_alloc(c, b);
+  t.dT1 = (double*)ctx_alloc(c, b);
   PSX_HIP(hipMemcpy(t.dG, G.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dZ, Z.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dD, D.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dIG, IG.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dT, T.data(), b, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(t.dT1, T1.data(), b, hipMemcpyHostToDevice));
   t.n = n;
 }
 
@@ -148,12 +161,16 @@ int psx_ctx_destroy(psx_ctx* h) {
   if (!c) return PSX_ERR_ARGUMENT;
   {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    for (auto& kv : c->allocations) hipFree(kv.first);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    try {
+      resolve_timers(c);
+    } catch (...) {
+    }
+    for (auto& kv : c->allocations) (void)hipFree(kv.first);
     c->allocations.clear();
     release_blas(c);
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
   }
   delete c;
   return PSX_OK;
@@ -162,7 +179,7 @@ int psx_ctx_destroy(psx_ctx* h) {
 int psx_ctx_synchronize(psx_ctx* h) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
-    hipSetDevice(c->device);
+    PSX_HIP(hipSetDevice(c->device));
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -188,6 +205,7 @@ int psx_ctx_kernel_time(psx_ctx* h, const char* name, int reset,
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
     PSX_REQUIRE(name != nullptr, PSX_ERR_ARGUMENT, "null kernel name");
+    resolve_timers(c);
     auto it = c->timers.find(name);
     double ms = 0;
     int64_t n = 0;
@@ -205,7 +223,7 @@ int psx_dev_alloc(psx_ctx* h, size_t bytes, void** dptr) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
     PSX_REQUIRE(dptr != nullptr, PSX_ERR_ARGUMENT, "null output pointer");
-    hipSetDevice(c->device);
+    PSX_HIP(hipSetDevice(c->device));
     *dptr = ctx_alloc(c, bytes);
   });
 }
@@ -213,7 +231,7 @@ int psx_dev_alloc(psx_ctx* h, size_t bytes, void** dptr) {
 int psx_dev_free(psx_ctx* h, void* dptr) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
-    hipSetDevice(c->device);
+    PSX_HIP(hipSetDevice(c->device));
     PSX_HIP(hipStreamSynchronize(c->stream));
     ctx_free(c, dptr);
   });
@@ -222,7 +240,7 @@ int psx_dev_free(psx_ctx* h, void* dptr) {
 int psx_memcpy_h2d(psx_ctx* h, void* dst, const void* src, size_t bytes) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
-    hipSetDevice(c->device);
+    PSX_HIP(hipSetDevice(c->device));
     PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
@@ -231,7 +249,7 @@ int psx_memcpy_h2d(psx_ctx* h, void* dst, const void* src, size_t bytes) {
 int psx_memcpy_d2h(psx_ctx* h, void* dst, const void* src, size_t bytes) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   return guarded(c, [&] {
-    hipSetDevice(c->device);
+    PSX_HIP(hipSetDevice(c->device));
     PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     PSX_HIP(hipStreamSynchronize(c->stream));
   });

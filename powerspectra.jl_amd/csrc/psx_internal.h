@@ -49,13 +49,16 @@ struct Tables {
   double* dZ = nullptr;
   double* dD = nullptr;
   double* dIG = nullptr;
-  double* dT = nullptr;  // T[l] = l (l+1) for l >= -kTabOff (exact)
+  double* dT = nullptr;   // T[l]  = l (l+1)      (exact)
+  double* dT1 = nullptr;  // T1[l] = l (l+1) + 1  (exact)
 };
 
 // ---- context -------------------------------------------------------------
 struct KernelTimer {
   double total_ms = 0;
   int64_t launches = 0;
+  // event pairs recorded on the context stream, resolved on query
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
 
 struct Ctx {
@@ -77,6 +80,7 @@ struct Ctx {
   void* blas = nullptr;
 };
 
+void resolve_timers(Ctx* c);
 void* ctx_alloc(Ctx* c, size_t bytes);
 void ctx_free(Ctx* c, void* p);
 void* ctx_scratch(Ctx* c, size_t bytes);

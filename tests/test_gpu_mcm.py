@@ -125,3 +125,19 @@ def test_errors(ctx, pkg):
     st = L.psx_mcm(ctx.handle, 0, 0, 40, W.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                    W.size, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 30)
     assert st == 2  # PSX_ERR_AXES
+
+
+def test_reference_namaster_diagonals_gpu(ctx):
+    """The HIP path against the reference's NaMaster fixtures
+    (test/data/mcm_{TT,EE,TE}_diag.txt): with the even-l3 mask spectrum fitted
+    from the TT diagonal (oracle families, CPU), the GPU diagonals reproduce
+    TT and predict EE (M⁺⁺) and TE."""
+    from test_oracle import _namaster_fit
+    W, tt, ee, te, _ = _namaster_fit()
+    lmax = 767
+    dtt = np.diag(ctx.mcm(0, 0, lmax, W))[2:767]
+    dee = np.diag(ctx.mcm(2, 0, lmax, W))[2:767]
+    dte = np.diag(ctx.mcm(1, 0, lmax, W))[2:767]
+    assert np.max(np.abs(dtt - tt) / np.abs(tt)) < 1e-12
+    assert np.max(np.abs(dee - ee) / np.abs(ee)) < 5e-7
+    assert np.max(np.abs(dte - te) / np.abs(te)) < 5e-7
