@@ -7,7 +7,8 @@ cross-spectra W (TT, TE, ET, PP) of length lmax+1 (the reference truncation,
 src/modecoupling.jl:197), into five 3072 x 3072 FP64 matrices.  Inputs and
 outputs are resident in HBM; the timed region is the device work only.
 
-Prints ONE JSON line (see README for the fields).  `roofline` is for the
+Prints ONE JSON line.  `value` is elements/s of the whole step (5 n^2 matrix
+entries per step); `roofline` is for the
 dominant kernel (psx_mcm_set), timed with HIP events on the stream it runs on;
 `cpu_baseline` times the CPU oracle (a C restatement of the reference
 algorithm, OpenMP over l1 rows) on a bounded sample of the same workload.
@@ -30,8 +31,8 @@ METRIC = "MCM elements/s at lmax 3071 (FP64 vs mixed, ≤1e-9 rel); req/s + p99 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# FP64 flops the closed-form contraction spends per (l1, l2, c) step, FMA = 2
-# ("Roofline"): set = p 1, TT 2, E/Q 3, TE 1+2, ET 2, ++ 1+2,
+# FP64 flops the closed-form contraction spends per (l1, l2, c) step, FMA = 2:
+# set = lane product 1, TT fma 2, E and Q 1+2, TE 1+2, ET 2, ++ 1+2,
 # -- 1+1+1+1+2.
 FLOPS_PER_STEP = {"psx_mcm_set": 20, "psx_mcm_00": 3}
 
@@ -127,8 +128,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(see DESIGN.md, 'More than one GPU')", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
+              "(row shards of the l1 <= l2 triangle plus one all-gather would scale it; "
+              "not built)", file=sys.stderr)
         return 2
 
     from __graft_entry__ import load_package
@@ -155,7 +157,7 @@ def main() -> int:
         step()
     ctx.synchronize()
     ctx.kernel_time("psx_mcm_set", reset=True)
-    ctx.kernel_time("psx_vtable", reset=True)
+    ctx.kernel_time("psx_what", reset=True)
     ctx.set_timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -165,7 +167,7 @@ def main() -> int:
     t1 = time.perf_counter()
     ctx.set_timing(False)
     k_ms, k_n = ctx.kernel_time("psx_mcm_set", reset=True)
-    v_ms, v_n = ctx.kernel_time("psx_vtable", reset=True)
+    v_ms, v_n = ctx.kernel_time("psx_what", reset=True)
 
     dt = t1 - t0
     elements = 5 * n * n
@@ -174,8 +176,6 @@ def main() -> int:
     flops = FLOPS_PER_STEP["psx_mcm_set"] * ev
     k_avg = k_ms / max(k_n, 1)
     achieved_tflops = flops / (k_avg * 1e-3) / 1e12
-    # survey convention v1 (prices a per-term recurrence this design does not run)
-    t_fam_even = None
     traffic = read_traffic("psx_mcm_set_lmax3071")
 
     out = {
@@ -201,7 +201,7 @@ def main() -> int:
             "elements_per_step": elements,
             "precision": "fp64",
         },
-        "kernel_ms": {"psx_mcm_set": k_avg, "psx_vtable": v_ms / max(v_n, 1)},
+        "kernel_ms": {"psx_mcm_set": k_avg, "psx_what": v_ms / max(v_n, 1)},
         "roofline": {
             "bound": "fp64-valu",
             "achieved": achieved_tflops,

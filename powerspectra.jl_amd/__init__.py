@@ -1,13 +1,28 @@
 """MI355X (gfx950) drop-in for PowerSpectra.jl's mode-coupling and covariance
 hot path.  The compute path is the HIP library libpsx_hip.so (C ABI in
 include/powerspectra_hip.h); this package mirrors the reference's Julia API
-on top of it.
+on top of it (names as exported by the reference's src/PowerSpectra.jl:40-51).
 """
 from . import _abi
-from ._abi import (ArgumentError, AxisError, BudgetError, Context, DeviceError,
-                   DomainError, PowerSpectraError, SingularException, default_context)
+from ._abi import (ArgumentError, AxisError, BudgetError, Context, DeviceError, DomainError,
+                   PowerSpectraError, SingularException, default_context)
+from .alm import Alm, alm2cl
+from .blockspectralmatrix import (BlockSpectralMatrix, block_vector, getblock, hcat, hvcat, solve,
+                                  spectra, vcat)
+from .covariance import coupledcov, coupledcovTEEE, decouple_covmat
+from .modecoupling import mask, mcm, mcm_from_spectrum, mcm_set, scale
+from .sht import HealpixMap, alm2map, map2alm, nside2lmax, nside2npix
+from .spectralarray import (SpectralArray, SpectralVector, inv, parent, spectralones,
+                            spectralzeros)
+from .workspace import (ConstantDict, CovarianceWorkspace, CovField, SpectralWorkspace,
+                        SpectrumName)
 
 __all__ = [
-    "ArgumentError", "AxisError", "BudgetError", "Context", "DeviceError", "DomainError",
-    "PowerSpectraError", "SingularException", "default_context",
+    "Alm", "ArgumentError", "AxisError", "BlockSpectralMatrix", "BudgetError", "ConstantDict",
+    "Context", "CovField", "CovarianceWorkspace", "DeviceError", "DomainError", "HealpixMap",
+    "PowerSpectraError", "SingularException", "SpectralArray", "SpectralVector",
+    "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "block_vector", "coupledcov",
+    "coupledcovTEEE", "decouple_covmat", "default_context", "getblock", "hcat", "hvcat", "inv",
+    "map2alm", "mask", "mcm", "mcm_from_spectrum", "mcm_set", "nside2lmax", "nside2npix",
+    "parent", "scale", "solve", "spectra", "spectralones", "spectralzeros", "vcat",
 ]

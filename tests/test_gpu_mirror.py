@@ -1,0 +1,141 @@
+"""GPU: the Julia-API mirror end to end (masks -> SHT -> mcm / coupledcov /
+decouple_covmat / `\\`), against the oracle on the same mask spectra."""
+import numpy as np
+import pytest
+from helpers import RTOL_FP64, norm_err, rel_err, spectra_set
+
+pytestmark = pytest.mark.gpu
+
+NSIDE = 32
+
+
+def _masks(seed=0):
+    """Two smooth cut-sky masks on a HEALPix nside-32 grid (apodised latitude
+    cut times a few smoothed holes)."""
+    from __graft_entry__ import load_package
+    pkg = load_package()
+    from powerspectra_amd.sht import RingInfo
+    info = RingInfo(NSIDE)
+    npix = 12 * NSIDE * NSIDE
+    z = np.zeros(npix)
+    phi = np.zeros(npix)
+    for k in range(info.first.size):
+        n = int(info.count[k])
+        z[info.first[k]: info.first[k] + n] = info.cth[k]
+        phi[info.first[k]: info.first[k] + n] = info.phi0[k] + 2 * np.pi * np.arange(n) / n
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(2):
+        b = np.arcsin(z)
+        m = 0.5 * (1 + np.tanh((np.abs(b) - 0.3) / 0.1))
+        for _ in range(5):
+            z0, p0 = rng.uniform(-1, 1), rng.uniform(0, 2 * np.pi)
+            cosd = z * z0 + np.sqrt(1 - z * z) * np.sqrt(1 - z0 * z0) * np.cos(phi - p0)
+            m *= 1 - np.exp(-(1 - cosd) / 0.005)
+        out.append(pkg.HealpixMap(m, NSIDE))
+    return out
+
+
+@pytest.fixture(scope="module")
+def masks():
+    return _masks(0)
+
+
+def test_mcm_from_masks_vs_oracle(pkg, oracle, masks):
+    m1, m2 = masks
+    a1, a2 = pkg.map2alm(m1), pkg.map2alm(m2)
+    lmax = a1.lmax
+    V = pkg.alm2cl(a1, a2)[: lmax + 1]
+    for spec, kind in (("TT", 0), ("TE", 1), ("M⁺⁺", 2), ("M⁻⁻", 3)):
+        M = pkg.mcm(spec, a1, a2)
+        R = oracle.mcm(kind, 0, lmax, V)
+        assert M.axes() == (range(0, lmax + 1), range(0, lmax + 1))
+        assert rel_err(M.parent, R) <= RTOL_FP64 and norm_err(M.parent, R) <= RTOL_FP64
+    # HealpixMap inputs go through map2alm first (src/modecoupling.jl:250-251)
+    M = pkg.mcm("TT", m1, m2, lmin=2)
+    R = oracle.mcm(0, 2, lmax, V)
+    assert rel_err(M.parent, R) <= RTOL_FP64
+
+
+def test_block_mcms_and_decoupling(pkg, masks):
+    m1, m2 = masks
+    a1, a2 = pkg.map2alm(m1), pkg.map2alm(m2)
+    lmin = 2
+    EE_BB, EB_BE = pkg.mcm(("EE_BB", "EB_BE"), a1, a2, lmin=lmin)
+    Mpp = pkg.mcm("M⁺⁺", a1, a2, lmin=lmin).parent
+    Mmm = pkg.mcm("M⁻⁻", a1, a2, lmin=lmin).parent
+    n = Mpp.shape[0]
+    assert np.array_equal(EE_BB.parent, np.block([[Mpp, Mmm], [Mmm, Mpp]]))
+    assert np.array_equal(EB_BE.parent, np.block([[Mpp, -Mmm], [-Mmm, Mpp]]))
+    # the EE/BB joint solve of maskedalm2spectra (src/modecoupling.jl:362-369)
+    rng = np.random.default_rng(0)
+    ell = np.arange(lmin, lmin + n)
+    clee = pkg.SpectralVector(1e3 / (ell + 1.0) ** 2, lmin - 1)
+    clbb = pkg.SpectralVector(1e2 / (ell + 1.0) ** 2, lmin - 1)
+    pcl = pkg.block_vector(*(pkg.SpectralVector(v, lmin - 1) for v in (
+        Mpp @ clee.parent + Mmm @ clbb.parent, Mmm @ clee.parent + Mpp @ clbb.parent)))
+    ee, bb = pkg.spectra(pkg.solve(EE_BB, pcl))
+    assert ee.firstindex() == lmin
+    assert rel_err(ee.parent, clee.parent) < 1e-9 and rel_err(bb.parent, clbb.parent) < 1e-9
+    # single block `\` keeps the offsets (src/blockspectralmatrix.jl:124-129)
+    M = pkg.mcm("TT", a1, a2, lmin=lmin)
+    x = pkg.solve(M, pkg.SpectralVector(M.parent @ clee.parent, lmin - 1))
+    assert x.firstindex() == lmin and rel_err(x.parent, clee.parent) < 1e-9
+
+
+def test_coupledcov_workspace_vs_oracle(pkg, oracle, masks):
+    m1, m2 = masks
+    npix = 12 * NSIDE * NSIDE
+    rng = np.random.default_rng(3)
+    s2a = (rng.uniform(1, 2, npix),) * 3
+    s2b = (rng.uniform(1, 2, npix),) * 3
+    f1 = pkg.CovField("143_hm1", m1, m2, s2a)
+    f2 = pkg.CovField("143_hm2", m2, m1, s2b)
+    ws = pkg.CovarianceWorkspace(f1, f2, f1, f2)
+    lmax = ws.lmax
+    names = ("143_hm1", "143_hm2")
+    sp, rt = spectra_set(lmax, 2)
+    spectra, ratios = {}, {}
+    for XY, base in (("TT", sp[0]), ("EE", sp[1]), ("TE", 0.3 * sp[2])):
+        for a in names:
+            for b in names:
+                spectra[(XY, a, b)] = base * (1.1 if a == b else 1.0)
+                ratios[(XY, a, b)] = rt[0] if a == b else np.ones(lmax + 1)
+    i, j, p, q = ws.field_names
+    W = ws.window_function_W
+    E = "∅∅"
+    # TTTT through the mirror vs the oracle fed the same spectra, ratios and W
+    C = pkg.coupledcov("TT", "TT", ws, spectra, ratios)
+    Wk = [W(E, E, i, p, "TT", j, q, "TT"), W(E, E, i, q, "TT", j, p, "TT"),
+          W(E, "TT", i, p, "TT", j, q, "TT"), W(E, "TT", j, q, "TT", i, p, "TT"),
+          W(E, "TT", i, q, "TT", j, p, "TT"), W(E, "TT", j, p, "TT", i, q, "TT"),
+          W("TT", "TT", i, p, "TT", j, q, "TT"), W("TT", "TT", i, q, "TT", j, p, "TT")]
+    R = oracle.cov(0, 0, lmax,
+                   [spectra["TT", i, p], spectra["TT", j, q], spectra["TT", i, q], spectra["TT", j, p]],
+                   [ratios["TT", i, p], ratios["TT", j, q], ratios["TT", i, q], ratios["TT", j, p]], Wk)
+    assert rel_err(C.parent, R) <= RTOL_FP64
+    # TETE
+    C = pkg.coupledcov("TE", "TE", ws, spectra, ratios)
+    Wk = [W(E, E, i, p, "TT", j, q, "PP"), W(E, E, i, q, "TP", j, p, "PT"),
+          W(E, "PP", i, p, "TT", j, q, "PP"), W(E, "TT", j, q, "PP", i, p, "TT"),
+          W("TT", "PP", i, p, "TT", j, q, "PP")]
+    R = oracle.cov(3, 0, lmax,
+                   [spectra["TT", i, p], spectra["EE", j, q], spectra["TE", i, q], spectra["TE", j, p]],
+                   [ratios["TT", i, p], ratios["EE", j, q]], Wk)
+    assert rel_err(C.parent, R) <= RTOL_FP64 and norm_err(C.parent, R) <= RTOL_FP64
+    # the other pairs run and are symmetric
+    for a, b in (("EE", "EE"), ("TT", "TE"), ("TT", "EE"), ("TE", "EE")):
+        C = pkg.coupledcov(a, b, ws, spectra, ratios)
+        assert np.isfinite(C.parent).all() and np.array_equal(C.parent, C.parent.T)
+
+
+def test_decouple_covmat_mirror(pkg, masks):
+    m1, m2 = masks
+    M = pkg.mcm("TT", m1, m2, lmin=2)
+    rng = np.random.default_rng(4)
+    X = rng.uniform(size=M.shape)
+    Cc = pkg.SpectralArray(X.T @ X, firsts=[2, 2])
+    Cd = pkg.decouple_covmat(Cc, M, M)
+    assert Cd.axes() == Cc.axes()
+    back = M.parent @ Cd.parent @ M.parent.T
+    assert np.linalg.norm(back - Cc.parent) / np.linalg.norm(Cc.parent) < 1e-9

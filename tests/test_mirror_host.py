@@ -1,0 +1,191 @@
+"""Host-side logic of the Julia-API mirror (no GPU): SpectralArray and block
+conventions, alm2cl, the HEALPix transforms, the W-spectra cache and the
+dispatch of mcm / coupledcov onto the C ABI (recorded with a fake context)."""
+import numpy as np
+import pytest
+
+
+class FakeCtx:
+    """Stands in for a device context and records what the mirror asks of the ABI."""
+
+    def __init__(self):
+        self.calls = []
+
+    def mcm(self, kind, lmin, lmax, W):
+        self.calls.append(("mcm", kind, lmin, lmax, np.array(W)))
+        n = lmax - lmin + 1
+        return np.full((n, n), float(kind + 1), order="F")
+
+    def mcm_pm(self, lmin, lmax, W):
+        self.calls.append(("mcm_pm", lmin, lmax, np.array(W)))
+        n = lmax - lmin + 1
+        return np.full((n, n), 1.0, order="F"), np.full((n, n), 2.0, order="F")
+
+    def cov(self, kind, lmin, lmax, spectra, ratios, W, precision=0):
+        self.calls.append(("cov", kind, lmin, lmax, spectra, ratios, W))
+        n = lmax - lmin + 1
+        return np.zeros((n, n), order="F")
+
+    def lu_solve(self, A, B):
+        self.calls.append(("lu_solve",))
+        return np.linalg.solve(A, B)
+
+    def decouple_covmat(self, Y, B1, B2):
+        return np.linalg.solve(B1, Y) @ np.linalg.inv(B2).T
+
+
+# ---------------------------------------------------- SpectralArray --------
+def test_spectralarray_reference_semantics(pkg):
+    """Mirrors test/test_spectralarray.jl:8-74 of the reference."""
+    x = pkg.spectralones(5)
+    assert np.all(x.parent == 1) and x.firstindex() == 0
+    x = pkg.spectralones(range(31, 36))
+    assert x[35] == 1.0 and x.firstindex() == 31
+    x = pkg.spectralzeros(range(0, 5))
+    assert x[0] == 0.0
+    A = pkg.spectralones(range(35, 40), range(5, 10))
+    assert A.offsets == (34, 4)
+    A = pkg.SpectralArray(np.ones((5, 5)), offsets=(34, 4))
+    assert A.axes(0) == range(35, 40) and A.axes(1) == range(5, 10)
+    assert pkg.SpectralVector(np.ones(5), 1).firstindex() == 2
+    B = pkg.spectralzeros(range(5, 11), range(15, 21))
+    B.parent[:] = 2 * np.eye(6)
+    Bi = pkg.inv(B)
+    assert np.all(B.parent == 4 * Bi.parent) and Bi.offsets == B.offsets
+    ones = pkg.spectralones(5, 5)
+    assert np.array_equal(ones[range(0, 3), range(2, 5)], np.ones((3, 3)))
+    v = ones.view(range(0, 3), range(2, 5))
+    assert v.axes(1) == range(2, 5)
+
+
+def test_block_assembly_and_unpack(pkg):
+    a = pkg.SpectralArray(np.arange(4.0).reshape(2, 2), firsts=[2, 2])
+    b = pkg.SpectralArray(10 + np.arange(4.0).reshape(2, 2), firsts=[2, 2])
+    B = pkg.hvcat([[a, b], [b, a]])
+    assert B.parent.shape == (4, 4)
+    assert np.array_equal(B.parent[:2, 2:], b.parent) and B.m_ells == (range(2, 4), range(2, 4))
+    v = pkg.vcat(pkg.SpectralVector(np.array([1.0, 2.0])), pkg.SpectralVector(np.array([3.0, 4.0])))
+    e, f = pkg.spectra(v)
+    assert np.array_equal(e.parent, [1.0, 2.0]) and np.array_equal(f.parent, [3.0, 4.0])
+
+
+# --------------------------------------------------------------- alm / SHT --
+def test_alm2cl_definition(pkg):
+    a = pkg.Alm(3)
+    a[2, 0] = 1.0
+    a[2, 1] = 1.0 + 1.0j
+    b = pkg.Alm(3)
+    b[2, 0] = 2.0
+    b[2, 1] = 1.0
+    cl = pkg.alm2cl(a, b)
+    assert cl[2] == pytest.approx((1.0 * 2.0 + 2 * 1.0) / 5.0)
+    assert cl[0] == 0 and cl[1] == 0 and cl[3] == 0
+
+
+def test_sht_roundtrip_and_monopole(pkg):
+    rng = np.random.default_rng(0)
+    a = pkg.Alm(20)
+    a.alm[:] = rng.standard_normal(a.alm.size) + 1j * rng.standard_normal(a.alm.size)
+    for l in range(21):
+        a[l, 0] = a[l, 0].real
+    m = pkg.alm2map(a, 16)
+    b = pkg.map2alm(m, lmax=20, niter=10)
+    assert np.abs(b.alm - a.alm).max() < 1e-12
+    c = pkg.map2alm(np.ones(12 * 16 * 16), lmax=20, niter=0)
+    assert abs(c[0, 0] - np.sqrt(4 * np.pi)) < 1e-13     # exact without iterations
+    c = pkg.map2alm(np.ones(12 * 16 * 16), lmax=20, niter=10)
+    assert np.abs(c.alm[1:]).max() < 1e-10              # quadrature leakage iterated away
+
+
+def test_sht_orthogonality(pkg):
+    """Y_lm synthesised on the grid analyses back to the unit coefficient."""
+    for (l, m) in ((0, 0), (3, 1), (7, 7), (12, 5)):
+        a = pkg.Alm(15)
+        a[l, m] = 1.0
+        b = pkg.map2alm(pkg.alm2map(a, 8), lmax=15, niter=12)
+        assert abs(b[l, m] - 1.0) < 1e-10
+        b.alm[(m * (2 * 15 + 1 - m)) // 2 + l] = 0
+        assert np.abs(b.alm).max() < 1e-10
+
+
+def test_workspace_full_sky_W(pkg):
+    """Full-sky masks: W^{∅∅} = 4 pi delta_l0, W^{II,II} = 4 pi (sigma^2 Omega_p)^2 delta_l0
+    (src/workspace.jl:141-213), and the i != j weight terms vanish."""
+    nside = 8
+    npix = 12 * nside * nside
+    ones = np.ones(npix)
+    s2 = 3.0 * ones
+    f1 = pkg.CovField("a", ones, ones, (s2, s2, s2))
+    f2 = pkg.CovField("b", ones, ones, (s2, s2, s2))
+    assert pkg.CovarianceWorkspace(f1, f2, f1, f2).lmax == 23   # 3 nside - 1 by default
+    # at lmax = 2 nside the iterated HEALPix transform converges to machine precision
+    ws = pkg.CovarianceWorkspace(f1, f2, f1, f2, lmax=2 * nside,
+                                 transform=lambda m, lmax: pkg.map2alm(m, lmax=lmax, niter=12))
+    W = ws.window_function_W("∅∅", "∅∅", "a", "a", "TT", "b", "b", "TT")
+    assert abs(W[0] - 4 * np.pi) < 1e-10 and np.abs(W[1:]).max() < 1e-10
+    omega = 4 * np.pi / npix
+    W2 = ws.window_function_W("TT", "TT", "a", "a", "TT", "a", "a", "TT")
+    assert abs(W2[0] - 4 * np.pi * (3.0 * omega) ** 2) < 1e-12
+    W3 = ws.window_function_W("TT", "TT", "a", "b", "TT", "a", "b", "TT")
+    assert np.abs(W3).max() == 0.0
+    assert ws.window_function_W("∅∅", "∅∅", "a", "a", "TT", "b", "b", "TT") is W  # cached
+
+
+# ---------------------------------------------------------- dispatch --------
+def test_mcm_dispatch(pkg):
+    ctx = FakeCtx()
+    V = np.arange(10.0)
+    for name, kind in (("TT", 0), (":TT", 0), ("M⁰⁰", 0), ("TE", 1), ("ET", 1), ("TB", 1),
+                       ("BT", 1), ("M⁰²", 1), ("M²⁰", 1), ("M⁺⁺", 2), ("M⁻⁻", 3)):
+        M = pkg.mcm_from_spectrum(name, V, lmin=2, lmax=7, ctx=ctx)
+        assert ctx.calls[-1][1] == kind and M.axes() == (range(2, 8), range(2, 8))
+        assert np.array_equal(ctx.calls[-1][4], V[:8])   # truncated to lmax + 1
+    B = pkg.mcm_from_spectrum("EE_BB", V, lmax=5, ctx=ctx)
+    assert np.all(B.parent[:6, 6:] == 2.0) and np.all(B.parent[:6, :6] == 1.0)
+    B = pkg.mcm_from_spectrum("EB_BE", V, lmax=5, ctx=ctx)
+    assert np.all(B.parent[:6, 6:] == -2.0)
+    pair = pkg.mcm_from_spectrum(("EE_BB", "EB_BE"), V, lmax=5, ctx=ctx)
+    assert len(pair) == 2
+    assert pkg.mcm_from_spectrum(("TT", "EE"), V, lmax=5, ctx=ctx) is None
+    with pytest.raises(pkg.ArgumentError):
+        pkg.mcm_from_spectrum("EE", V, lmax=5, ctx=ctx)
+
+
+def test_coupledcov_dispatch_and_keys(pkg, capsys):
+    """The spectra / noise ratio / W selection of coupledcov*! (src/covariance.jl:64-418)."""
+    nside = 4
+    npix = 12 * nside * nside
+    rng = np.random.default_rng(1)
+    mk = lambda: rng.uniform(0.5, 1.0, npix)
+    s2 = np.ones(npix)
+    f1 = pkg.CovField("143_hm1", mk(), mk(), (s2, s2, s2))
+    f2 = pkg.CovField("143_hm2", mk(), mk(), (s2, s2, s2))
+    ws = pkg.CovarianceWorkspace(f1, f2, f1, f2)
+    lmax = ws.lmax
+    names = ("143_hm1", "143_hm2")
+    spectra = {}
+    for XY in ("TT", "EE", "TE"):
+        for a in names:
+            for b in names:
+                spectra[(XY, a, b)] = np.full(lmax + 1, hash((XY, a, b)) % 97 + 1.0)
+    ctx = FakeCtx()
+    expect = {("TT", "TT"): (0, 4, 8), ("EE", "EE"): (1, 4, 8), ("TT", "TE"): (2, 2, 4),
+              ("TE", "TE"): (3, 2, 5), ("TE", "EE"): (5, 2, 4), ("TT", "EE"): (6, 0, 2)}
+    for (a, b), (kind, nR, nW) in expect.items():
+        pkg.coupledcov(a, b, ws, spectra, ctx=ctx)
+        call = ctx.calls[-1]
+        assert call[1] == kind and len(call[5]) == nR and len(call[6]) == nW
+        for r in call[5]:
+            assert np.all(r == 1.0)  # ConstantDict default
+    # TETE takes TT_ip, EE_jq, TE_iq, TE_jp (src/covariance.jl:247)
+    pkg.coupledcov("TE", "TE", ws, spectra, ctx=ctx)
+    i, j, p, q = ws.field_names
+    got = ctx.calls[-1][4]
+    for g, key in zip(got, [("TT", i, p), ("EE", j, q), ("TE", i, q), ("TE", j, p)]):
+        assert np.array_equal(g, spectra[key])
+    # exact TEEE through coupledcovTEEE(planck=False)
+    pkg.coupledcovTEEE(ws, spectra, pkg.ConstantDict(np.ones(lmax + 1)), 0, lmax, ctx=ctx,
+                       planck=False)
+    assert ctx.calls[-1][1] == 4
+    assert pkg.coupledcov("EE", "TT", ws, spectra, ctx=ctx) is None
+    assert "EE,TT not implemented" in capsys.readouterr().out
