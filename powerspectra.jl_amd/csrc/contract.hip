@@ -435,15 +435,15 @@ Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
   if (rq.epi == EPI_MCM) {
     if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
-      return cfg<8, 1, 0, 0, 0, EPI_MCM, 0>(4, "psx_mcm_00");
+      return cfg<16, 1, 0, 0, 0, EPI_MCM, 0>(4, "psx_mcm_00");
     if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
-      return cfg<4, 0, 1, 0, 0, EPI_MCM, 0>(4, "psx_mcm_02");
+      return cfg<8, 0, 1, 0, 0, EPI_MCM, 0>(4, "psx_mcm_02");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
-      return cfg<4, 0, 0, 1, 0, EPI_MCM, 0>(4, "psx_mcm_pp");
+      return cfg<8, 0, 0, 1, 0, EPI_MCM, 0>(4, "psx_mcm_pp");
     if (f[0] == 0 && f[1] == 0 && f[2] == 0 && f[3] == 1)
-      return cfg<4, 0, 0, 0, 1, EPI_MCM, 0>(4, "psx_mcm_mm");
+      return cfg<8, 0, 0, 0, 1, EPI_MCM, 0>(4, "psx_mcm_mm");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 1)
-      return cfg<4, 0, 0, 1, 1, EPI_MCM, 0>(4, "psx_mcm_pm");
+      return cfg<8, 0, 0, 1, 1, EPI_MCM, 0>(4, "psx_mcm_pm");
     if (f[0] == 1 && f[1] == 2 && f[2] == 1 && f[3] == 1)
       return cfg<4, 1, 2, 1, 1, EPI_MCM, 0>(4, "psx_mcm_set");
     if (f[0] == 1 && f[1] == 1 && f[2] == 1 && f[3] == 1)
