@@ -157,7 +157,7 @@ def main() -> int:
         step()
     ctx.synchronize()
     ctx.kernel_time("psx_mcm_set", reset=True)
-    ctx.kernel_time("psx_what", reset=True)
+    ctx.kernel_time("psx_records", reset=True)
     ctx.set_timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -167,7 +167,7 @@ def main() -> int:
     t1 = time.perf_counter()
     ctx.set_timing(False)
     k_ms, k_n = ctx.kernel_time("psx_mcm_set", reset=True)
-    v_ms, v_n = ctx.kernel_time("psx_what", reset=True)
+    v_ms, v_n = ctx.kernel_time("psx_records", reset=True)
 
     dt = t1 - t0
     elements = 5 * n * n
@@ -201,7 +201,7 @@ def main() -> int:
             "elements_per_step": elements,
             "precision": "fp64",
         },
-        "kernel_ms": {"psx_mcm_set": k_avg, "psx_what": v_ms / max(v_n, 1)},
+        "kernel_ms": {"psx_mcm_set": k_avg, "psx_records": v_ms / max(v_n, 1)},
         "roofline": {
             "bound": "fp64-valu",
             "achieved": achieved_tflops,

@@ -69,10 +69,9 @@ struct KArgs {
   const double* Z;   // (2m+1) gamma_m
   const double* D;   // 1 / ((2m+1) gamma_m)
   const double* IG;  // 1 / gamma_m
-  const double* T;   // m (m+1)
-  const double* T1;  // m (m+1) + 1
-  // What_k[l3 + kWOff] = (2 l3 + 1) W_k[l3] / (4 pi), zero outside [0, LW)
-  const double* Wh[kMaxV];
+  // per-l3 records (record l3 at [(l3 + kWOff) * RS]), fields as in
+  // contract_kernel
+  const double* rec;
   double* out[kMaxV];
   int64_t ldo;
   const double* spec[4];
@@ -149,14 +148,32 @@ __device__ __forceinline__ double cov_combine(int a, int b, const double* xi,
 }
 
 // ---- the contraction kernel ---------------------------------------------
+// Uniform per-l3 data lives in one interleaved record table: record l3 holds
+// RS doubles, fields
+//   [0, NE)         What_k(l3)       for the even-family tables (00, 02, ++)
+//   [NE, NV)        What_k(l3 - 1)   for the odd-family tables (--)
+//   FT  = NV        l3 (l3 + 1)                      (if a 02/++ table is present)
+//   FT1 = NV + kE   (l3 - 1) l3 + 1                  (if a -- table is present)
+// so the R records a step needs are R consecutive records behind one base
+// pointer: few scalar loads, one address register, immediate offsets.
+__host__ __device__ constexpr int rec_fields(int nv, bool e, bool z) {
+  return nv + (e ? 1 : 0) + (z ? 1 : 0);
+}
+__host__ __device__ constexpr int rec_stride(int nf) {
+  return nf <= 1 ? 1 : nf <= 2 ? 2 : nf <= 4 ? 4 : nf <= 8 ? 8 : 16;
+}
+
 // EPI: EPI_MCM or EPI_COV; COVK: covariance kind for EPI_COV.
 template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK>
 __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   constexpr int NV = N00 + N02 + NPE + NPO;
-  constexpr bool kG = (N00 + N02 + NPE) > 0;
+  constexpr int NE = N00 + N02 + NPE;
+  constexpr bool kG = NE > 0;
   constexpr bool kZ = NPO > 0;
   constexpr bool kE = (N02 + NPE) > 0;
   constexpr bool kS = kE || kZ;  // spin-2 polynomials needed
+  constexpr int RS = rec_stride(rec_fields(NV, kE, kZ));
+  constexpr int FT = NV, FT1 = NV + (kE ? 1 : 0);
 
   const int lane = threadIdx.x & 63;
   const int wave = ufl(threadIdx.x >> 6);
@@ -166,8 +183,6 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   const int S0 = sload(&A.tiles[tile].S0, 0);
   const int lmax = A.lmax;
   const int l1 = x0 + lane;
-  const double* G = A.G + kTabOff;
-  const double* Z = A.Z + kTabOff;
 
   // summation range over c, rounded out to whole blocks of R
   const int chi = min(x0 + 63, lmax);
@@ -192,70 +207,89 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[k][r] = 0.0;
 
+  // Walking pointers; step c = cbeg + blk R + u reads
+  //   lane:    G[l1 - c] = gl[-u],  Z[l1 - c - 1] = zl[-u]
+  //   uniform: gamma_c = sg[u],  z_c = sz[u]
+  //   window refill (q = -c - 1): G[t + q] = gwp[-u], Z[t + q - 1] = zwp[-u],
+  //            delta_{S0+q} = sdp[-u], igamma_{S0+q} = sip[-u]
+  //   records: l3 = S0 + r - 2c at rec[(r - 2u) RS]
+  // with t = S0 - x0 - lane; every pointer advances once per block.
+  const int t = S0 - x0 - lane;
+  const double* gl = A.G + kTabOff + (l1 - cbeg);
+  const double* zl = A.Z + kTabOff + (l1 - cbeg - 1);
+  const double* gwp = A.G + kTabOff + (t - cbeg - 1);
+  const double* zwp = A.Z + kTabOff + (t - cbeg - 2);
+  const double* sg = A.G + kTabOff + cbeg;
+  const double* sz = A.Z + kTabOff + cbeg;
+  const double* sdp = A.D + kTabOff + (S0 - cbeg - 1);
+  const double* sip = A.IG + kTabOff + (S0 - cbeg - 1);
+  const double* rec = A.rec + (int64_t)(kWOff + S0 - 2 * cbeg) * RS;
+
   // sliding windows: slot r at step c needs (q = r - c)
   //   gw: gamma_{t+q} delta_{S0+q},  zw: z_{t+q-1} igamma_{S0+q},
-  // t = S0 - x0 - lane, held at index q mod R (cbeg % R == 0).
-  const int t = S0 - x0 - lane;
+  // held at index q mod R (cbeg % R == 0).
   double gw[kG ? R : 1], zw[kZ ? R : 1];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int q = r - cbeg;
-    if constexpr (kG) gw[r] = G[t + q] * sload(A.D, (int64_t)kTabOff + S0 + q);
-    if constexpr (kZ) zw[r] = Z[t + q - 1] * sload(A.IG, (int64_t)kTabOff + S0 + q);
+    if constexpr (kG) gw[r] = gwp[r + 1] * sload(sdp, r + 1);
+    if constexpr (kZ) zw[r] = zwp[r + 1] * sload(sip, r + 1);
   }
 
   for (int blk = 0; blk < nblk; ++blk) {
-    const int c0 = cbeg + blk * R;
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-      const int c = c0 + u;
       double g1 = 0.0, z1 = 0.0;
-      if constexpr (kG) g1 = G[l1 - c] * sload(A.G, (int64_t)kTabOff + c);
-      if constexpr (kZ) z1 = Z[l1 - c - 1] * sload(A.Z, (int64_t)kTabOff + c);
-      const int64_t w0 = (int64_t)kWOff + S0 - 2 * c;      // What index of r = 0
-      const int64_t t0 = (int64_t)kTabOff + S0 - 2 * c;    // T index of r = 0
+      if constexpr (kG) g1 = gl[-u] * sload(sg, u);
+      if constexpr (kZ) z1 = zl[-u] * sload(sz, u);
+      const double* ru = rec - 2 * u * RS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int wi = ((r - u) % R + R) % R;
+        const double* rr = ru + r * RS;
         double p = 0.0, Q = 0.0;
         if constexpr (kG) p = g1 * gw[wi];
         if constexpr (kE) {
-          const double E = sig[r] - sload(A.T, t0 + r);
+          const double E = sig[r] - sload(rr, FT);
           Q = fma(E, E, -kap[r]);
         }
 #pragma unroll
-        for (int k = 0; k < N00; ++k)
-          acc[k][r] = fma(sload(A.Wh[k], w0 + r), p, acc[k][r]);
+        for (int k = 0; k < N00; ++k) acc[k][r] = fma(sload(rr, k), p, acc[k][r]);
         double pq = 0.0;
         if constexpr (N02 > 0) {
           pq = p * Q;
 #pragma unroll
           for (int k = 0; k < N02; ++k)
-            acc[N00 + k][r] = fma(sload(A.Wh[N00 + k], w0 + r), pq, acc[N00 + k][r]);
+            acc[N00 + k][r] = fma(sload(rr, N00 + k), pq, acc[N00 + k][r]);
         }
         if constexpr (NPE > 0) {
           const double pqq = (N02 > 0 ? pq : p * Q) * Q;
 #pragma unroll
           for (int k = 0; k < NPE; ++k)
-            acc[N00 + N02 + k][r] =
-                fma(sload(A.Wh[N00 + N02 + k], w0 + r), pqq, acc[N00 + N02 + k][r]);
+            acc[N00 + N02 + k][r] = fma(sload(rr, N00 + N02 + k), pqq, acc[N00 + N02 + k][r]);
         }
         if constexpr (NPO > 0) {
-          const double D1 = sig[r] - sload(A.T1, t0 - 1 + r);
+          const double D1 = sig[r] - sload(rr, FT1);
           const double pd = (z1 * zw[wi]) * (D1 * D1);
 #pragma unroll
           for (int k = 0; k < NPO; ++k)
-            acc[N00 + N02 + NPE + k][r] = fma(sload(A.Wh[N00 + N02 + NPE + k], w0 - 1 + r),
-                                              pd, acc[N00 + N02 + NPE + k][r]);
+            acc[NE + k][r] = fma(sload(rr, NE + k), pd, acc[NE + k][r]);
         }
       }
       // advance the windows to step c + 1: the slot that held r = R-1 now
       // receives q = -c - 1
       const int ni = (R - 1 - u) % R;
-      const int qn = -c - 1;
-      if constexpr (kG) gw[ni] = G[t + qn] * sload(A.D, (int64_t)kTabOff + S0 + qn);
-      if constexpr (kZ) zw[ni] = Z[t + qn - 1] * sload(A.IG, (int64_t)kTabOff + S0 + qn);
+      if constexpr (kG) gw[ni] = gwp[-u] * sload(sdp, -u);
+      if constexpr (kZ) zw[ni] = zwp[-u] * sload(sip, -u);
     }
+    gl -= R;
+    zl -= R;
+    gwp -= R;
+    zwp -= R;
+    sg += R;
+    sz += R;
+    sdp -= R;
+    sip -= R;
+    rec -= 2 * R * RS;
   }
 
   // ---- epilogue ----
@@ -297,22 +331,41 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   }
 }
 
-// ---- What tables: (2 l3 + 1) W[l3] / (4 pi), zero-padded ------------------
-struct WhArgs {
-  double* Wh[kMaxV];
+// ---- record table: What_k = (2 l3 + 1) W_k[l3] / (4 pi), zero outside
+// [0, LW), and the l3 polynomials, interleaved as described above ----------
+struct RecArgs {
+  double* rec;
   const double* W[kMaxV];
-  int len;  // entries per table (index l3 + kWOff)
+  int len;  // records (index l3 + kWOff)
   int LW;
+  int nv, ne, rs;
+  int ft, ft1;  // field of T / T1, or -1
+  int* err;     // cleared here (covariance calls), or null
 };
 
-__global__ void __launch_bounds__(256) what_kernel(WhArgs a) {
+__device__ __forceinline__ double what_at(const double* W, int l3, int LW) {
+  return (l3 >= 0 && l3 < LW) ? (2.0 * l3 + 1.0) * W[l3] * kInv4Pi : 0.0;
+}
+
+__global__ void __launch_bounds__(256) record_kernel(RecArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
+  if (i == 0 && a.err) *a.err = 0;
   if (i >= a.len) return;
   const int l3 = i - kWOff;
-  double v = 0.0;
-  if (l3 >= 0 && l3 < a.LW) v = (2.0 * l3 + 1.0) * a.W[k][l3] * kInv4Pi;
-  a.Wh[k][i] = v;
+  double* out = a.rec + (int64_t)i * a.rs;
+  for (int f = 0; f < a.rs; ++f) {
+    double v = 0.0;
+    if (f < a.ne) {
+      v = what_at(a.W[f], l3, a.LW);
+    } else if (f < a.nv) {
+      v = what_at(a.W[f], l3 - 1, a.LW);
+    } else if (f == a.ft) {
+      v = (double)l3 * (double)(l3 + 1);
+    } else if (f == a.ft1) {
+      v = (double)(l3 - 1) * (double)l3 + 1.0;
+    }
+    out[f] = v;
+  }
 }
 
 // ---- tile lists ----------------------------------------------------------
@@ -349,14 +402,14 @@ std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R) {
 
 struct Geometry {
   int R;
-  int wh_len;     // entries per What table
+  int wh_len;     // records (l3 = -kWOff .. wh_len - kWOff - 1)
   int tab_need;   // largest 1D table index any read touches
 };
 
 // Index ranges touched by contract_kernel for this tile set, checked against
 // the buffers before launch (the access list follows the kernel: lane reads
 // of G/Z at l1-c(-1) and t+q(-1), uniform reads of D/IG at S0+q, of G/Z at c,
-// of What/T/T1 at l3 = S0+r-2c(-1)).
+// of the l3 records at l3 = S0+r-2c).
 Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R,
                         int tables_n) {
   Geometry g{};
@@ -472,11 +525,27 @@ int nv_of(const ContractRequest& rq) {
 static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
                                     void* scratch, size_t scratch_bytes);
 
+struct RecLayout {
+  int nv, ne, rs, ft, ft1;
+};
+
+RecLayout rec_layout(const ContractRequest& rq) {
+  const int* f = rq.nfam;
+  RecLayout L{};
+  L.nv = f[0] + f[1] + f[2] + f[3];
+  L.ne = f[0] + f[1] + f[2];
+  const bool e = (f[1] + f[2]) > 0, z = f[3] > 0;
+  L.rs = rec_stride(rec_fields(L.nv, e, z));
+  L.ft = e ? L.nv : -1;
+  L.ft1 = z ? L.nv + (e ? 1 : 0) : -1;
+  return L;
+}
+
 size_t contraction_scratch_bytes(const ContractRequest& rq) {
-  // What tables (nv x wh_len doubles) + error flag
+  // record table (len x rs doubles) + error flag
   const int64_t LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
   const int64_t len = kWOff + LW + 3 * 16 + 32;
-  return (size_t)nv_of(rq) * len * sizeof(double) + 512;
+  return (size_t)rec_layout(rq).rs * len * sizeof(double) + 512;
 }
 
 void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
@@ -502,30 +571,34 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ensure_tables(c, (int)rq.LW + 2 * rq.lmax + 512);
   const Geometry g = check_geometry(tl.host, rq.lmax, (int)rq.LW, cf.R, c->tables.n);
 
-  // scratch: [What tables][err flag]
-  const size_t whbytes = (size_t)nv * g.wh_len * sizeof(double);
-  PSX_REQUIRE(whbytes + 512 <= scratch_bytes, PSX_ERR_BUDGET, "scratch too small");
-  double* Wh = (double*)scratch;
-  int* err = (int*)((char*)scratch + ((whbytes + 255) & ~(size_t)255));
+  // scratch: [record table][err flag]
+  const RecLayout rl = rec_layout(rq);
+  const size_t recbytes = (size_t)rl.rs * g.wh_len * sizeof(double);
+  PSX_REQUIRE(recbytes + 512 <= scratch_bytes, PSX_ERR_BUDGET, "scratch too small");
+  double* rec = (double*)scratch;
+  int* err = (int*)((char*)scratch + ((recbytes + 255) & ~(size_t)255));
 
-  WhArgs wa{};
-  for (int k = 0; k < nv; ++k) {
-    wa.Wh[k] = Wh + (size_t)k * g.wh_len;
-    wa.W[k] = rq.dW[k];
-  }
-  wa.len = g.wh_len;
-  wa.LW = (int)rq.LW;
+  RecArgs ra{};
+  ra.rec = rec;
+  for (int k = 0; k < nv; ++k) ra.W[k] = rq.dW[k];
+  ra.len = g.wh_len;
+  ra.LW = (int)rq.LW;
+  ra.nv = rl.nv;
+  ra.ne = rl.ne;
+  ra.rs = rl.rs;
+  ra.ft = rl.ft;
+  ra.ft1 = rl.ft1;
+  ra.err = rq.epi == EPI_COV ? err : nullptr;
   {
-    dim3 grid((unsigned)((g.wh_len + 255) / 256), (unsigned)nv);
+    dim3 grid((unsigned)((g.wh_len + 255) / 256));
     if (s == c->stream) {
-      TimedLaunch tlv(c, "psx_what");
-      hipLaunchKernelGGL(what_kernel, grid, dim3(256), 0, s, wa);
+      TimedLaunch tlv(c, "psx_records");
+      hipLaunchKernelGGL(record_kernel, grid, dim3(256), 0, s, ra);
     } else {
-      hipLaunchKernelGGL(what_kernel, grid, dim3(256), 0, s, wa);
+      hipLaunchKernelGGL(record_kernel, grid, dim3(256), 0, s, ra);
       PSX_HIP(hipGetLastError());
     }
   }
-  if (rq.epi == EPI_COV) PSX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
 
   KArgs ka{};
   ka.tiles = tl.d;
@@ -537,9 +610,7 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ka.Z = c->tables.dZ;
   ka.D = c->tables.dD;
   ka.IG = c->tables.dIG;
-  ka.T = c->tables.dT;
-  ka.T1 = c->tables.dT1;
-  for (int k = 0; k < nv; ++k) ka.Wh[k] = wa.Wh[k];
+  ka.rec = rec;
   ka.ldo = rq.ldo;
   if (rq.epi == EPI_MCM) {
     for (int k = 0; k < nv; ++k) ka.out[k] = rq.dout[k];

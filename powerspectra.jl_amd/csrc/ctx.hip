@@ -84,7 +84,7 @@ void ensure_tables(Ctx* c, int nmax_index) {
   if (t.n >= nmax_index + 2 * kTabOff) return;
   int n = nmax_index + 2 * kTabOff;
   n = (n + 1023) & ~1023;
-  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0), T(n, 0.0), T1(n, 0.0);
+  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0);
   long double g = 1.0L;
   for (int m = 0; m + kTabOff < n; ++m) {
     if (m > 0) g = g * (long double)(2 * m - 1) / (long double)(2 * m);
@@ -94,33 +94,22 @@ void ensure_tables(Ctx* c, int nmax_index) {
     D[i] = (double)(1.0L / ((long double)(2 * m + 1) * g));
     IG[i] = (double)(1.0L / g);
   }
-  for (int i = 0; i < n; ++i) {
-    long l = (long)i - kTabOff;
-    T[i] = (double)(l * (l + 1));
-    T1[i] = (double)(l * (l + 1) + 1);
-  }
   PSX_HIP(hipStreamSynchronize(c->stream));
   if (t.dG) {
     ctx_free(c, t.dG);
     ctx_free(c, t.dZ);
     ctx_free(c, t.dD);
     ctx_free(c, t.dIG);
-    ctx_free(c, t.dT);
-    ctx_free(c, t.dT1);
   }
   size_t b = sizeof(double) * n;
   t.dG = (double*)ctx_alloc(c, b);
   t.dZ = (double*)ctx_alloc(c, b);
   t.dD = (double*)ctx_alloc(c, b);
   t.dIG = (double*)ctx_alloc(c, b);
-  t.dT = (double*)ctx_alloc(c, b);
-  t.dT1 = (double*)ctx_alloc(c, b);
   PSX_HIP(hipMemcpy(t.dG, G.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dZ, Z.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dD, D.data(), b, hipMemcpyHostToDevice));
   PSX_HIP(hipMemcpy(t.dIG, IG.data(), b, hipMemcpyHostToDevice));
-  PSX_HIP(hipMemcpy(t.dT, T.data(), b, hipMemcpyHostToDevice));
-  PSX_HIP(hipMemcpy(t.dT1, T1.data(), b, hipMemcpyHostToDevice));
   t.n = n;
 }
 

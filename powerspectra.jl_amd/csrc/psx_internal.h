@@ -49,8 +49,6 @@ struct Tables {
   double* dZ = nullptr;
   double* dD = nullptr;
   double* dIG = nullptr;
-  double* dT = nullptr;   // T[l]  = l (l+1)      (exact)
-  double* dT1 = nullptr;  // T1[l] = l (l+1) + 1  (exact)
 };
 
 // ---- context -------------------------------------------------------------
