@@ -53,8 +53,10 @@ namespace {
 constexpr int kWOff = 192;   // What index of l3 = 0
 constexpr double kInv4Pi = 0.07957747154594766788444188168625718;  // 1/(4 pi)
 
+// A tile: lanes l1 = x0 .. x0+63 on anti-diagonals S0 .. S0+R-1; the tile
+// owns (computes over c and writes) l1 in [lo, hi] only.
 struct Tile {
-  int x0, S0;
+  int x0, S0, lo, hi;
 };
 
 constexpr int kMaxV = 12;
@@ -163,9 +165,11 @@ __host__ __device__ constexpr int rec_stride(int nf) {
   return nf <= 1 ? 1 : nf <= 2 ? 2 : nf <= 4 ? 4 : nf <= 8 ? 8 : 16;
 }
 
-// EPI: EPI_MCM or EPI_COV; COVK: covariance kind for EPI_COV.
-template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK>
-__global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
+// EPI: EPI_MCM or EPI_COV; COVK: covariance kind for EPI_COV; WPE: minimum
+// waves per SIMD the register allocation must allow.
+template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+contract_kernel(KArgs A) {
   constexpr int NV = N00 + N02 + NPE + NPO;
   constexpr int NE = N00 + N02 + NPE;
   constexpr bool kG = NE > 0;
@@ -181,11 +185,14 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
   if (tile >= A.ntiles) return;
   const int x0 = sload(&A.tiles[tile].x0, 0);
   const int S0 = sload(&A.tiles[tile].S0, 0);
+  const int own_lo = sload(&A.tiles[tile].lo, 0);
+  const int own_hi = sload(&A.tiles[tile].hi, 0);
   const int lmax = A.lmax;
   const int l1 = x0 + lane;
 
-  // summation range over c, rounded out to whole blocks of R
-  const int chi = min(x0 + 63, lmax);
+  // summation range over c (c <= min(l1, l2) = l1 <= own_hi), rounded out
+  // to whole blocks of R
+  const int chi = own_hi;
   const int clo = max(0, (S0 - A.LW + 1) >> 1);
   const int cbeg = ufl((clo / R) * R);
   const int nblk = ufl(max(0, (chi + 1 - cbeg + R - 1) / R));
@@ -300,7 +307,7 @@ __global__ void __launch_bounds__(256) contract_kernel(KArgs A) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int l2 = S0 + r - l1;
-    if (l1 < lmin || l1 > l2 || l2 > lmax) continue;
+    if (l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax) continue;
     const double L2 = (double)l2;
     const double pi2 = (L2 - 1.0) * L2 * (L2 + 1.0) * (L2 + 2.0);
     const bool spin2 = (l1 >= 2) && (l2 >= 2);
@@ -376,20 +383,31 @@ struct TileKey {
   }
 };
 
+// Tiles follow the anti-diagonals: for each group S0 .. S0+R-1 the owned
+// range l1 in [max(lmin, S0-lmax), min((S0+R-1)/2, lmax)] (the l1 <= l2
+// half of the group) is cut into chunks of 64 from the diagonal down, so
+// only the cheap chunk nearest l1 = lmin (or l2 = lmax) is partial.  Lanes
+// never go below l1 = 0: a bottom chunk that would is shifted up and owns
+// only its part.  Sorted by cost (c-range length), longest first.
 std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R) {
   struct TC {
     Tile t;
     int cost;
   };
   std::vector<TC> v;
-  for (int x0 = lmin; x0 <= lmax; x0 += 64) {
-    const int l1hi = std::min(x0 + 63, lmax);
-    const int Slo = 2 * x0, Shi = l1hi + lmax;
-    for (int S0 = Slo; S0 <= Shi; S0 += R) {
+  for (int S0 = 2 * lmin; S0 <= 2 * lmax; S0 += R) {
+    const int hi = std::min((S0 + R - 1) / 2, lmax);
+    const int lo = std::max(lmin, S0 - lmax);
+    for (int top = hi; top >= lo; top -= 64) {
+      Tile t;
+      t.x0 = std::max(top - 63, 0);
+      t.S0 = S0;
+      t.lo = std::max(lo, top - 63);
+      t.hi = top;
       const int clo = std::max(0, (S0 - LW + 1) >> 1);
       const int cbeg = (clo / R) * R;
-      const int nblk = std::max(0, (l1hi + 1 - cbeg + R - 1) / R);
-      v.push_back({{x0, S0}, nblk});
+      const int nblk = std::max(0, (t.hi + 1 - cbeg + R - 1) / R);
+      v.push_back({t, nblk});
     }
   }
   std::stable_sort(v.begin(), v.end(),
@@ -417,7 +435,9 @@ Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R,
   long lo = 0, hi = 0, l3lo = 0, l3hi = 0;
   bool first = true;
   for (const Tile& t : tiles) {
-    const int chi = std::min(t.x0 + 63, lmax);
+    PSX_REQUIRE(t.x0 >= 0 && t.lo >= t.x0 && t.hi <= t.x0 + 63 && t.hi <= lmax,
+                PSX_ERR_ARGUMENT, "tile geometry: bad lane range");
+    const int chi = t.hi;
     const int clo = std::max(0, (t.S0 - LW + 1) >> 1);
     const int cbeg = (clo / R) * R;
     const int nblk = std::max(0, (chi + 1 - cbeg + R - 1) / R);
@@ -479,38 +499,78 @@ struct Config {
   const char* name;
 };
 
-template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK>
+template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE = 1>
 Config cfg(int waves, const char* name) {
-  return Config{R, waves, contract_kernel<R, N00, N02, NPE, NPO, EPI, COVK>, name};
+  return Config{R, waves, contract_kernel<R, N00, N02, NPE, NPO, EPI, COVK, WPE>, name};
 }
+
+// Anti-diagonals per tile (R) and minimum waves per SIMD for each kernel,
+// chosen by timing at lmax = 3071 on gfx950.
+#ifndef PSX_R_00
+#define PSX_R_00 16
+#endif
+#ifndef PSX_R_02
+#define PSX_R_02 8
+#endif
+#ifndef PSX_R_PP
+#define PSX_R_PP 8
+#endif
+#ifndef PSX_R_MM
+#define PSX_R_MM 8
+#endif
+#ifndef PSX_R_PM
+#define PSX_R_PM 4
+#endif
+#ifndef PSX_R_SET
+#define PSX_R_SET 4
+#endif
+#ifndef PSX_WPE_00
+#define PSX_WPE_00 1
+#endif
+#ifndef PSX_WPE_02
+#define PSX_WPE_02 1
+#endif
+#ifndef PSX_WPE_PM
+#define PSX_WPE_PM 1
+#endif
+#ifndef PSX_WPE_SET
+#define PSX_WPE_SET 4
+#endif
+#ifndef PSX_WPE_COV
+#define PSX_WPE_COV 1
+#endif
+#ifndef PSX_R_COV
+#define PSX_R_COV 4
+#endif
 
 Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
   if (rq.epi == EPI_MCM) {
     if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
-      return cfg<16, 1, 0, 0, 0, EPI_MCM, 0>(4, "psx_mcm_00");
+      return cfg<PSX_R_00, 1, 0, 0, 0, EPI_MCM, 0, PSX_WPE_00>(4, "psx_mcm_00");
     if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
-      return cfg<8, 0, 1, 0, 0, EPI_MCM, 0>(4, "psx_mcm_02");
+      return cfg<PSX_R_02, 0, 1, 0, 0, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_02");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
-      return cfg<8, 0, 0, 1, 0, EPI_MCM, 0>(4, "psx_mcm_pp");
+      return cfg<PSX_R_PP, 0, 0, 1, 0, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_pp");
     if (f[0] == 0 && f[1] == 0 && f[2] == 0 && f[3] == 1)
-      return cfg<8, 0, 0, 0, 1, EPI_MCM, 0>(4, "psx_mcm_mm");
+      return cfg<PSX_R_MM, 0, 0, 0, 1, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_mm");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 1)
-      return cfg<8, 0, 0, 1, 1, EPI_MCM, 0>(4, "psx_mcm_pm");
+      return cfg<PSX_R_PM, 0, 0, 1, 1, EPI_MCM, 0, PSX_WPE_PM>(4, "psx_mcm_pm");
     if (f[0] == 1 && f[1] == 2 && f[2] == 1 && f[3] == 1)
-      return cfg<4, 1, 2, 1, 1, EPI_MCM, 0>(4, "psx_mcm_set");
+      return cfg<PSX_R_SET, 1, 2, 1, 1, EPI_MCM, 0, PSX_WPE_SET>(4, "psx_mcm_set");
     if (f[0] == 1 && f[1] == 1 && f[2] == 1 && f[3] == 1)
-      return cfg<4, 1, 1, 1, 1, EPI_MCM, 0>(4, "psx_mcm_set1");
+      return cfg<PSX_R_SET, 1, 1, 1, 1, EPI_MCM, 0, PSX_WPE_SET>(4, "psx_mcm_set1");
   } else {
+    constexpr int RC = PSX_R_COV;
     switch (rq.cov_kind) {
-      case PSX_COV_TTTT: return cfg<4, 8, 0, 0, 0, EPI_COV, PSX_COV_TTTT>(4, "psx_cov_tttt");
-      case PSX_COV_EEEE: return cfg<4, 0, 0, 8, 0, EPI_COV, PSX_COV_EEEE>(4, "psx_cov_eeee");
-      case PSX_COV_TTTE: return cfg<4, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE>(4, "psx_cov_ttte");
-      case PSX_COV_TETE: return cfg<4, 1, 4, 0, 0, EPI_COV, PSX_COV_TETE>(4, "psx_cov_tete");
-      case PSX_COV_TEEE: return cfg<4, 0, 4, 0, 0, EPI_COV, PSX_COV_TEEE>(4, "psx_cov_teee");
+      case PSX_COV_TTTT: return cfg<RC, 8, 0, 0, 0, EPI_COV, PSX_COV_TTTT, PSX_WPE_COV>(4, "psx_cov_tttt");
+      case PSX_COV_EEEE: return cfg<RC, 0, 0, 8, 0, EPI_COV, PSX_COV_EEEE, PSX_WPE_COV>(4, "psx_cov_eeee");
+      case PSX_COV_TTTE: return cfg<RC, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE, PSX_WPE_COV>(4, "psx_cov_ttte");
+      case PSX_COV_TETE: return cfg<RC, 1, 4, 0, 0, EPI_COV, PSX_COV_TETE, PSX_WPE_COV>(4, "psx_cov_tete");
+      case PSX_COV_TEEE: return cfg<RC, 0, 4, 0, 0, EPI_COV, PSX_COV_TEEE, PSX_WPE_COV>(4, "psx_cov_teee");
       case PSX_COV_TEEE_PLANCK:
-        return cfg<4, 0, 0, 4, 0, EPI_COV, PSX_COV_TEEE_PLANCK>(4, "psx_cov_teee_planck");
-      case PSX_COV_TTEE: return cfg<4, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE>(4, "psx_cov_ttee");
+        return cfg<RC, 0, 0, 4, 0, EPI_COV, PSX_COV_TEEE_PLANCK, PSX_WPE_COV>(4, "psx_cov_teee_planck");
+      case PSX_COV_TTEE: return cfg<RC, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE, PSX_WPE_COV>(4, "psx_cov_ttee");
     }
   }
   throw Error{PSX_ERR_ARGUMENT, "unsupported contraction configuration"};
