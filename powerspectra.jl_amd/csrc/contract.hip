@@ -377,9 +377,9 @@ __global__ void __launch_bounds__(256) record_kernel(RecArgs a) {
 
 // ---- tile lists ----------------------------------------------------------
 struct TileKey {
-  int lmin, lmax, LW, R;
+  int lmin, lmax, LW, R, waves;
   bool operator<(const TileKey& o) const {
-    return std::tie(lmin, lmax, LW, R) < std::tie(o.lmin, o.lmax, o.LW, o.R);
+    return std::tie(lmin, lmax, LW, R, waves) < std::tie(o.lmin, o.lmax, o.LW, o.R, o.waves);
   }
 };
 
@@ -388,17 +388,25 @@ struct TileKey {
 // half of the group) is cut into chunks of 64 from the diagonal down, so
 // only the cheap chunk nearest l1 = lmin (or l2 = lmax) is partial.  Lanes
 // never go below l1 = 0: a bottom chunk that would is shifted up and owns
-// only its part.  Sorted by cost (c-range length), longest first.
-std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R) {
+// only its part.
+//
+// Launch order: the `waves` tiles of one workgroup are the same chunk (k-th
+// from the diagonal) of `waves` consecutive groups, so one CU writes
+// waves*R consecutive anti-diagonals of the same l1 range and the output
+// column segments it produces (waves*R doubles) fill whole cache lines in
+// one L2.  Workgroups are sorted by cost (c-range length), longest first.
+std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R, int waves) {
   struct TC {
     Tile t;
     int cost;
   };
-  std::vector<TC> v;
+  // chunks[k] = tiles k-th from the diagonal, in S0 order
+  std::vector<std::vector<TC>> chunks;
   for (int S0 = 2 * lmin; S0 <= 2 * lmax; S0 += R) {
     const int hi = std::min((S0 + R - 1) / 2, lmax);
     const int lo = std::max(lmin, S0 - lmax);
-    for (int top = hi; top >= lo; top -= 64) {
+    int k = 0;
+    for (int top = hi; top >= lo; top -= 64, ++k) {
       Tile t;
       t.x0 = std::max(top - 63, 0);
       t.S0 = S0;
@@ -407,14 +415,62 @@ std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R) {
       const int clo = std::max(0, (S0 - LW + 1) >> 1);
       const int cbeg = (clo / R) * R;
       const int nblk = std::max(0, (t.hi + 1 - cbeg + R - 1) / R);
-      v.push_back({t, nblk});
+      if ((int)chunks.size() <= k) chunks.resize(k + 1);
+      chunks[k].push_back({t, nblk});
     }
   }
-  std::stable_sort(v.begin(), v.end(),
-                   [](const TC& a, const TC& b) { return a.cost > b.cost; });
+  struct WG {
+    int first, n, cost;
+  };
+  std::vector<TC> flat;
+  std::vector<WG> wgs;
+  for (auto& ck : chunks) {
+    for (size_t i = 0; i < ck.size(); i += waves) {
+      WG g{(int)flat.size(), 0, 0};
+      for (size_t j = i; j < std::min(ck.size(), i + waves); ++j) {
+        flat.push_back(ck[j]);
+        g.n++;
+        g.cost = std::max(g.cost, ck[j].cost);
+      }
+      wgs.push_back(g);
+    }
+  }
+  // Longest first, in coarse cost buckets; inside a bucket by (chunk, S0) so
+  // neighbouring anti-diagonal groups stay neighbours in the order.
+  int maxc = 1;
+  for (const WG& g : wgs) maxc = std::max(maxc, g.cost);
+  const int bucket = std::max(1, maxc / 64);
+  std::stable_sort(wgs.begin(), wgs.end(), [bucket](const WG& a, const WG& b) {
+    return a.cost / bucket > b.cost / bucket;
+  });
+  // Workgroup b runs on XCD b mod kXcds.  Give each XCD runs of kRun
+  // consecutive workgroups of the order above, so the partial output lines
+  // two neighbouring groups write meet in the same L2; XCDs still take
+  // turns along the cost order.
+  constexpr int kXcds = 8, kRun = 8;
+  const int nwg = (int)wgs.size();
+  std::vector<int> perm;
+  perm.reserve(nwg);
+  for (int b = 0; (int)perm.size() < nwg; ++b) {
+    const int x = b % kXcds, m = b / kXcds;
+    const long w = (long)(m / kRun) * kXcds * kRun + (long)x * kRun + m % kRun;
+    if (w < nwg) perm.push_back((int)w);
+    if (b > 4 * nwg + 64 * kXcds * kRun) break;  // unreachable guard
+  }
+  // every workgroup occupies exactly `waves` slots; short groups are padded
+  // with empty tiles (hi = -1 < lo: no lane owns anything, no c step runs)
   std::vector<Tile> out;
-  out.reserve(v.size());
-  for (auto& e : v) out.push_back(e.t);
+  out.reserve((size_t)nwg * waves);
+  for (int w : perm) {
+    const WG& g = wgs[w];
+    for (int j = 0; j < waves; ++j) {
+      if (j < g.n) {
+        out.push_back(flat[g.first + j].t);
+      } else {
+        out.push_back(Tile{0, 2 * lmin, 0, -1});
+      }
+    }
+  }
   return out;
 }
 
@@ -476,13 +532,13 @@ struct CachedTiles {
 std::map<std::pair<void*, TileKey>, CachedTiles> g_tile_cache;
 std::mutex g_tile_mu;
 
-const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R) {
+const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int waves) {
   std::lock_guard<std::mutex> lk(g_tile_mu);
-  auto key = std::make_pair((void*)c, TileKey{lmin, lmax, LW, R});
+  auto key = std::make_pair((void*)c, TileKey{lmin, lmax, LW, R, waves});
   auto it = g_tile_cache.find(key);
   if (it != g_tile_cache.end()) return it->second;
   CachedTiles ct;
-  ct.host = make_tiles(lmin, lmax, LW, R);
+  ct.host = make_tiles(lmin, lmax, LW, R, waves);
   ct.n = (int)ct.host.size();
   ct.d = (Tile*)ctx_alloc(c, sizeof(Tile) * std::max(1, ct.n));
   PSX_HIP(hipMemcpy(ct.d, ct.host.data(), sizeof(Tile) * ct.n, hipMemcpyHostToDevice));
@@ -627,7 +683,7 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   const int n = rq.lmax - rq.lmin + 1;
   PSX_REQUIRE(rq.ldo >= n, PSX_ERR_AXES, "leading dimension smaller than lmax-lmin+1");
   const Config cf = pick_config(rq);
-  const CachedTiles& tl = get_tiles(c, rq.lmin, rq.lmax, (int)rq.LW, cf.R);
+  const CachedTiles& tl = get_tiles(c, rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves);
   ensure_tables(c, (int)rq.LW + 2 * rq.lmax + 512);
   const Geometry g = check_geometry(tl.host, rq.lmax, (int)rq.LW, cf.R, c->tables.n);
 
