@@ -16,6 +16,7 @@ algorithm, OpenMP over l1 rows) on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -33,8 +34,15 @@ FP64_PEAK_TFLOPS = 78.6
 
 # FP64 flops the closed-form contraction spends per (l1, l2, c) step, FMA = 2:
 # set = lane product 1, TT fma 2, E and Q 1+2, TE 1+2, ET 2, ++ 1+2,
-# -- 1+1+1+1+2.
-FLOPS_PER_STEP = {"psx_mcm_set": 20, "psx_mcm_00": 3}
+# -- 1+1+1+1+2.  Single blocks and covariance kernels likewise (lane product
+# 1, E 1, Q 2, p*Q 1, p*Q*Q 1, odd-family product 1+1+1+1, 2 per W spectrum).
+FLOPS_PER_STEP = {"psx_mcm_set": 20, "psx_mcm_00": 3, "psx_mcm_02": 7, "psx_mcm_pp": 8,
+                  "psx_mcm_mm": 6, "psx_mcm_pm": 14,
+                  "psx_cov_tttt": 17, "psx_cov_eeee": 22, "psx_cov_ttte": 9,
+                  "psx_cov_tete": 15, "psx_cov_teee": 13, "psx_cov_teee_planck": 14,
+                  "psx_cov_ttee": 5}
+# which l3 parity the kernel's useful steps count (the -- block sums odd l3)
+ODD_STEPS = {"psx_mcm_mm"}
 
 
 def csteps(lmin: int, lmax: int, LW: int) -> tuple[int, int]:
@@ -116,15 +124,211 @@ def cpu_baseline(lmax: int, LW: int, Ws, stride: int, nthreads: int):
     }
 
 
+def _kernel_avg(ctx, name: str) -> float:
+    ms, cnt = ctx.kernel_time(name, reset=True)
+    return ms / max(cnt, 1)
+
+
+def _timed(ctx, fn, names, steps: int, warmup: int):
+    """Run fn warmup + steps times; return (wall seconds of the timed steps,
+    {kernel name: average ms per launch from HIP events})."""
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    for nm in names:
+        ctx.kernel_time(nm, reset=True)
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    return t1 - t0, {nm: _kernel_avg(ctx, nm) for nm in names}
+
+
+def _line(metric, value, unit, args, ms_per_step, config, **extra):
+    out = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded analytic stand-ins for mask cross-spectra and spectra",
+        "config": config,
+    }
+    out.update(extra)
+    return out
+
+
+def _kernel_roofline(name: str, ms: float, ev: int, od: int) -> dict:
+    steps = od if name in ODD_STEPS else ev
+    tf = FLOPS_PER_STEP[name] * steps / (ms * 1e-3) / 1e12
+    return {"ms": ms, "tflops": tf, "frac": tf / FP64_PEAK_TFLOPS, "steps": steps,
+            "flops_per_step": FLOPS_PER_STEP[name]}
+
+
+def run_mcm_blocks(args, pkg) -> dict:
+    """Each single-block kernel (TT, TE, ++, --, fused ++/--) at lmax, device
+    resident, timed separately."""
+    ctx = pkg.Context(0)
+    L = ctx.lib
+    lmax = args.lmax
+    LW = n = lmax + 1
+    Ws = [synthetic_W(LW, s, cross) for s, cross in ((1, False), (2, True), (3, True), (4, False))]
+    dW = [ctx.alloc(8 * LW) for _ in range(4)]
+    for d, w in zip(dW, Ws):
+        ctx.h2d(d, w)
+    dM = [ctx.alloc(8 * n * n) for _ in range(2)]
+    ev, od = csteps(0, lmax, LW)
+    res, total = {}, 0.0
+    jobs = [("psx_mcm_00", lambda: L.psx_mcm_d(ctx.handle, 0, 0, lmax, dW[0], LW, dM[0], n)),
+            ("psx_mcm_02", lambda: L.psx_mcm_d(ctx.handle, 1, 0, lmax, dW[1], LW, dM[0], n)),
+            ("psx_mcm_pp", lambda: L.psx_mcm_d(ctx.handle, 2, 0, lmax, dW[3], LW, dM[0], n)),
+            ("psx_mcm_mm", lambda: L.psx_mcm_d(ctx.handle, 3, 0, lmax, dW[3], LW, dM[0], n)),
+            ("psx_mcm_pm", lambda: L.psx_mcm_pm_d(ctx.handle, 0, lmax, dW[3], LW, dM[0], dM[1], n))]
+    for name, f in jobs:
+        dt, ks = _timed(ctx, lambda: ctx.check(f(), name), [name], args.steps, args.warmup)
+        res[name] = _kernel_roofline(name, ks[name], ev, od)
+        total += dt / args.steps
+    for d in dW + dM:
+        ctx.free(d)
+    ctx.close()
+    return _line("single-block MCM kernels at lmax %d" % lmax, 6 * n * n / total, "elements/s",
+                 args, total * 1e3,
+                 {"workload": f"mcm_blocks_lmax{lmax}: TT, TE, ++, --, fused ++/-- one at a time",
+                  "lmax": lmax, "LW": LW}, kernels=res)
+
+
+def run_cov(args, pkg) -> dict:
+    """The seven coupled-covariance kernels at lmax (default 2508), device
+    resident: spectra C = A (l+10)^-2.5, ratios 1 + 0.1 U, K W spectra."""
+    ctx = pkg.Context(0)
+    L = ctx.lib
+    lmax = args.lmax if args.lmax != 3071 else 2508
+    LW = n = lmax + 1
+    rng = np.random.default_rng(0)
+    ell = np.arange(LW, dtype=np.float64)
+    dW = [ctx.alloc(8 * LW) for _ in range(8)]
+    for k, d in enumerate(dW):
+        ctx.h2d(d, synthetic_W(LW, 10 + k, False))
+    sp = [ctx.alloc(8 * LW) for _ in range(4)]
+    rt = [ctx.alloc(8 * LW) for _ in range(4)]
+    for d in sp:
+        ctx.h2d(d, 1e3 / (ell + 10.0) ** 2.5 * rng.uniform(1, 2))
+    for d in rt:
+        ctx.h2d(d, 1.0 + 0.1 * rng.uniform(size=LW))
+    dC = ctx.alloc(8 * n * n)
+    VP = ctypes.c_void_p
+    arr = lambda ptrs: (VP * len(ptrs))(*[VP(p) for p in ptrs])
+    ev, od = csteps(0, lmax, LW)
+    names = ["tttt", "eeee", "ttte", "tete", "teee", "teee_planck", "ttee"]
+    res, total = {}, 0.0
+    for kind, nm in enumerate(names):
+        nW = pkg._abi.cov_arity(kind)[0]
+        S, R, W = arr(sp), arr(rt), arr(dW[:nW])
+        kname = "psx_cov_" + nm
+        f = lambda: ctx.check(L.psx_cov_d(ctx.handle, kind, 0, lmax, S, R, W, LW, 0, dC, n), kname)
+        dt, ks = _timed(ctx, f, [kname], args.steps, args.warmup)
+        res[kname] = _kernel_roofline(kname, ks[kname], ev, od)
+        total += dt / args.steps
+    for d in dW + sp + rt + [dC]:
+        ctx.free(d)
+    ctx.close()
+    return _line("coupled covariance blocks at lmax %d" % lmax, 7 * n * n / total, "elements/s",
+                 args, total * 1e3,
+                 {"workload": f"cov_lmax{lmax}: TTTT, EEEE, TTTE, TETE, TEEE, TEEE-Planck, TTEE "
+                              "blocks one at a time", "lmax": lmax, "LW": LW}, kernels=res)
+
+
+def run_solve(args, pkg) -> dict:
+    """lmax 6143: the TT and fused ++/-- kernels, then the LU solves the
+    reference's maskedalm2spectra does (M \\ pCl for TT, [M++ M--; M-- M++]
+    \\ [pEE; pBB]), checked by the recovered spectra."""
+    ctx = pkg.Context(0)
+    L = ctx.lib
+    lmax = args.lmax if args.lmax != 3071 else 6143
+    LW = n = lmax + 1
+    Wt, Wp = synthetic_W(LW, 1, False), synthetic_W(LW, 4, False)
+    dW = [ctx.alloc(8 * LW) for _ in range(2)]
+    ctx.h2d(dW[0], Wt)
+    ctx.h2d(dW[1], Wp)
+    dM = [ctx.alloc(8 * n * n) for _ in range(3)]
+    ev, od = csteps(0, lmax, LW)
+    f1 = lambda: ctx.check(L.psx_mcm_d(ctx.handle, 0, 0, lmax, dW[0], LW, dM[0], n), "mcm")
+    f2 = lambda: ctx.check(L.psx_mcm_pm_d(ctx.handle, 0, lmax, dW[1], LW, dM[1], dM[2], n), "pm")
+    steps = max(1, min(args.steps, 3))
+    dt1, k1 = _timed(ctx, f1, ["psx_mcm_00"], steps, 1)
+    dt2, k2 = _timed(ctx, f2, ["psx_mcm_pm"], steps, 1)
+    M = np.empty((n, n), order="F")
+    Mp = np.empty((n, n), order="F")
+    Mm = np.empty((n, n), order="F")
+    ctx.d2h(M, dM[0])
+    ctx.d2h(Mp, dM[1])
+    ctx.d2h(Mm, dM[2])
+    ell = np.arange(LW, dtype=np.float64)
+    cl = 1.0 / (ell + 1.0) ** 2
+    # TT: rows/cols from l = 2 (the monopole/dipole rows of a cut-sky M are
+    # kept; the solve is of the full block as the reference does)
+    t0 = time.perf_counter()
+    x = ctx.lu_solve(M, (M @ cl)[:, None])[:, 0]
+    t1 = time.perf_counter()
+    lo = 2
+    A = np.block([[Mp[lo:, lo:], Mm[lo:, lo:]], [Mm[lo:, lo:], Mp[lo:, lo:]]])
+    clb = 0.1 * cl[lo:]
+    rhs = A @ np.concatenate([cl[lo:], clb])
+    t2 = time.perf_counter()
+    y = ctx.lu_solve(A, rhs[:, None])[:, 0]
+    t3 = time.perf_counter()
+    err_tt = float(np.abs(x - cl).max() / np.abs(cl).max())
+    err_eb = float(np.abs(y - np.concatenate([cl[lo:], clb])).max() / np.abs(cl[lo:]).max())
+    for d in dW + dM:
+        ctx.free(d)
+    ctx.close()
+    res = {"psx_mcm_00": _kernel_roofline("psx_mcm_00", k1["psx_mcm_00"], ev, od),
+           "psx_mcm_pm": _kernel_roofline("psx_mcm_pm", k2["psx_mcm_pm"], ev, od)}
+    total = dt1 / steps + dt2 / steps
+    return _line("MCM TT + fused ++/-- at lmax %d" % lmax, 3 * n * n / total, "elements/s",
+                 args, total * 1e3,
+                 {"workload": f"mcm_solve_lmax{lmax}: TT and fused ++/-- kernels, then LU solves "
+                              f"n = {n} (TT) and n = {2 * (n - lo)} (EE/BB) through psx_lu_solve "
+                              "(host buffers, PCIe included)", "lmax": lmax, "LW": LW},
+                 kernels=res,
+                 solve={"lu_tt_s": t1 - t0, "lu_eebb_s": t3 - t2, "n_tt": n, "n_eebb": 2 * (n - lo),
+                        "recovered_cl_max_rel_err_tt": err_tt,
+                        "recovered_cl_max_rel_err_eebb": err_eb})
+
+
+def run_e2e(args, pkg) -> dict:
+    """The headline workload through the host-pointer entry point psx_mcm_set:
+    W copied in, five blocks computed and copied out each step (PCIe
+    included)."""
+    ctx = pkg.Context(0)
+    lmax = args.lmax
+    LW = n = lmax + 1
+    Ws = [synthetic_W(LW, s, cross) for s, cross in ((1, False), (2, True), (3, True), (4, False))]
+    outs = [np.empty((n, n), order="F") for _ in range(5)]
+    f = lambda: ctx.mcm_set(0, lmax, *Ws, out=outs)
+    dt, ks = _timed(ctx, f, ["psx_mcm_set"], args.steps, args.warmup)
+    ctx.close()
+    elements = 5 * n * n
+    return _line(METRIC, elements * args.steps / dt, "elements/s", args, dt / args.steps * 1e3,
+                 {"workload": f"mcm_set_lmax{lmax}_host: the headline workload with W in and M out "
+                              "over PCIe every step", "lmax": lmax, "LW": LW},
+                 kernel_ms={"psx_mcm_set": ks["psx_mcm_set"]})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--lmax", type=int, default=3071)
-    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="mcm_set",
+                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e"],
+                    help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +340,11 @@ def main() -> int:
     from __graft_entry__ import load_package
 
     pkg = load_package()
+    if args.workload != "mcm_set":
+        fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
+              "e2e": run_e2e}[args.workload]
+        print(json.dumps(fn(args, pkg)), flush=True)
+        return 0
     abi = sys.modules[pkg.__name__ + "._abi"]
     ctx = pkg.Context(0)
     lmax = args.lmax
@@ -203,7 +412,11 @@ def main() -> int:
         },
         "kernel_ms": {"psx_mcm_set": k_avg, "psx_records": v_ms / max(v_n, 1)},
         "roofline": {
-            "bound": "fp64-valu",
+            # compute-bound on FP64: the kernel issues v_fma_f64 / v_mul_f64 on
+            # the vector ALUs; MI355X's FP64 vector and FP64 matrix peaks are
+            # the same 78.6 TFLOP/s
+            "bound": "mfma",
+            "pipe": "fp64 vector (no MFMA)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

@@ -223,7 +223,9 @@ class Context:
                    "psx_mcm_pm")
         return P, M
 
-    def mcm_set(self, lmin: int, lmax: int, W_TT, W_TE, W_ET, W_PP):
+    def mcm_set(self, lmin: int, lmax: int, W_TT, W_TE, W_ET, W_PP, out=None):
+        """The five blocks (TT, TE, ET, ++, --) in one pass; `out` may hold five
+        preallocated n x n Fortran-ordered float64 arrays (None entries skip)."""
         Ws = [None if w is None else np.ascontiguousarray(w, dtype=np.float64)
               for w in (W_TT, W_TE, W_ET, W_PP)]
         LW = next(w.size for w in Ws if w is not None)
@@ -232,7 +234,15 @@ class Context:
         n = _block_size(lmin, lmax)
         want = [Ws[0] is not None, Ws[1] is not None, Ws[2] is not None,
                 Ws[3] is not None, Ws[3] is not None]
-        outs = [np.empty((n, n), order="F") if w else None for w in want]
+        if out is None:
+            outs = [np.empty((n, n), order="F") if w else None for w in want]
+        else:
+            outs = list(out)
+            for o, w in zip(outs, want):
+                if w and (o is None or o.shape != (n, n) or o.dtype != np.float64
+                          or not o.flags.f_contiguous):
+                    raise ArgumentError("out: five n x n Fortran-ordered float64 arrays expected")
+            outs = [o if w else None for o, w in zip(outs, want)]
         nullp = ctypes.cast(None, _dp)
         args = [nullp if w is None else dptr(w) for w in Ws]
         oargs = [nullp if o is None else dptr(o) for o in outs]

@@ -38,9 +38,9 @@ void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 // copy an n x n device block (ld n) into a host block with leading dim ld
+// (staged through pinned memory; returns when the host block is complete)
 void d2h_block(Ctx* c, double* host, int64_t ld, const double* dev, int n) {
-  PSX_HIP(hipMemcpy2DAsync(host, ld * sizeof(double), dev, n * sizeof(double),
-                           n * sizeof(double), n, hipMemcpyDeviceToHost, c->stream));
+  d2h_matrix(c, host, ld, dev, n, n, n);
 }
 
 int mcm_family(int kind) {

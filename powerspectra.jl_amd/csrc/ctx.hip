@@ -1,8 +1,10 @@
 // Context, device memory, kernel timing and the Wigner tables.
 #include "psx_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace psx {
 
@@ -159,6 +161,7 @@ int psx_ctx_destroy(psx_ctx* h) {
     for (auto& kv : c->allocations) (void)hipFree(kv.first);
     c->allocations.clear();
     release_blas(c);
+    release_stage(c);
     (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -245,3 +248,115 @@ int psx_memcpy_d2h(psx_ctx* h, void* dst, const void* src, size_t bytes) {
 }
 
 }  // extern "C"
+
+// ---- staged host copies ---------------------------------------------------
+// Copies between device memory and pageable host memory run far below PCIe
+// speed; large ones go through two pinned buffers so that the DMA into (or
+// out of) one overlaps host threads copying the other.
+namespace psx {
+
+namespace {
+constexpr size_t kStageBytes = size_t(32) << 20;
+constexpr size_t kDirectBytes = size_t(1) << 20;
+
+void ensure_stage(Ctx* c) {
+  if (c->stage[0]) return;
+  for (int i = 0; i < 2; ++i) {
+    PSX_HIP(hipHostMalloc(&c->stage[i], kStageBytes, hipHostMallocDefault));
+    PSX_HIP(hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+  }
+}
+
+// dst(:, j) = src(:, j) for j < cols, column-major, split over host threads
+void copy_cols(double* dst, int64_t ldd, const double* src, int64_t lds, int64_t rows,
+               int64_t cols) {
+  const size_t bytes = sizeof(double) * (size_t)rows * (size_t)cols;
+  const int nt = (int)std::min<int64_t>(
+      {8, (int64_t)std::max(1u, std::thread::hardware_concurrency()), cols,
+       (int64_t)(bytes >> 22) + 1});
+  auto work = [&](int t) {
+    const int64_t j0 = cols * t / nt, j1 = cols * (t + 1) / nt;
+    if (ldd == rows && lds == rows) {
+      std::memcpy(dst + j0 * rows, src + j0 * rows, sizeof(double) * rows * (j1 - j0));
+    } else {
+      for (int64_t j = j0; j < j1; ++j)
+        std::memcpy(dst + j * ldd, src + j * lds, sizeof(double) * rows);
+    }
+  };
+  if (nt <= 1) {
+    work(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+}
+}  // namespace
+
+void release_stage(Ctx* c) {
+  for (int i = 0; i < 2; ++i) {
+    if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+    if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+    c->stage_ev[i] = nullptr;
+    c->stage[i] = nullptr;
+  }
+}
+
+void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
+                int64_t rows, int64_t cols) {
+  const size_t colb = sizeof(double) * (size_t)rows;
+  if (rows <= 0 || cols <= 0) return;
+  if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
+    PSX_HIP(hipMemcpy2DAsync(host, ldh * sizeof(double), dev, ldd * sizeof(double), colb,
+                             (size_t)cols, hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  ensure_stage(c);
+  const int64_t cpc = (int64_t)(kStageBytes / colb);  // columns per chunk
+  const int64_t nch = (cols + cpc - 1) / cpc;
+  auto issue = [&](int64_t k) {
+    const int b = (int)(k & 1);
+    const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
+    PSX_HIP(hipMemcpy2DAsync(c->stage[b], colb, dev + j0 * ldd, ldd * sizeof(double), colb,
+                             (size_t)nc, hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+  };
+  issue(0);
+  if (nch > 1) issue(1);
+  for (int64_t k = 0; k < nch; ++k) {
+    const int b = (int)(k & 1);
+    const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
+    PSX_HIP(hipEventSynchronize(c->stage_ev[b]));
+    copy_cols(host + j0 * ldh, ldh, (const double*)c->stage[b], rows, rows, nc);
+    if (k + 2 < nch) issue(k + 2);
+  }
+}
+
+void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
+                int64_t rows, int64_t cols) {
+  const size_t colb = sizeof(double) * (size_t)rows;
+  if (rows <= 0 || cols <= 0) return;
+  if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
+    PSX_HIP(hipMemcpy2DAsync(dev, ldd * sizeof(double), host, ldh * sizeof(double), colb,
+                             (size_t)cols, hipMemcpyHostToDevice, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  ensure_stage(c);
+  const int64_t cpc = (int64_t)(kStageBytes / colb);
+  const int64_t nch = (cols + cpc - 1) / cpc;
+  for (int64_t k = 0; k < nch; ++k) {
+    const int b = (int)(k & 1);
+    const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
+    if (k >= 2) PSX_HIP(hipEventSynchronize(c->stage_ev[b]));  // buffer b free again
+    copy_cols((double*)c->stage[b], rows, host + j0 * ldh, ldh, rows, nc);
+    PSX_HIP(hipMemcpy2DAsync(dev + j0 * ldd, ldd * sizeof(double), c->stage[b], colb, colb,
+                             (size_t)nc, hipMemcpyHostToDevice, c->stream));
+    PSX_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+  }
+  PSX_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace psx

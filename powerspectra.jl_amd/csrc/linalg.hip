@@ -101,17 +101,10 @@ int psx_lu_solve(psx_ctx* hctx, int n, const double* A, int64_t lda, double* B,
     double* dA = (double*)ctx_alloc(c, sizeof(double) * nn);
     double* dB = (double*)ctx_alloc(c, sizeof(double) * nb);
     try {
-      PSX_HIP(hipMemcpy2DAsync(dA, n * sizeof(double), A, lda * sizeof(double),
-                               n * sizeof(double), n, hipMemcpyHostToDevice, c->stream));
-      if (nrhs > 0)
-        PSX_HIP(hipMemcpy2DAsync(dB, n * sizeof(double), B, ldb * sizeof(double),
-                                 n * sizeof(double), nrhs, hipMemcpyHostToDevice,
-                                 c->stream));
+      h2d_matrix(c, dA, n, A, lda, n, n);
+      if (nrhs > 0) h2d_matrix(c, dB, n, B, ldb, n, nrhs);
       lu_solve_dev(c, n, dA, n, dB, n, nrhs);
-      if (nrhs > 0)
-        PSX_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double), dB, n * sizeof(double),
-                                 n * sizeof(double), nrhs, hipMemcpyDeviceToHost,
-                                 c->stream));
+      if (nrhs > 0) d2h_matrix(c, B, ldb, dB, n, n, nrhs);
       PSX_HIP(hipStreamSynchronize(c->stream));
     } catch (...) {
       (void)hipStreamSynchronize(c->stream);
@@ -134,11 +127,11 @@ int psx_decouple_covmat(psx_ctx* hctx, int n, const double* Y, const double* B1,
     const size_t nn = (size_t)n * n, b = sizeof(double) * nn;
     double* d = (double*)ctx_alloc(c, 4 * b);
     try {
-      PSX_HIP(hipMemcpyAsync(d, Y, b, hipMemcpyHostToDevice, c->stream));
-      PSX_HIP(hipMemcpyAsync(d + nn, B1, b, hipMemcpyHostToDevice, c->stream));
-      PSX_HIP(hipMemcpyAsync(d + 2 * nn, B2, b, hipMemcpyHostToDevice, c->stream));
+      h2d_matrix(c, d, n, Y, n, n, n);
+      h2d_matrix(c, d + nn, n, B1, n, n, n);
+      h2d_matrix(c, d + 2 * nn, n, B2, n, n, n);
       decouple_dev(c, n, d, d + nn, d + 2 * nn, d + 3 * nn);
-      PSX_HIP(hipMemcpyAsync(out, d + 3 * nn, b, hipMemcpyDeviceToHost, c->stream));
+      d2h_matrix(c, out, n, d + 3 * nn, n, n, n);
       PSX_HIP(hipStreamSynchronize(c->stream));
     } catch (...) {
       (void)hipStreamSynchronize(c->stream);

@@ -76,6 +76,9 @@ struct Ctx {
   size_t scratch_bytes = 0;
   // rocSOLVER/rocBLAS handle (created lazily)
   void* blas = nullptr;
+  // two pinned host staging buffers for large host <-> device copies
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 void resolve_timers(Ctx* c);
@@ -84,6 +87,16 @@ void ctx_free(Ctx* c, void* p);
 void* ctx_scratch(Ctx* c, size_t bytes);
 void ensure_tables(Ctx* c, int nmax_index);
 void release_blas(Ctx* c);
+void release_stage(Ctx* c);
+
+// Column-major matrix copies between host memory (any pageable buffer,
+// leading dimension ldh) and device memory (leading dimension ldd), in
+// stream order on the context stream; they return when the host side is
+// complete.  Large copies are staged through the context's pinned buffers.
+void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
+                int64_t rows, int64_t cols);
+void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
+                int64_t rows, int64_t cols);
 
 // Launch helper: records HIP events around the launch when timing is on.
 struct TimedLaunch {
