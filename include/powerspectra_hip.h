@@ -160,6 +160,64 @@ int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
 int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
                  double* B, int64_t ldb, int nrhs);
 
+/* ---- serving: a persistent request loop ------------------------------- */
+/* The reference has no serving layer (its callers run mcm / coupledcov one
+ * call at a time); these entry points batch those same calls for callers that
+ * issue many of them.  A server owns `slots` worker threads, each with its own
+ * HIP stream, and a device arena of `arena_bytes` from which every request's
+ * outputs and scratch are taken.  Requests are admitted in arrival order as
+ * soon as a slot is free and the arena can hold the request; admitted requests
+ * run concurrently on the device.  Spectra (mask W, signal spectra, noise
+ * ratios) are made resident once under a caller-chosen 64-bit key and are
+ * referenced by key.  Results are bit-identical to the single-call entry points
+ * (same kernels, same summation order), alone or in a mix. */
+typedef struct psx_server psx_server;
+
+enum psx_request_type {
+  PSX_REQ_MCM = 0,     /* one block, kind = psx_mcm_kind, W[0]        (psx_mcm)     */
+  PSX_REQ_MCM_PM = 1,  /* ++ and -- from W[0]; out[0] = ++, out[1] = -- (psx_mcm_pm) */
+  PSX_REQ_MCM_SET = 2, /* TT, TE, ET, ++, -- from W[0..3] = TT, TE, ET, PP (psx_mcm_set) */
+  PSX_REQ_COV = 3      /* one covariance block, kind = psx_cov_kind     (psx_cov)     */
+};
+
+typedef struct psx_request {
+  int type;            /* psx_request_type                                         */
+  int kind;            /* psx_mcm_kind (PSX_REQ_MCM) or psx_cov_kind (PSX_REQ_COV)  */
+  int lmin, lmax;
+  int64_t LW;          /* W length used; every W key must hold at least LW values  */
+  uint64_t W[8];       /* resident keys, count as for the matching entry point     */
+  uint64_t spectra[4]; /* PSX_REQ_COV: resident keys, length >= lmax + 1           */
+  uint64_t ratios[4];
+  double* out[5];      /* host outputs (ld >= lmax-lmin+1), or all NULL to keep the */
+  int64_t ld;          /* result on the device until psx_fetch / psx_release        */
+} psx_request;
+
+/* slots: concurrent requests (1..16); arena_bytes: device memory for requests
+ * (taken from the context budget); lmax_max: largest lmax accepted. */
+int psx_server_create(psx_ctx* ctx, int slots, size_t arena_bytes, int lmax_max,
+                      psx_server** out);
+/* Waits for admitted work, drops queued requests, frees everything. */
+int psx_server_destroy(psx_server* s);
+/* Copy `len` doubles to the device under `key` (replacing a previous entry of
+ * that key that no pending request uses). */
+int psx_resident_put(psx_server* s, uint64_t key, const double* values, int64_t len);
+/* PSX_ERR_ARGUMENT while a queued or running request references the key. */
+int psx_resident_evict(psx_server* s, uint64_t key);
+/* Validate and enqueue; *ticket identifies the request.  PSX_ERR_BUDGET if the
+ * request can never fit in the arena. */
+int psx_submit(psx_server* s, const psx_request* rq, uint64_t* ticket);
+/* *done = 1 once the request has finished (successfully or not). */
+int psx_poll(psx_server* s, uint64_t ticket, int* done);
+/* Block until the request has finished; *status receives its own status,
+ * times (if not NULL) {seconds queued, seconds running incl. copy-out, total}. */
+int psx_wait(psx_server* s, uint64_t ticket, int* status, double* times);
+/* Copy a device-held result (out[] all NULL at submit) into host arrays. */
+int psx_fetch(psx_server* s, uint64_t ticket, double* const* out, int64_t ld);
+/* Forget a finished ticket and free its device result, if any. */
+int psx_release(psx_server* s, uint64_t ticket);
+/* Message of the last failed server call or request status (thread-local). */
+const char* psx_server_last_error(psx_server* s);
+
 #ifdef __cplusplus
 }
 #endif

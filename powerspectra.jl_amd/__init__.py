@@ -5,7 +5,7 @@ on top of it (names as exported by the reference's src/PowerSpectra.jl:40-51).
 """
 from . import _abi
 from ._abi import (ArgumentError, AxisError, BudgetError, Context, DeviceError, DomainError,
-                   PowerSpectraError, SingularException, default_context)
+                   PowerSpectraError, Server, SingularException, default_context)
 from .alm import Alm, alm2cl
 from .blockspectralmatrix import (BlockSpectralMatrix, block_vector, getblock, hcat, hvcat, solve,
                                   spectra, vcat)
@@ -20,7 +20,7 @@ from .workspace import (ConstantDict, CovarianceWorkspace, CovField, SpectralWor
 __all__ = [
     "Alm", "ArgumentError", "AxisError", "BlockSpectralMatrix", "BudgetError", "ConstantDict",
     "Context", "CovField", "CovarianceWorkspace", "DeviceError", "DomainError", "HealpixMap",
-    "PowerSpectraError", "SingularException", "SpectralArray", "SpectralVector",
+    "PowerSpectraError", "Server", "SingularException", "SpectralArray", "SpectralVector",
     "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "block_vector", "coupledcov",
     "coupledcovTEEE", "decouple_covmat", "default_context", "getblock", "hcat", "hvcat", "inv",
     "map2alm", "mask", "mcm", "mcm_from_spectrum", "mcm_set", "nside2lmax", "nside2npix",

@@ -31,6 +31,17 @@ MCM_00, MCM_02, MCM_PP, MCM_MM = 0, 1, 2, 3
 COV_TTTT, COV_EEEE, COV_TTTE, COV_TETE, COV_TEEE, COV_TEEE_PLANCK, COV_TTEE = range(7)
 # enum psx_precision
 FP64, MIXED = 0, 1
+# enum psx_request_type
+REQ_MCM, REQ_MCM_PM, REQ_MCM_SET, REQ_COV = 0, 1, 2, 3
+
+
+class PsxRequest(ctypes.Structure):
+    """struct psx_request of include/powerspectra_hip.h."""
+    _fields_ = [("type", ctypes.c_int), ("kind", ctypes.c_int), ("lmin", ctypes.c_int),
+                ("lmax", ctypes.c_int), ("LW", ctypes.c_int64),
+                ("W", ctypes.c_uint64 * 8), ("spectra", ctypes.c_uint64 * 4),
+                ("ratios", ctypes.c_uint64 * 4), ("out", ctypes.c_void_p * 5),
+                ("ld", ctypes.c_int64)]
 
 
 class PowerSpectraError(RuntimeError):
@@ -77,6 +88,16 @@ _int = ctypes.c_int
 
 # name -> (restype, argtypes); mirrors include/powerspectra_hip.h
 SIGNATURES = {
+    "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
+    "psx_server_destroy": (_int, [_vp]),
+    "psx_resident_put": (_int, [_vp, ctypes.c_uint64, _dp, _i64]),
+    "psx_resident_evict": (_int, [_vp, ctypes.c_uint64]),
+    "psx_submit": (_int, [_vp, ctypes.POINTER(PsxRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    "psx_poll": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_int)]),
+    "psx_wait": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_int), _dp]),
+    "psx_fetch": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _i64]),
+    "psx_release": (_int, [_vp, ctypes.c_uint64]),
+    "psx_server_last_error": (ctypes.c_char_p, [_vp]),
     "psx_cov_arity": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "psx_ctx_create": (_int, [_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "psx_ctx_destroy": (_int, [_vp]),
@@ -298,6 +319,110 @@ class Context:
         self.check(self.lib.psx_lu_solve(self._h, n, dptr(A), n, dptr(X), n, X.shape[1]),
                    "psx_lu_solve")
         return X[:, 0] if vec else X
+
+
+class Server:
+    """Persistent request loop on one context (psx_server): `slots` requests
+    run concurrently on the device, admitted in arrival order as slots and
+    arena memory free up; spectra are resident under integer keys."""
+
+    _NOUT = {REQ_MCM: 1, REQ_MCM_PM: 2, REQ_MCM_SET: 5, REQ_COV: 1}
+
+    def __init__(self, ctx: "Context", slots: int = 4, arena_bytes: int = 16 << 30,
+                 lmax_max: int = 4095):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = _vp()
+        st = self.lib.psx_server_create(ctx.handle, slots, arena_bytes, lmax_max, ctypes.byref(h))
+        self._check(st, "psx_server_create", None)
+        self._h = h
+        self._pending = {}   # ticket -> (lmin, lmax, nout, host outputs)
+
+    def _check(self, st, what, h="self"):
+        if st != PSX_OK:
+            msg = self.lib.psx_server_last_error(self._h if h == "self" else None)
+            msg = msg.decode() if msg else ""
+            raise _ERRORS.get(st, PowerSpectraError)(f"{what}: {msg} (status {st})")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def put(self, key: int, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.psx_resident_put(self._h, key, dptr(v), v.size), "psx_resident_put")
+
+    def evict(self, key: int):
+        self._check(self.lib.psx_resident_evict(self._h, key), "psx_resident_evict")
+
+    def submit(self, rtype: int, kind: int = 0, lmin: int = 0, lmax: int = 0, LW: int = 0,
+               W=(), spectra=(), ratios=(), out=None) -> int:
+        """Enqueue a request; `out` = list of n x n Fortran float64 arrays to
+        receive the result, or None to keep it on the device (psx_fetch)."""
+        n = _block_size(lmin, lmax)
+        r = PsxRequest()
+        r.type, r.kind, r.lmin, r.lmax = rtype, kind, lmin, lmax
+        r.LW = LW or lmax + 1
+        for i, k in enumerate(W):
+            r.W[i] = k
+        for i, k in enumerate(spectra):
+            r.spectra[i] = k
+        for i, k in enumerate(ratios):
+            r.ratios[i] = k
+        nout = self._NOUT.get(rtype, 1)
+        if out is not None:
+            if len(out) != nout:
+                raise ArgumentError(f"request type {rtype} has {nout} outputs")
+            for i, o in enumerate(out):
+                if o.shape != (n, n) or o.dtype != np.float64 or not o.flags.f_contiguous:
+                    raise ArgumentError("outputs must be n x n Fortran-ordered float64")
+                r.out[i] = o.ctypes.data
+            r.ld = n
+        t = ctypes.c_uint64(0)
+        self._check(self.lib.psx_submit(self._h, ctypes.byref(r), ctypes.byref(t)), "psx_submit")
+        self._pending[t.value] = (lmin, lmax, nout, out)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = _int(0)
+        self._check(self.lib.psx_poll(self._h, ticket, ctypes.byref(d)), "psx_poll")
+        return bool(d.value)
+
+    def wait(self, ticket: int, raise_on_error: bool = True):
+        """Block until done; returns (status, (queued s, running s, total s))."""
+        st = _int(0)
+        times = (ctypes.c_double * 3)()
+        self._check(self.lib.psx_wait(self._h, ticket, ctypes.byref(st),
+                                      ctypes.cast(times, _dp)), "psx_wait")
+        if st.value != PSX_OK and raise_on_error:
+            self._check(st.value, f"request {ticket}")
+        return st.value, tuple(times)
+
+    def fetch(self, ticket: int):
+        """Copy a device-held result to new host arrays."""
+        if ticket not in self._pending:
+            raise ArgumentError(f"psx_fetch: unknown ticket {ticket}")
+        lmin, lmax, nout, _ = self._pending[ticket]
+        n = lmax - lmin + 1
+        outs = [np.empty((n, n), order="F") for _ in range(nout)]
+        arr = (_vp * nout)(*[o.ctypes.data for o in outs])
+        self._check(self.lib.psx_fetch(self._h, ticket, arr, n), "psx_fetch")
+        return outs
+
+    def release(self, ticket: int):
+        self._check(self.lib.psx_release(self._h, ticket), "psx_release")
+        self._pending.pop(ticket, None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.psx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def cov_arity(kind: int):

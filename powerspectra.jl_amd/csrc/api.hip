@@ -3,6 +3,45 @@
 
 #include <cstring>
 
+namespace psx {
+
+int mcm_family(int kind) {
+  switch (kind) {
+    case PSX_MCM_00: return FAM_00;
+    case PSX_MCM_02: return FAM_02;
+    case PSX_MCM_PP: return FAM_PE;
+    case PSX_MCM_MM: return FAM_PO;
+  }
+  throw Error{PSX_ERR_ARGUMENT, "unknown mcm kind (valid: 00, 02, PP, MM)"};
+}
+
+// W -> V table order for each covariance kind, and the family of each table
+// (reference argument order W1..WK of loop_cov*!).
+void cov_layout(int kind, ContractRequest& rq, const double* const* W) {
+  int nW = 0;
+  PSX_REQUIRE(psx_cov_arity(kind, &nW, nullptr, nullptr) == PSX_OK, PSX_ERR_ARGUMENT,
+              "unknown covariance kind");
+  auto put = [&](int fam, std::initializer_list<int> idx) {
+    int k = rq.nfam[0] + rq.nfam[1] + rq.nfam[2] + rq.nfam[3];
+    for (int i : idx) rq.dW[k++] = W[i];
+    rq.nfam[fam] += (int)idx.size();
+  };
+  switch (kind) {
+    case PSX_COV_TTTT: put(FAM_00, {0, 1, 2, 3, 4, 5, 6, 7}); break;
+    case PSX_COV_EEEE: put(FAM_PE, {0, 1, 2, 3, 4, 5, 6, 7}); break;
+    case PSX_COV_TTTE: put(FAM_00, {0, 1, 2, 3}); break;
+    case PSX_COV_TETE:
+      put(FAM_00, {1});
+      put(FAM_02, {0, 2, 3, 4});
+      break;
+    case PSX_COV_TEEE: put(FAM_02, {0, 1, 2, 3}); break;
+    case PSX_COV_TEEE_PLANCK: put(FAM_PE, {0, 1, 2, 3}); break;
+    case PSX_COV_TTEE: put(FAM_00, {0, 1}); break;
+  }
+}
+
+}  // namespace psx
+
 using namespace psx;
 
 namespace {
@@ -41,41 +80,6 @@ void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
 // (staged through pinned memory; returns when the host block is complete)
 void d2h_block(Ctx* c, double* host, int64_t ld, const double* dev, int n) {
   d2h_matrix(c, host, ld, dev, n, n, n);
-}
-
-int mcm_family(int kind) {
-  switch (kind) {
-    case PSX_MCM_00: return FAM_00;
-    case PSX_MCM_02: return FAM_02;
-    case PSX_MCM_PP: return FAM_PE;
-    case PSX_MCM_MM: return FAM_PO;
-  }
-  throw Error{PSX_ERR_ARGUMENT, "unknown mcm kind (valid: 00, 02, PP, MM)"};
-}
-
-// W -> V table order for each covariance kind, and the family of each table
-// (reference argument order W1..WK of loop_cov*!).
-void cov_layout(int kind, ContractRequest& rq, const double* const* W) {
-  int nW = 0;
-  PSX_REQUIRE(psx_cov_arity(kind, &nW, nullptr, nullptr) == PSX_OK, PSX_ERR_ARGUMENT,
-              "unknown covariance kind");
-  auto put = [&](int fam, std::initializer_list<int> idx) {
-    int k = rq.nfam[0] + rq.nfam[1] + rq.nfam[2] + rq.nfam[3];
-    for (int i : idx) rq.dW[k++] = W[i];
-    rq.nfam[fam] += (int)idx.size();
-  };
-  switch (kind) {
-    case PSX_COV_TTTT: put(FAM_00, {0, 1, 2, 3, 4, 5, 6, 7}); break;
-    case PSX_COV_EEEE: put(FAM_PE, {0, 1, 2, 3, 4, 5, 6, 7}); break;
-    case PSX_COV_TTTE: put(FAM_00, {0, 1, 2, 3}); break;
-    case PSX_COV_TETE:
-      put(FAM_00, {1});
-      put(FAM_02, {0, 2, 3, 4});
-      break;
-    case PSX_COV_TEEE: put(FAM_02, {0, 1, 2, 3}); break;
-    case PSX_COV_TEEE_PLANCK: put(FAM_PE, {0, 1, 2, 3}); break;
-    case PSX_COV_TTEE: put(FAM_00, {0, 1}); break;
-  }
 }
 
 }  // namespace

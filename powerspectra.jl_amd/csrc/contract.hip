@@ -638,7 +638,7 @@ int nv_of(const ContractRequest& rq) {
 
 }  // namespace
 
-static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
+static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
                                     void* scratch, size_t scratch_bytes);
 
 struct RecLayout {
@@ -664,7 +664,7 @@ size_t contraction_scratch_bytes(const ContractRequest& rq) {
   return (size_t)rec_layout(rq).rs * len * sizeof(double) + 512;
 }
 
-void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
+int* run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
                         void* scratch, size_t scratch_bytes) {
   const int nv = nv_of(rq);
   PSX_REQUIRE(nv >= 1 && nv <= kMaxV, PSX_ERR_ARGUMENT, "bad number of W spectra");
@@ -674,10 +674,10 @@ void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
   // l3 never exceeds l1 + l2 <= 2 lmax: longer W spectra are truncated
   ContractRequest rqe = rq;
   rqe.LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
-  run_contraction_checked(c, rqe, s, scratch, scratch_bytes);
+  return run_contraction_checked(c, rqe, s, scratch, scratch_bytes);
 }
 
-static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
+static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
                                     void* scratch, size_t scratch_bytes) {
   const int nv = nv_of(rq);
   const int n = rq.lmax - rq.lmin + 1;
@@ -756,6 +756,7 @@ static void run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
     PSX_REQUIRE(h == 0, PSX_ERR_DOMAIN,
                 "sqrt of a negative spectrum product (Julia DomainError)");
   }
+  return rq.epi == EPI_COV ? err : nullptr;
 }
 
 void run_contraction(Ctx* c, const ContractRequest& rq) {

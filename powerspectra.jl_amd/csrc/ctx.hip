@@ -96,7 +96,8 @@ void ensure_tables(Ctx* c, int nmax_index) {
     D[i] = (double)(1.0L / ((long double)(2 * m + 1) * g));
     IG[i] = (double)(1.0L / g);
   }
-  PSX_HIP(hipStreamSynchronize(c->stream));
+  // other streams (a server's slots) may read the old tables
+  PSX_HIP(hipDeviceSynchronize());
   if (t.dG) {
     ctx_free(c, t.dG);
     ctx_free(c, t.dZ);
@@ -256,7 +257,6 @@ int psx_memcpy_d2h(psx_ctx* h, void* dst, const void* src, size_t bytes) {
 namespace psx {
 
 namespace {
-constexpr size_t kStageBytes = size_t(32) << 20;
 constexpr size_t kDirectBytes = size_t(1) << 20;
 
 void ensure_stage(Ctx* c) {
@@ -303,35 +303,40 @@ void release_stage(Ctx* c) {
   }
 }
 
-void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
-                int64_t rows, int64_t cols) {
+void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
+                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols) {
   const size_t colb = sizeof(double) * (size_t)rows;
   if (rows <= 0 || cols <= 0) return;
   if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
     PSX_HIP(hipMemcpy2DAsync(host, ldh * sizeof(double), dev, ldd * sizeof(double), colb,
-                             (size_t)cols, hipMemcpyDeviceToHost, c->stream));
-    PSX_HIP(hipStreamSynchronize(c->stream));
+                             (size_t)cols, hipMemcpyDeviceToHost, s));
+    PSX_HIP(hipStreamSynchronize(s));
     return;
   }
-  ensure_stage(c);
   const int64_t cpc = (int64_t)(kStageBytes / colb);  // columns per chunk
   const int64_t nch = (cols + cpc - 1) / cpc;
   auto issue = [&](int64_t k) {
     const int b = (int)(k & 1);
     const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
-    PSX_HIP(hipMemcpy2DAsync(c->stage[b], colb, dev + j0 * ldd, ldd * sizeof(double), colb,
-                             (size_t)nc, hipMemcpyDeviceToHost, c->stream));
-    PSX_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+    PSX_HIP(hipMemcpy2DAsync(stage[b], colb, dev + j0 * ldd, ldd * sizeof(double), colb,
+                             (size_t)nc, hipMemcpyDeviceToHost, s));
+    PSX_HIP(hipEventRecord(ev[b], s));
   };
   issue(0);
   if (nch > 1) issue(1);
   for (int64_t k = 0; k < nch; ++k) {
     const int b = (int)(k & 1);
     const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
-    PSX_HIP(hipEventSynchronize(c->stage_ev[b]));
-    copy_cols(host + j0 * ldh, ldh, (const double*)c->stage[b], rows, rows, nc);
+    PSX_HIP(hipEventSynchronize(ev[b]));
+    copy_cols(host + j0 * ldh, ldh, (const double*)stage[b], rows, rows, nc);
     if (k + 2 < nch) issue(k + 2);
   }
+}
+
+void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
+                int64_t rows, int64_t cols) {
+  if ((size_t)rows * (size_t)cols * sizeof(double) > kDirectBytes) ensure_stage(c);
+  d2h_matrix_on(c->stream, c->stage, c->stage_ev, host, ldh, dev, ldd, rows, cols);
 }
 
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,

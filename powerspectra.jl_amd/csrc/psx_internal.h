@@ -95,6 +95,11 @@ void release_stage(Ctx* c);
 // complete.  Large copies are staged through the context's pinned buffers.
 void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
                 int64_t rows, int64_t cols);
+// the same on stream s through two caller-owned pinned buffers of
+// kStageBytes each and their events
+constexpr size_t kStageBytes = size_t(32) << 20;
+void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
+                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols);
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
                 int64_t rows, int64_t cols);
 
@@ -140,9 +145,17 @@ struct ContractRequest {
 };
 
 void run_contraction(Ctx* c, const ContractRequest& rq);
-void run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
+// Enqueue a contraction on stream s with caller-provided scratch (at least
+// contraction_scratch_bytes(rq)).  Returns the device address of the
+// covariance domain-error flag inside the scratch (nonzero after the kernel
+// when a spectrum product under a square root was negative), or nullptr for
+// mode-coupling requests.  On the context stream the flag is checked here.
+int* run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
                         void* scratch, size_t scratch_bytes);
 size_t contraction_scratch_bytes(const ContractRequest& rq);
+// request building shared by the entry points and the server
+int mcm_family(int kind);
+void cov_layout(int kind, ContractRequest& rq, const double* const* W);
 
 // ---- linear algebra ------------------------------------------------------
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB,
