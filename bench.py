@@ -317,6 +317,145 @@ def run_e2e(args, pkg) -> dict:
                  kernel_ms={"psx_mcm_set": ks["psx_mcm_set"]})
 
 
+def serve_requests(n_req: int, seed: int, lmin_: int = 511, lmax_: int = 4095):
+    """The serving stream: each request is an MCM (spin pair TT, TE, EE or BB;
+    EE and BB need the ++ and -- blocks) or a covariance block (TTTT, EEEE,
+    TETE, TTEE) with lmax log-uniform in [lmin_, lmax_]; its mask spectra are
+    taken from a resident pool by a per-request seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_req):
+        lmax = int(np.exp(rng.uniform(np.log(lmin_), np.log(lmax_ + 1))))
+        lmax = min(max(lmax, lmin_), lmax_)
+        if rng.uniform() < 0.5:
+            out.append(("mcm", ["TT", "TE", "EE", "BB"][rng.integers(4)], lmax,
+                        int(rng.integers(1 << 30))))
+        else:
+            out.append(("cov", ["TTTT", "EEEE", "TETE", "TTEE"][rng.integers(4)], lmax,
+                        int(rng.integers(1 << 30))))
+    return out
+
+
+def run_serve(args, pkg) -> dict:
+    """Continuous batching through psx_server: saturation throughput, then a
+    Poisson stream at 80 % of it; latency of every request in the mix against
+    the same request alone.  Results stay on the device (released after
+    completion)."""
+    import threading
+
+    abi = pkg._abi
+    ctx = pkg.Context(0)
+    n_req = 2000 if args.requests == 0 else args.requests
+    LMAX = 4095
+    srv = pkg.Server(ctx, slots=args.slots, arena_bytes=48 << 30, lmax_max=LMAX)
+    npool = 64
+    ell = np.arange(LMAX + 1, dtype=np.float64)
+    for k in range(npool):                                     # mask spectra pool
+        srv.put(k, synthetic_W(LMAX + 1, 100 + k, cross=(k % 3 == 1)))
+    rng = np.random.default_rng(7)
+    for k in range(16):                                        # signal spectra, ratios
+        srv.put(1000 + k, 1e3 * rng.uniform(0.5, 2) * (ell + 10.0) ** -2.5
+                * np.exp(-(ell / 2000.0) ** 2))
+        srv.put(2000 + k, np.sqrt(1.0 + rng.uniform(0, 2) * (ell / (LMAX + 1.0)) ** 2))
+    cov_kind = {"TTTT": abi.COV_TTTT, "EEEE": abi.COV_EEEE, "TETE": abi.COV_TETE,
+                "TTEE": abi.COV_TTEE}
+
+    def args_of(req):
+        what, spec, lmax, sd = req
+        r = np.random.default_rng(sd)
+        pick = lambda k: [int(x) for x in r.integers(npool, size=k)]
+        if what == "mcm":
+            if spec == "TT":
+                return dict(rtype=abi.REQ_MCM, kind=abi.MCM_00, lmax=lmax, W=pick(1))
+            if spec == "TE":
+                return dict(rtype=abi.REQ_MCM, kind=abi.MCM_02, lmax=lmax, W=pick(1))
+            return dict(rtype=abi.REQ_MCM_PM, lmax=lmax, W=pick(1))
+        kind = cov_kind[spec]
+        nW, nS, nR = abi.cov_arity(kind)
+        return dict(rtype=abi.REQ_COV, kind=kind, lmax=lmax, W=pick(nW),
+                    spectra=[1000 + int(x) for x in r.integers(16, size=nS)],
+                    ratios=[2000 + int(x) for x in r.integers(16, size=nR)])
+
+    reqs = [args_of(q) for q in serve_requests(n_req, seed=11)]
+    # warm the tile lists and kernels of every (kind, lmax) once, untimed
+    warm = [srv.submit(**r) for r in reqs]
+    for t in warm:
+        srv.wait(t)
+        srv.release(t)
+
+    # 1. each request alone
+    alone = np.zeros(n_req)
+    for i, r in enumerate(reqs):
+        t = srv.submit(**r)
+        alone[i] = srv.wait(t)[1][2]
+        srv.release(t)
+
+    # 2. saturation: everything submitted at once
+    t0 = time.perf_counter()
+    ts = [srv.submit(**r) for r in reqs]
+    for t in ts:
+        srv.wait(t)
+        srv.release(t)
+    sat = n_req / (time.perf_counter() - t0)
+
+    # 3. Poisson arrivals at a fraction of saturation, per admission policy
+    def poisson(rate, policy, slack=2.0, seed=3):
+        srv.set_policy(policy, slack)
+        gaps = np.random.default_rng(seed).exponential(1.0 / rate, size=n_req)
+        tickets = [None] * n_req
+        ready = threading.Semaphore(0)
+        lat = np.zeros(n_req)
+        qtime = np.zeros(n_req)
+
+        def waiter():
+            for i in range(n_req):
+                ready.acquire()
+                _, tm = srv.wait(tickets[i])
+                qtime[i], lat[i] = tm[0], tm[2]
+                srv.release(tickets[i])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        due = start
+        for i, r in enumerate(reqs):
+            due += gaps[i]
+            while True:
+                d = due - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.002))
+            tickets[i] = srv.submit(**r)
+            ready.release()
+        th.join()
+        span = time.perf_counter() - start
+        slow = lat / alone
+        pct = lambda x, q: float(np.percentile(x, q))
+        return {"policy": "deadline" if policy == abi.SCHED_DEADLINE else "fifo",
+                "slack": slack if policy == abi.SCHED_DEADLINE else None,
+                "offered_req_per_s": rate, "achieved_req_per_s": n_req / span,
+                "latency_p50_ms": pct(lat, 50) * 1e3, "latency_p99_ms": pct(lat, 99) * 1e3,
+                "queue_p99_ms": pct(qtime, 99) * 1e3,
+                "slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99),
+                "p99_over_alone_p99": pct(lat, 99) / pct(alone, 99)}
+
+    runs = [poisson(0.8 * sat, abi.SCHED_FIFO), poisson(0.8 * sat, abi.SCHED_DEADLINE, 2.0),
+            poisson(0.8 * sat, abi.SCHED_DEADLINE, 4.0), poisson(0.5 * sat, abi.SCHED_DEADLINE, 2.0)]
+    srv.close()
+    ctx.close()
+    head = runs[1]
+    pct = lambda x, q: float(np.percentile(x, q))
+    return _line("serving req/s + p99 latency (MCM and covariance requests, lmax 511-4095)",
+                 head["achieved_req_per_s"], "req/s", args, 1e3 / head["achieved_req_per_s"],
+                 {"workload": f"serve_{n_req}: Poisson arrivals at 80 % of saturation; half MCM "
+                              "(TT, TE, EE, BB), half covariance (TTTT, EEEE, TETE, TTEE); lmax "
+                              "log-uniform in [511, 4095]; results kept on the device",
+                  "slots": args.slots, "requests": n_req},
+                 serving={"saturation_req_per_s": sat,
+                          "alone_p50_ms": pct(alone, 50) * 1e3, "alone_p99_ms": pct(alone, 99) * 1e3,
+                          "runs": runs})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,8 +465,10 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
-                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e"],
+                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -342,7 +483,7 @@ def main() -> int:
     pkg = load_package()
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
-              "e2e": run_e2e}[args.workload]
+              "e2e": run_e2e, "serve": run_serve}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]

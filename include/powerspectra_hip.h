@@ -198,6 +198,14 @@ int psx_server_create(psx_ctx* ctx, int slots, size_t arena_bytes, int lmax_max,
                       psx_server** out);
 /* Waits for admitted work, drops queued requests, frees everything. */
 int psx_server_destroy(psx_server* s);
+/* Admission order.  PSX_SCHED_FIFO (default): arrival order.
+ * PSX_SCHED_DEADLINE: earliest deadline first, deadline = arrival time +
+ * slack x the request's estimated run time alone (from its size and kind), so
+ * small requests are not held behind large ones while no request waits
+ * indefinitely.  Either way the request chosen waits for arena space before
+ * any other is admitted. */
+enum psx_sched { PSX_SCHED_FIFO = 0, PSX_SCHED_DEADLINE = 1 };
+int psx_server_set_policy(psx_server* s, int policy, double slack);
 /* Copy `len` doubles to the device under `key` (replacing a previous entry of
  * that key that no pending request uses). */
 int psx_resident_put(psx_server* s, uint64_t key, const double* values, int64_t len);

@@ -33,6 +33,8 @@ COV_TTTT, COV_EEEE, COV_TTTE, COV_TETE, COV_TEEE, COV_TEEE_PLANCK, COV_TTEE = ra
 FP64, MIXED = 0, 1
 # enum psx_request_type
 REQ_MCM, REQ_MCM_PM, REQ_MCM_SET, REQ_COV = 0, 1, 2, 3
+# enum psx_sched
+SCHED_FIFO, SCHED_DEADLINE = 0, 1
 
 
 class PsxRequest(ctypes.Structure):
@@ -90,6 +92,7 @@ _int = ctypes.c_int
 SIGNATURES = {
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
     "psx_server_destroy": (_int, [_vp]),
+    "psx_server_set_policy": (_int, [_vp, _int, ctypes.c_double]),
     "psx_resident_put": (_int, [_vp, ctypes.c_uint64, _dp, _i64]),
     "psx_resident_evict": (_int, [_vp, ctypes.c_uint64]),
     "psx_submit": (_int, [_vp, ctypes.POINTER(PsxRequest), ctypes.POINTER(ctypes.c_uint64)]),
@@ -347,6 +350,9 @@ class Server:
     @property
     def handle(self):
         return self._h
+
+    def set_policy(self, policy: int, slack: float = 2.0):
+        self._check(self.lib.psx_server_set_policy(self._h, policy, slack), "psx_server_set_policy")
 
     def put(self, key: int, values):
         v = np.ascontiguousarray(values, dtype=np.float64)

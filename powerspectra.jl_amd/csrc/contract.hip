@@ -527,6 +527,7 @@ struct CachedTiles {
   Tile* d = nullptr;
   int n = 0;
   std::vector<Tile> host;
+  Geometry geo;  // checked once per tile list (the table size is checked per call)
 };
 
 std::map<std::pair<void*, TileKey>, CachedTiles> g_tile_cache;
@@ -539,6 +540,7 @@ const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int wave
   if (it != g_tile_cache.end()) return it->second;
   CachedTiles ct;
   ct.host = make_tiles(lmin, lmax, LW, R, waves);
+  ct.geo = check_geometry(ct.host, lmax, LW, R, 0);
   ct.n = (int)ct.host.size();
   ct.d = (Tile*)ctx_alloc(c, sizeof(Tile) * std::max(1, ct.n));
   PSX_HIP(hipMemcpy(ct.d, ct.host.data(), sizeof(Tile) * ct.n, hipMemcpyHostToDevice));
@@ -685,7 +687,9 @@ static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   const Config cf = pick_config(rq);
   const CachedTiles& tl = get_tiles(c, rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves);
   ensure_tables(c, (int)rq.LW + 2 * rq.lmax + 512);
-  const Geometry g = check_geometry(tl.host, rq.lmax, (int)rq.LW, cf.R, c->tables.n);
+  const Geometry& g = tl.geo;
+  PSX_REQUIRE(g.tab_need + kTabOff < c->tables.n, PSX_ERR_ARGUMENT,
+              "tile geometry: table overflow");
 
   // scratch: [record table][err flag]
   const RecLayout rl = rec_layout(rq);

@@ -8,8 +8,11 @@
 // only, copies the result out if the caller gave host outputs, and frees the
 // memory.  Several requests therefore run on the GPU concurrently, and a
 // request that arrives while others run is admitted as soon as a slot and
-// arena space free up.  Admission is strictly in arrival order (the head
-// waits for space, later requests do not overtake it), so no request starves.
+// arena space free up.  The request to admit next is the oldest (FIFO) or the
+// one with the earliest deadline (arrival + slack x estimated run time); it
+// waits for arena space and no other request overtakes it meanwhile, so no
+// request starves.  Requests big enough to fill the GPU alone run one at a
+// time: while one runs, the next small request is admitted instead.
 //
 // Spectra live on the device under caller-chosen 64-bit keys (put once,
 // referenced by many requests; reference counted while requests use them).
@@ -106,6 +109,8 @@ struct Job {
   char* scratch_mem = nullptr;
   std::vector<uint64_t> keys;
   Clock::time_point t_submit, t_start, t_done;
+  double est = 0.0;       // estimated seconds alone
+  double deadline = 0.0;  // seconds since the server started (PSX_SCHED_DEADLINE)
   int status = PSX_OK;
   std::string msg;
   bool done = false;
@@ -133,7 +138,18 @@ struct Server {
   std::unordered_map<uint64_t, Resident> resident;
   uint64_t next_ticket = 1;
   bool stop = false;
+  int policy = PSX_SCHED_FIFO;
+  double slack = 2.0;
+  Clock::time_point t0 = Clock::now();
+  // requests whose kernels fill the GPU alone run one at a time (two of them
+  // sharing the device would each take twice as long); smaller ones run
+  // beside them in the other slots
+  int big_running = 0;
 };
+
+// A request is "big" when its kernels alone keep every CU busy for longer
+// than this (estimated seconds).
+constexpr double kBigSeconds = 5e-4;
 
 namespace {
 
@@ -175,7 +191,22 @@ void worker(Server* S, Slot* sl) {
       for (;;) {
         if (S->stop) return;
         if (!S->queue.empty()) {
-          auto& h = S->queue.front();
+          // next request by policy; while a big request runs, the next small
+          // one (big ones wait their turn, ahead of later big ones)
+          auto better = [&](const std::shared_ptr<Job>& x, const std::shared_ptr<Job>& y) {
+            return S->policy == PSX_SCHED_DEADLINE ? x->deadline < y->deadline
+                                                   : x->ticket < y->ticket;
+          };
+          auto pick = S->queue.end();
+          for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
+            if (S->big_running > 0 && (*it)->est > kBigSeconds) continue;
+            if (pick == S->queue.end() || better(*it, *pick)) pick = it;
+          }
+          if (pick == S->queue.end()) {
+            S->cv_work.wait(lk);
+            continue;
+          }
+          auto& h = *pick;
           char* a = S->arena.alloc(h->scratch_bytes);
           if (a) {
             char* b = S->arena.alloc(h->out_bytes);
@@ -183,7 +214,8 @@ void worker(Server* S, Slot* sl) {
               h->scratch_mem = a;
               h->out_mem = b;
               j = h;
-              S->queue.pop_front();
+              S->queue.erase(pick);
+              if (j->est > kBigSeconds) S->big_running++;
               break;
             }
             S->arena.release(a);
@@ -221,6 +253,7 @@ void worker(Server* S, Slot* sl) {
         auto it = S->resident.find(k);
         if (it != S->resident.end()) it->second.refs--;
       }
+      if (j->est > kBigSeconds) S->big_running--;
       j->done = true;
     }
     S->cv_done.notify_all();
@@ -316,6 +349,16 @@ std::shared_ptr<Job> build_job(Server* S, const psx_request& r) {
     PSX_REQUIRE(r.ld >= j->n, PSX_ERR_AXES, "leading dimension smaller than lmax-lmin+1");
   j->out_bytes = (size_t)j->nout * j->n * (size_t)j->n * sizeof(double);
   j->scratch_bytes = contraction_scratch_bytes(rq);
+  // run time alone: ~n^3/8 (l1 <= l2, c) steps at the FP64 rate the
+  // contraction kernels sustain, ~1.6 VALU ops per step per table, plus the
+  // launch and copy-back overheads
+  {
+    const double n3 = (double)j->n * j->n * j->n;
+    const int nv = rq.nfam[0] + rq.nfam[1] + rq.nfam[2] + rq.nfam[3];
+    const double ops = (2.0 + 2.5 * nv) * n3 / 8.0;
+    j->est = ops / 1.6e13 + 3e-5 +
+             (j->host_out ? (double)j->out_bytes / 3e10 : 0.0);
+  }
   PSX_REQUIRE(Arena::granule(j->out_bytes) + Arena::granule(j->scratch_bytes) <= S->arena.size(),
               PSX_ERR_BUDGET, "request does not fit in the server arena");
   return j;
@@ -440,6 +483,18 @@ int psx_server_destroy(psx_server* hs) {
   return PSX_OK;
 }
 
+int psx_server_set_policy(psx_server* hs, int policy, double slack) {
+  Server* S = reinterpret_cast<Server*>(hs);
+  return sguard(S, [&] {
+    PSX_REQUIRE(policy == PSX_SCHED_FIFO || policy == PSX_SCHED_DEADLINE, PSX_ERR_ARGUMENT,
+                "unknown scheduling policy");
+    PSX_REQUIRE(slack > 0.0, PSX_ERR_ARGUMENT, "slack must be positive");
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->policy = policy;
+    S->slack = slack;
+  });
+}
+
 int psx_resident_put(psx_server* hs, uint64_t key, const double* values, int64_t len) {
   Server* S = reinterpret_cast<Server*>(hs);
   return sguard(S, [&] {
@@ -484,6 +539,7 @@ int psx_submit(psx_server* hs, const psx_request* rq, uint64_t* ticket) {
       auto j = build_job(S, *rq);
       j->ticket = S->next_ticket++;
       j->t_submit = Clock::now();
+      j->deadline = secs(S->t0, j->t_submit) + S->slack * j->est;
       for (uint64_t k : j->keys) S->resident[k].refs++;
       S->jobs[j->ticket] = j;
       S->queue.push_back(j);
