@@ -68,7 +68,8 @@ int psx_cov_arity(int kind, int* nW, int* nspectra, int* nratios);
 
 enum psx_precision {
   PSX_FP64 = 0,   /* all arithmetic in FP64                                   */
-  PSX_MIXED = 1   /* covariance contraction on MFMA, 3j terms and epilogue FP64 */
+  PSX_MIXED = 1   /* reserved: FP32-MFMA covariance contraction; not available,
+                     psx_cov returns PSX_ERR_ARGUMENT                          */
 };
 
 /* ---- context ----------------------------------------------------------- */
@@ -139,7 +140,7 @@ int psx_mcm_set_d(psx_ctx* ctx, int lmin, int lmax, int64_t LW,
  * pointers in exactly the reference loop's argument order; psx_cov_arity
  * gives their counts.  Spectra and ratios are 0-based with length >= lmax+1,
  * W has length LW (reference: workspace.lmax+1).  Fills C[l1,l2] = C[l2,l1]
- * for lmin <= l1 <= l2 <= lmax.  `precision` is PSX_FP64 or PSX_MIXED. */
+ * for lmin <= l1 <= l2 <= lmax.  `precision` must be PSX_FP64. */
 int psx_cov(psx_ctx* ctx, int kind, int lmin, int lmax,
             const double* const* spectra, const double* const* ratios,
             const double* const* W, int64_t LW, int precision,
