@@ -10,8 +10,9 @@ from .alm import Alm, alm2cl
 from .blockspectralmatrix import (BlockSpectralMatrix, block_vector, getblock, hcat, hvcat, solve,
                                   spectra, vcat)
 from .covariance import coupledcov, coupledcovTEEE, decouple_covmat
+from .master import binning_matrix, maskedalm2spectra, master
 from .modecoupling import mask, mcm, mcm_from_spectrum, mcm_set, scale
-from .sht import HealpixMap, alm2map, map2alm, nside2lmax, nside2npix
+from .sht import HealpixMap, alm2map, alm2map_pol, map2alm, map2alm_pol, nside2lmax, nside2npix
 from .spectralarray import (SpectralArray, SpectralVector, inv, parent, spectralones,
                             spectralzeros)
 from .workspace import (ConstantDict, CovarianceWorkspace, CovField, SpectralWorkspace,
@@ -21,8 +22,10 @@ __all__ = [
     "Alm", "ArgumentError", "AxisError", "BlockSpectralMatrix", "BudgetError", "ConstantDict",
     "Context", "CovField", "CovarianceWorkspace", "DeviceError", "DomainError", "HealpixMap",
     "PowerSpectraError", "Server", "SingularException", "SpectralArray", "SpectralVector",
-    "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "block_vector", "coupledcov",
+    "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "alm2map_pol", "binning_matrix",
+    "block_vector", "coupledcov",
     "coupledcovTEEE", "decouple_covmat", "default_context", "getblock", "hcat", "hvcat", "inv",
-    "map2alm", "mask", "mcm", "mcm_from_spectrum", "mcm_set", "nside2lmax", "nside2npix",
+    "map2alm", "map2alm_pol", "mask", "maskedalm2spectra", "master", "mcm", "mcm_from_spectrum",
+    "mcm_set", "nside2lmax", "nside2npix",
     "parent", "scale", "solve", "spectra", "spectralones", "spectralzeros", "vcat",
 ]

@@ -161,3 +161,135 @@ def map2alm(m, lmax: int | None = None, niter: int = 3) -> Alm:
         resid = hm.pixels - alm2map(alm, hm.nside).pixels
         alm = Alm(lmax, lmax, alm.alm + _map2alm_once(resid, info, lmax).alm)
     return alm
+
+
+# ---- spin 2 (polarisation) -----------------------------------------------------
+# Convention of HEALPix (and Healpix.jl): (Q +- iU)(n) = sum_lm _{+-2}a_lm
+# _{+-2}Y_lm(n), a_E = -(_2a + _-2a)/2, a_B = i(_2a - _-2a)/2, with
+# _sY_lm(theta, phi) = (-1)^s sqrt((2l+1)/(4 pi)) d^l_{m,-s}(theta) e^{i m phi}.
+# E.g. a pure E mode a_E(2,0) = 1 gives Q = -sqrt(15/(32 pi)) sin^2 theta, U = 0.
+
+def _d_rows(m: int, mp: int, lmax: int, cth: np.ndarray) -> np.ndarray:
+    """Wigner d^l_{m,mp}(theta) for l = 0..lmax on every ring (zero for
+    l < max(|m|, |mp|)), by the three-term recurrence in l; m >= 0, |mp| <= 2."""
+    out = np.zeros((lmax + 1, cth.size))
+    l0 = max(m, abs(mp))
+    if l0 > lmax:
+        return out
+    c2 = np.sqrt(np.maximum(0.0, 0.5 * (1.0 + cth)))   # cos(theta/2)
+    s2 = np.sqrt(np.maximum(0.0, 0.5 * (1.0 - cth)))   # sin(theta/2)
+    lg = lambda x: float(np.sum(np.log(np.arange(1, x + 1)))) if x > 0 else 0.0
+
+    def start(j, a, b):
+        # d^j_{j,b} = (-1)^(j-b) sqrt((2j)! / ((j+b)! (j-b)!)) c^(j+b) s^(j-b)
+        coef = 0.5 * (lg(2 * j) - lg(j + b) - lg(j - b))
+        with np.errstate(divide="ignore"):
+            v = coef + (j + b) * np.log(c2) + (j - b) * np.log(s2)
+        return (-1.0) ** (j - b) * np.exp(v)
+
+    if m >= abs(mp):
+        d0 = start(m, m, mp)
+    elif mp > 0:                       # d^l_{m,mp} = (-1)^(m-mp) d^l_{mp,m}
+        d0 = (-1.0) ** (m - mp) * start(mp, mp, m)
+    else:                              # d^l_{m,mp} = d^l_{-mp,-m}
+        d0 = start(-mp, -mp, -m)
+    out[l0] = d0
+    prev = np.zeros_like(d0)
+    cur = d0
+    for l in range(l0, lmax):
+        if l == 0:
+            nxt = cth * cur            # m = mp = 0 is not needed for spin 2
+        else:
+            a = (2 * l + 1) * (l * (l + 1) * cth - m * mp)
+            b = (l + 1) * np.sqrt(max(0.0, (l * l - m * m) * (l * l - mp * mp)))
+            den = l * np.sqrt(((l + 1) ** 2 - m * m) * ((l + 1) ** 2 - mp * mp))
+            nxt = (a * cur - b * prev) / den
+        out[l + 1] = nxt
+        prev, cur = cur, nxt
+    return out
+
+
+def _ring_phases_c(info: RingInfo, z: np.ndarray, mmax: int) -> np.ndarray:
+    """F[k, m] = sum_j z_j e^{-i m phi_j} for a complex ring field z."""
+    nr = info.first.size
+    F = np.zeros((nr, mmax + 1), dtype=np.complex128)
+    ms = np.arange(mmax + 1)
+    for k in range(nr):
+        n = int(info.count[k])
+        ring = z[info.first[k]: info.first[k] + n]
+        f = np.fft.fft(ring)
+        F[k] = f[ms % n] * np.exp(-1j * ms * info.phi0[k])
+    return F
+
+
+def _pol_analysis_once(Q, U, info: RingInfo, lmax: int):
+    w = 4 * np.pi / Q.size
+    Fp = _ring_phases_c(info, Q + 1j * U, lmax)
+    Fm = _ring_phases_c(info, Q - 1j * U, lmax)
+    aE = np.zeros(num_alms(lmax), dtype=np.complex128)
+    aB = np.zeros(num_alms(lmax), dtype=np.complex128)
+    norm = np.sqrt((2 * np.arange(lmax + 1) + 1) / (4 * np.pi))
+    for m in range(lmax + 1):
+        dp = _d_rows(m, -2, lmax, info.cth) * norm[:, None]   # _2Y  (s = 2): d^l_{m,-2}
+        dm = _d_rows(m, 2, lmax, info.cth) * norm[:, None]    # _-2Y (s = -2): d^l_{m,2}
+        a2 = w * (dp @ Fp[:, m])
+        am2 = w * (dm @ Fm[:, m])
+        i0 = alm_index(lmax, m, m)
+        sl = slice(i0, i0 + lmax - m + 1)
+        aE[sl] = (-(a2 + am2) / 2)[m:]
+        aB[sl] = (1j * (a2 - am2) / 2)[m:]
+    for a in (aE, aB):   # l < 2 carry no spin-2 power
+        for m in range(min(2, lmax + 1)):
+            for l in range(m, min(2, lmax + 1)):
+                a[alm_index(lmax, l, m)] = 0.0
+    return aE, aB
+
+
+def alm2map_pol(almT: Alm, almE: Alm, almB: Alm, nside: int):
+    """(I, Q, U) maps from T, E, B coefficients (HEALPix convention)."""
+    info = RingInfo(nside)
+    lmax = almE.lmax
+    nr = info.first.size
+    I = alm2map(almT, nside).pixels
+    Gp = np.zeros((nr, lmax + 1), dtype=np.complex128)
+    Gm = np.zeros((nr, lmax + 1), dtype=np.complex128)
+    norm = np.sqrt((2 * np.arange(lmax + 1) + 1) / (4 * np.pi))
+    for m in range(lmax + 1):
+        aE = np.zeros(lmax + 1, dtype=np.complex128)
+        aB = np.zeros(lmax + 1, dtype=np.complex128)
+        aE[m:] = almE.m_slice(m)
+        aB[m:] = almB.m_slice(m)
+        a2 = -(aE + 1j * aB)
+        am2 = -(aE - 1j * aB)
+        dp = _d_rows(m, -2, lmax, info.cth) * norm[:, None]
+        dm = _d_rows(m, 2, lmax, info.cth) * norm[:, None]
+        Gp[:, m] = a2 @ dp
+        Gm[:, m] = am2 @ dm
+    P = np.zeros(nside2npix(nside), dtype=np.complex128)
+    for k in range(nr):
+        n = int(info.count[k])
+        phi = info.phi0[k] + 2 * np.pi * np.arange(n) / n
+        ms = np.arange(0, lmax + 1)
+        E = np.exp(1j * np.outer(phi, ms))
+        val = E @ Gp[k] + np.conj(E[:, 1:]) @ np.conj(Gm[k, 1:])
+        P[info.first[k]: info.first[k] + n] = val
+    return HealpixMap(I, nside), HealpixMap(P.real.copy(), nside), HealpixMap(P.imag.copy(), nside)
+
+
+def map2alm_pol(I, Q, U, lmax: int | None = None, niter: int = 3):
+    """(T, E, B) coefficients of an (I, Q, U) map with `niter` Jacobi
+    refinements (Healpix.jl's polarised map2alm)."""
+    Im = I if isinstance(I, HealpixMap) else HealpixMap(I)
+    Qp = np.asarray(Q.pixels if isinstance(Q, HealpixMap) else Q, dtype=np.float64)
+    Up = np.asarray(U.pixels if isinstance(U, HealpixMap) else U, dtype=np.float64)
+    ns = Im.nside
+    lmax = nside2lmax(ns) if lmax is None else int(lmax)
+    info = RingInfo(ns)
+    T = map2alm(Im, lmax=lmax, niter=niter)
+    aE, aB = _pol_analysis_once(Qp, Up, info, lmax)
+    for _ in range(niter):
+        _, q, u = alm2map_pol(Alm(lmax), Alm(lmax, lmax, aE), Alm(lmax, lmax, aB), ns)
+        dE, dB = _pol_analysis_once(Qp - q.pixels, Up - u.pixels, info, lmax)
+        aE = aE + dE
+        aB = aB + dB
+    return T, Alm(lmax, lmax, aE), Alm(lmax, lmax, aB)

@@ -139,3 +139,77 @@ def test_decouple_covmat_mirror(pkg, masks):
     assert Cd.axes() == Cc.axes()
     back = M.parent @ Cd.parent @ M.parent.T
     assert np.linalg.norm(back - Cc.parent) / np.linalg.norm(Cc.parent) < 1e-9
+
+
+def test_maskedalm2spectra_block_solves(pkg, ctx, masks):
+    """maskedalm2spectra (src/modecoupling.jl:343-380): each spectrum is the
+    solve of the right block against the right pseudo-spectrum."""
+    from powerspectra_amd.master import maskedalm2spectra
+    m1, m2 = masks
+    aT1, aT2 = pkg.map2alm(m1), pkg.map2alm(m2)
+    aP1, aP2 = pkg.map2alm(m2), pkg.map2alm(m1)           # different T and P masks
+    lmax = aT1.lmax
+    rng = np.random.default_rng(5)
+
+    def rand_alm():
+        a = pkg.Alm(lmax)
+        a.alm[:] = rng.standard_normal(a.alm.size) + 1j * rng.standard_normal(a.alm.size)
+        return a
+
+    X1 = [rand_alm() for _ in range(3)]
+    X2 = [rand_alm() for _ in range(3)]
+    lmin = 2
+    out = maskedalm2spectra(X1, aT1, aP1, X2, aT2, aP2, lmin=lmin)
+    W = lambda a, b: pkg.alm2cl(a, b)[: lmax + 1]
+    cl = lambda a, b: pkg.alm2cl(a, b)[lmin: lmax + 1]
+    M = lambda k, a, b: ctx.mcm(k, lmin, lmax, W(a, b))
+    T, E, B = 0, 1, 2
+    for XY, k, (a, b), (x, y) in (("TT", 0, (aT1, aT2), (T, T)), ("TE", 1, (aT1, aP2), (T, E)),
+                                  ("ET", 1, (aP1, aT2), (E, T)), ("TB", 1, (aT1, aP2), (T, B)),
+                                  ("BT", 1, (aP1, aT2), (B, T))):
+        want = np.linalg.solve(M(k, a, b), cl(X1[x], X2[y]))
+        got = out[XY]
+        assert got.firstindex() == lmin
+        assert np.abs(got.parent - want).max() <= 1e-8 * np.abs(want).max()
+    Mpp, Mmm = ctx.mcm_pm(lmin, lmax, W(aP1, aP2))
+    A = np.block([[Mpp, Mmm], [Mmm, Mpp]])
+    want = np.linalg.solve(A, np.concatenate([cl(X1[E], X2[E]), cl(X1[B], X2[B])]))
+    n = lmax - lmin + 1
+    assert np.abs(out["EE"].parent - want[:n]).max() <= 1e-8 * np.abs(want).max()
+    assert np.abs(out["BB"].parent - want[n:]).max() <= 1e-8 * np.abs(want).max()
+    A = np.block([[Mpp, -Mmm], [-Mmm, Mpp]])
+    want = np.linalg.solve(A, np.concatenate([cl(X1[E], X2[B]), cl(X1[B], X2[E])]))
+    assert np.abs(out["EB"].parent - want[:n]).max() <= 1e-8 * np.abs(want).max()
+    assert np.abs(out["BE"].parent - want[n:]).max() <= 1e-8 * np.abs(want).max()
+
+
+def test_master_full_sky_is_identity(pkg):
+    """master (src/modecoupling.jl:317-341) on full-sky masks: every mode-coupling
+    block is the identity (M-- = 0), so the decoupled spectra are the
+    pseudo-spectra of the (I, Q, U) maps."""
+    from powerspectra_amd.master import master
+    from powerspectra_amd.sht import alm2map_pol, map2alm_pol
+    nside = 8
+    lmax = 2 * nside          # the iterated transform is exact to ~1e-14 here
+    rng = np.random.default_rng(9)
+    alms = []
+    for _ in range(3):
+        a = pkg.Alm(lmax)
+        a.alm[:] = rng.standard_normal(a.alm.size) + 1j * rng.standard_normal(a.alm.size)
+        for l in range(lmax + 1):
+            a[l, 0] = a[l, 0].real
+        alms.append(a)
+    for a in alms[1:]:
+        for m in range(2):
+            for l in range(m, 2):
+                a[l, m] = 0.0
+    maps = [x.pixels for x in alm2map_pol(*alms, nside)]
+    ones = np.ones(12 * nside * nside)
+    pol = lambda I, Q, U: map2alm_pol(I, Q, U, lmax=lmax, niter=12)
+    scal = lambda m: pkg.map2alm(m, lmax=lmax, niter=12)
+    out = master(maps, ones, ones, maps, ones, ones, lmin=2, transform=(pol, scal))
+    T, E, B = pol(*maps)
+    for XY, (a, b) in (("TT", (T, T)), ("TE", (T, E)), ("EE", (E, E)), ("BB", (B, B)),
+                       ("EB", (E, B))):
+        want = pkg.alm2cl(a, b)[2:]
+        assert np.abs(out[XY].parent - want).max() <= 1e-9 * np.abs(want).max()

@@ -189,3 +189,68 @@ def test_coupledcov_dispatch_and_keys(pkg, capsys):
     assert ctx.calls[-1][1] == 4
     assert pkg.coupledcov("EE", "TT", ws, spectra, ctx=ctx) is None
     assert "EE,TT not implemented" in capsys.readouterr().out
+
+
+# ----------------------------------------------------------- spin-2 SHT --------
+def _theta(pkg, nside):
+    from powerspectra_amd.sht import RingInfo
+    info = RingInfo(nside)
+    th = np.zeros(12 * nside * nside)
+    for k in range(info.first.size):
+        th[info.first[k]: info.first[k] + info.count[k]] = np.arccos(info.cth[k])
+    return th
+
+
+def test_pol_sht_pure_E_and_B_quadrupole(pkg):
+    """HEALPix convention Q + iU = -sum (a_E + i a_B) _2Y_lm with
+    _2Y_20 = sqrt(15 / (32 pi)) sin^2 theta: a pure E (B) quadrupole is pure Q (U)."""
+    from powerspectra_amd.sht import alm2map_pol
+    nside, lmax = 8, 10
+    th = _theta(pkg, nside)
+    y20 = np.sqrt(15 / (32 * np.pi)) * np.sin(th) ** 2
+    E = pkg.Alm(lmax)
+    E[2, 0] = 1.0
+    _, Q, U = alm2map_pol(pkg.Alm(lmax), E, pkg.Alm(lmax), nside)
+    assert np.abs(Q.pixels + y20).max() < 1e-14 and np.abs(U.pixels).max() < 1e-14
+    B = pkg.Alm(lmax)
+    B[2, 0] = 1.0
+    _, Q, U = alm2map_pol(pkg.Alm(lmax), pkg.Alm(lmax), B, nside)
+    assert np.abs(Q.pixels).max() < 1e-14 and np.abs(U.pixels + y20).max() < 1e-14
+
+
+def test_pol_sht_roundtrip_and_EB_separation(pkg):
+    from powerspectra_amd.sht import alm2map_pol, map2alm_pol
+    nside, lmax = 16, 24
+    rng = np.random.default_rng(2)
+    alms = []
+    for _ in range(3):
+        a = pkg.Alm(lmax)
+        a.alm[:] = rng.standard_normal(a.alm.size) + 1j * rng.standard_normal(a.alm.size)
+        for l in range(lmax + 1):
+            a[l, 0] = a[l, 0].real
+        alms.append(a)
+    T, E, B = alms
+    for a in (E, B):      # spin 2 starts at l = 2
+        for m in range(2):
+            for l in range(m, 2):
+                a[l, m] = 0.0
+    I, Q, U = alm2map_pol(T, E, B, nside)
+    T2, E2, B2 = map2alm_pol(I, Q, U, lmax=lmax, niter=12)
+    for x, y in ((T, T2), (E, E2), (B, B2)):
+        assert np.abs(x.alm - y.alm).max() < 1e-10
+    # a pure E map analyses to (numerically) no B, and vice versa
+    _, Q, U = alm2map_pol(pkg.Alm(lmax), E, pkg.Alm(lmax), nside)
+    _, E3, B3 = map2alm_pol(np.zeros_like(Q.pixels), Q, U, lmax=lmax, niter=12)
+    assert np.abs(B3.alm).max() < 1e-10 and np.abs(E3.alm - E.alm).max() < 1e-10
+
+
+def test_binning_matrix(pkg):
+    """src/util.jl:3-18: rows are normalised weights over each bin."""
+    from powerspectra_amd.master import binning_matrix
+    P = binning_matrix([2, 5, 9], [4, 8, 12], lambda l: l * (l + 1.0), lmax=10)
+    assert P.shape == (2, 11)                           # the last bin (right edge 12 > 10) is cut
+    w = np.array([6.0, 12.0, 20.0])
+    assert np.allclose(P[0, 2:5], w / w.sum()) and np.all(P[0, :2] == 0) and np.all(P[0, 5:] == 0)
+    assert np.allclose(P[1].sum(), 1.0) and np.all(P[1, :5] == 0)
+    P = binning_matrix([0, 3], [2, 5], lambda l: 1.0)
+    assert np.allclose(P, [[1 / 3] * 3 + [0] * 3, [0] * 3 + [1 / 3] * 3])
