@@ -161,6 +161,20 @@ int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
 int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
                  double* B, int64_t ldb, int nrhs);
 
+/* ---- spherical-harmonic transforms (mask -> a_lm -> W_l front end) ----- */
+/* Analysis of `nmaps` ring-ordered HEALPix maps (12 nside^2 doubles each,
+ * consecutive) into a_lm, 0 <= m <= l <= lmax, m-major (index
+ * m (2 lmax + 1 - m) / 2 + l), complex as interleaved (re, im) doubles,
+ * (lmax+1)(lmax+2)/2 coefficients per map, with `niter` Jacobi refinements.
+ * Replaces Healpix.jl's map2alm as called by mcm(spec, mask1, mask2) and the
+ * covariance workspace (reference src/modecoupling.jl:250-251,
+ * src/workspace.jl:155,163).  nside a power of two <= 1024, lmax <= 4 nside. */
+int psx_map2alm(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
+                const double* maps, double* alms);
+/* Synthesis (Healpix.jl alm2map): the inverse of one analysis step. */
+int psx_alm2map(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* alms,
+                double* maps);
+
 /* ---- serving: a persistent request loop ------------------------------- */
 /* The reference has no serving layer (its callers run mcm / coupledcov one
  * call at a time); these entry points batch those same calls for callers that

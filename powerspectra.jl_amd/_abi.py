@@ -90,6 +90,8 @@ _int = ctypes.c_int
 
 # name -> (restype, argtypes); mirrors include/powerspectra_hip.h
 SIGNATURES = {
+    "psx_map2alm": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
+    "psx_alm2map": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
     "psx_server_destroy": (_int, [_vp]),
     "psx_server_set_policy": (_int, [_vp, _int, ctypes.c_double]),
@@ -294,6 +296,26 @@ class Context:
         wp = (_dp * 8)(*([dptr(w) for w in W] + [ctypes.cast(None, _dp)] * (8 - nW)))
         self.check(self.lib.psx_cov(self._h, kind, lmin, lmax, sp, rp, wp, LW, precision,
                                     dptr(out), n), "psx_cov")
+        return out
+
+    def map2alm(self, maps, nside: int, lmax: int, niter: int = 3) -> np.ndarray:
+        """a_lm (complex, m-major, shape (nmaps, nalm)) of ring-ordered maps."""
+        M = np.ascontiguousarray(np.atleast_2d(maps), dtype=np.float64)
+        if M.shape[1] != 12 * nside * nside:
+            raise ArgumentError("maps must have 12 nside^2 pixels")
+        nalm = (lmax + 1) * (lmax + 2) // 2
+        out = np.empty((M.shape[0], nalm), dtype=np.complex128)
+        self.check(self.lib.psx_map2alm(self._h, nside, lmax, niter, M.shape[0], dptr(M),
+                                        out.ctypes.data_as(_dp)), "psx_map2alm")
+        return out
+
+    def alm2map(self, alms, nside: int, lmax: int) -> np.ndarray:
+        A = np.ascontiguousarray(np.atleast_2d(alms), dtype=np.complex128)
+        if A.shape[1] != (lmax + 1) * (lmax + 2) // 2:
+            raise ArgumentError("alm length does not match lmax")
+        out = np.empty((A.shape[0], 12 * nside * nside))
+        self.check(self.lib.psx_alm2map(self._h, nside, lmax, A.shape[0],
+                                        A.ctypes.data_as(_dp), dptr(out)), "psx_alm2map")
         return out
 
     def decouple_covmat(self, Y, B1, B2):

@@ -47,7 +47,7 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
         objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
-           f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
+           f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", "-lrocfft", f"-Wl,-rpath,{ROCM}/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

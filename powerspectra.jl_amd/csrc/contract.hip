@@ -533,6 +533,23 @@ struct CachedTiles {
 std::map<std::pair<void*, TileKey>, CachedTiles> g_tile_cache;
 std::mutex g_tile_mu;
 
+}  // namespace
+
+// Forget the tile lists of a context that is being destroyed (their device
+// copies go with the context's allocations; a later context could reuse the
+// address).
+void release_tile_cache(Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  for (auto it = g_tile_cache.begin(); it != g_tile_cache.end();) {
+    if (it->first.first == (void*)c)
+      it = g_tile_cache.erase(it);
+    else
+      ++it;
+  }
+}
+
+namespace {
+
 const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int waves) {
   std::lock_guard<std::mutex> lk(g_tile_mu);
   auto key = std::make_pair((void*)c, TileKey{lmin, lmax, LW, R, waves});

@@ -159,6 +159,8 @@ int psx_ctx_destroy(psx_ctx* h) {
       resolve_timers(c);
     } catch (...) {
     }
+    release_tile_cache(c);
+    release_sht_plans(c);
     for (auto& kv : c->allocations) (void)hipFree(kv.first);
     c->allocations.clear();
     release_blas(c);

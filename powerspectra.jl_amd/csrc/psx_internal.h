@@ -88,6 +88,12 @@ void* ctx_scratch(Ctx* c, size_t bytes);
 void ensure_tables(Ctx* c, int nmax_index);
 void release_blas(Ctx* c);
 void release_stage(Ctx* c);
+void release_tile_cache(Ctx* c);
+void release_sht_plans(Ctx* c);
+// spherical-harmonic transforms on device buffers (context stream)
+void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
+                 double* dalms);
+void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps);
 
 // Column-major matrix copies between host memory (any pageable buffer,
 // leading dimension ldh) and device memory (leading dimension ldd), in

@@ -1,0 +1,779 @@
+// Spherical-harmonic transforms of ring-ordered HEALPix maps on the GPU:
+// the mask -> a_lm -> W_l front end of mcm(spec, mask1, mask2) and of the
+// covariance workspace (reference src/modecoupling.jl:197,250-256,
+// src/workspace.jl:141-213, which call Healpix.jl's map2alm / alm2map).
+//
+// Analysis of one map (spin 0):
+//   1. ring Fourier coefficients.  The 2 nside + 1 equatorial rings all have
+//      4 nside pixels: one batched real-to-complex rocFFT.  The 2 (nside - 1)
+//      polar rings (4i pixels) go through a direct DFT kernel.  Each ring keeps
+//      its Hermitian half X[0 .. n/2].
+//   2. fold: F_k(m) = X_k[m mod n] (conjugated above n/2) e^{-i m phi0_k},
+//      combined over the north/south ring pair into F+ = F_N + F_S and
+//      F- = F_N - F_S (lambda_lm(-z) = (-1)^{l+m} lambda_lm(z)).
+//   3. Legendre sums a_lm = (4 pi / Npix) sum_pairs lambda_lm(z) F+-(m), one
+//      workgroup per (m, chunk of ring pairs), the sum over rings reduced
+//      through LDS in chunks of 16 l, partial sums per chunk added by a
+//      second kernel.
+// Synthesis runs the same steps backwards (one thread per ring pair
+// accumulating over l, no reduction), and map2alm refines with `niter` Jacobi
+// iterations a += A(map - S(a)) as Healpix.jl does.
+//
+// lambda_lm is the orthonormal associated Legendre function (Y_lm = lambda_lm
+// e^{i m phi}, Condon-Shortley phase) from the three-term recurrence in l,
+// started at lambda_mm = (-1)^m sqrt((2m+1)/(4 pi) prod_k (2k-1)/(2k)) sin^m.
+// That start underflows double range long before it stops mattering, so it
+// is carried as mantissa x 2^(600 s), s <= 0, mantissa <= 2^600 (rescaled
+// whenever the recurrence grows past it).  A term contributes its true value
+// mantissa x 2^(600 s) when s >= -1; below that it is under 2^-600.
+#include "psx_internal.h"
+
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace psx {
+
+namespace {
+
+constexpr int kLChunk = 16;                 // l values per LDS reduction round
+constexpr int kPairsPerThread = 2;          // ring pairs per analysis thread
+constexpr double kBig = 0x1p600;            // rescaling threshold
+constexpr double kInvBig = 0x1p-600;
+constexpr double kLnBig = 415.88830833596718565;  // 600 ln 2
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// ---- geometry -----------------------------------------------------------
+struct Rings {
+  int nside = 0, nr = 0, npairs = 0;
+  int64_t npix = 0, xlen = 0;
+  std::vector<int> n;
+  std::vector<int64_t> first, xoff;
+  std::vector<double> z, logsin, phi0;
+};
+
+Rings make_rings(int ns) {
+  Rings R;
+  R.nside = ns;
+  R.nr = 4 * ns - 1;
+  R.npairs = 2 * ns;
+  R.npix = 12LL * ns * ns;
+  R.n.resize(R.nr);
+  R.first.resize(R.nr);
+  R.xoff.resize(R.nr);
+  R.z.resize(R.nr);
+  R.logsin.resize(R.nr);
+  R.phi0.resize(R.nr);
+  const long double nsl = ns;
+  int64_t off = 0;
+  for (int k = 0; k < R.nr; ++k) {
+    const int i = k + 1;
+    long double z, phi0;
+    if (i < ns) {
+      R.n[k] = 4 * i;
+      R.first[k] = 2LL * i * (i - 1);
+      z = 1.0L - (long double)i * i / (3.0L * nsl * nsl);
+      phi0 = (long double)kPi / (4.0L * i);
+    } else if (i <= 3 * ns) {
+      R.n[k] = 4 * ns;
+      R.first[k] = 2LL * ns * (ns - 1) + (int64_t)(i - ns) * 4 * ns;
+      z = 4.0L / 3.0L - 2.0L * i / (3.0L * nsl);
+      phi0 = ((i - ns + 1) % 2) ? (long double)kPi / (4.0L * ns) : 0.0L;
+    } else {
+      const int ii = 4 * ns - i;
+      R.n[k] = 4 * ii;
+      R.first[k] = R.npix - 2LL * ii * (ii + 1);
+      z = -(1.0L - (long double)ii * ii / (3.0L * nsl * nsl));
+      phi0 = (long double)kPi / (4.0L * ii);
+    }
+    R.z[k] = (double)z;
+    const long double s2 = (1.0L - z) * (1.0L + z);
+    R.logsin[k] = (double)(0.5L * logl(s2));
+    R.phi0[k] = (double)phi0;
+    R.xoff[k] = off;
+    off += R.n[k] / 2 + 1;
+  }
+  R.xlen = off;
+  return R;
+}
+
+__host__ __device__ inline int64_t alm_idx(int lmax, int l, int m) {
+  return (int64_t)m * (2 * lmax + 1 - m) / 2 + l;
+}
+
+// ---- kernels ------------------------------------------------------------
+struct RingArgs {
+  const int* n;
+  const int64_t* first;
+  const int64_t* xoff;
+  const double* z;
+  const double* logsin;
+  const double* phi0;
+  int nr, npairs, lmax;
+};
+
+// Forward DFT of one polar ring (n = 4i <= 4 nside - 4 pixels):
+// X[q] = sum_j x_j e^{-2 pi i q j / n}, q = 0 .. n/2.  Cosine table in LDS;
+// sin(2 pi t / n) = cos(2 pi (t - n/4) / n).
+__global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
+                                                 double2* __restrict__ X, int ns) {
+  __shared__ double ctab[4096];
+  const int pr = blockIdx.x;                        // polar ring ordinal
+  const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
+  const int n = R.n[k];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  __syncthreads();
+  const double* x = map + R.first[k];
+  const int q4 = n / 4;
+  for (int q = threadIdx.x; q <= n / 2; q += blockDim.x) {
+    double re = 0.0, im = 0.0;
+    int t = 0;
+    for (int j = 0; j < n; ++j) {
+      const double v = x[j];
+      int ts = t - q4;
+      ts += ts < 0 ? n : 0;
+      re = fma(v, ctab[t], re);
+      im = fma(-v, ctab[ts], im);
+      t += q;
+      t -= t >= n ? n : 0;
+    }
+    X[R.xoff[k] + q] = make_double2(re, im);
+  }
+}
+
+// Inverse DFT of one polar ring from its Hermitian half Y[0 .. n/2]:
+// f_j = Y_0 + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}) + Y_{n/2} (-1)^j.
+__global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
+                                                  double* __restrict__ map, int ns) {
+  __shared__ double ctab[4096];
+  const int pr = blockIdx.x;
+  const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
+  const int n = R.n[k];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  __syncthreads();
+  const double2* Y = X + R.xoff[k];
+  const int q4 = n / 4;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double acc = Y[0].x + ((j & 1) ? -Y[n / 2].x : Y[n / 2].x);
+    int t = 0;
+    for (int q = 1; q < n / 2; ++q) {
+      t += j;
+      t -= t >= n ? n : 0;
+      int ts = t - q4;
+      ts += ts < 0 ? n : 0;
+      const double2 y = Y[q];
+      // Re(y e^{i a}) = y.x cos a - y.y sin a
+      acc = fma(2.0 * y.x, ctab[t], acc);
+      acc = fma(-2.0 * y.y, ctab[ts], acc);
+    }
+    map[R.first[k] + j] = acc;
+  }
+}
+
+__device__ inline double2 ring_coef(const RingArgs& R, const double2* __restrict__ X, int k,
+                                    int m) {
+  const int n = R.n[k];
+  int q = m % n;
+  double2 v;
+  if (2 * q <= n) {
+    v = X[R.xoff[k] + q];
+  } else {
+    v = X[R.xoff[k] + (n - q)];
+    v.y = -v.y;
+  }
+  double s, c;
+  sincos((double)m * R.phi0[k], &s, &c);
+  // v e^{-i m phi0}
+  return make_double2(v.x * c + v.y * s, v.y * c - v.x * s);
+}
+
+// F+-[m][p] from the ring spectra (north ring p, south ring nr-1-p; the
+// equator pairs with itself and counts once).
+__global__ void __launch_bounds__(256) fold_analysis(RingArgs R, const double2* __restrict__ X,
+                                                     double2* __restrict__ Fp,
+                                                     double2* __restrict__ Fm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (p >= R.npairs) return;
+  const int kn = p, ks = R.nr - 1 - p;
+  const double2 a = ring_coef(R, X, kn, m);
+  double2 b = make_double2(0.0, 0.0);
+  if (ks != kn) b = ring_coef(R, X, ks, m);
+  const int64_t o = (int64_t)m * R.npairs + p;
+  Fp[o] = make_double2(a.x + b.x, a.y + b.y);
+  Fm[o] = make_double2(a.x - b.x, a.y - b.y);
+}
+
+// Recurrence coefficients per (l, m), l >= m + 2, in a_lm order:
+//   lambda_l = A (z lambda_{l-1} - B lambda_{l-2}),
+//   A = sqrt((4l^2 - 1)/(l^2 - m^2)),  B = sqrt(((l-1)^2 - m^2)/(4(l-1)^2 - 1)).
+__global__ void __launch_bounds__(256) rec_coefs(int lmax, double2* __restrict__ AB) {
+  const int m = blockIdx.y;
+  const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > lmax) return;
+  double a = 0.0, b = 0.0;
+  if (l >= m + 2) {
+    const double L = l, M = m;
+    a = sqrt((4.0 * L * L - 1.0) / ((L - M) * (L + M)));
+    b = sqrt(((L - 1.0 - M) * (L - 1.0 + M)) / (4.0 * (L - 1.0) * (L - 1.0) - 1.0));
+  } else if (l == m + 1) {
+    a = sqrt(2.0 * m + 3.0);
+  }
+  AB[alm_idx(lmax, l, m)] = make_double2(a, b);
+}
+
+struct LegArgs {
+  RingArgs R;
+  const double2* AB;      // recurrence coefficients
+  const double* logc;     // log of sqrt((2m+1)/(4 pi) prod (2k-1)/(2k)), per m
+  int nchunk;             // pair chunks
+};
+
+// Scaled lambda_mm for ring k: mantissa and scale s <= 0 (value = mant 2^(600 s)).
+__device__ inline void lambda_start(const LegArgs& A, int k, int m, double& mant, int& s) {
+  const double L = A.logc[m] + (double)m * A.R.logsin[k];
+  const int sc = L >= 0.0 ? 0 : -(int)floor(-L / kLnBig);
+  mant = exp(L - (double)sc * kLnBig);
+  if (m & 1) mant = -mant;
+  s = sc;
+}
+
+// Analysis partial sums: part[chunk][idx(l, m)] = sum over the chunk's pairs of
+// lambda_lm(z_p) F+-_p(m).
+__global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double2* __restrict__ Fp,
+                                                         const double2* __restrict__ Fm,
+                                                         double2* __restrict__ part) {
+  __shared__ double red[kLChunk][2][256];
+  const int m = blockIdx.y;
+  const int chunk = blockIdx.x;
+  const int lmax = A.R.lmax;
+  const int tid = threadIdx.x;
+  double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread];
+  int sc[kPairsPerThread];
+  double2 fp[kPairsPerThread], fm[kPairsPerThread];
+  bool live[kPairsPerThread];
+#pragma unroll
+  for (int r = 0; r < kPairsPerThread; ++r) {
+    const int p = (chunk * kPairsPerThread + r) * 256 + tid;
+    live[r] = p < A.R.npairs;
+    const int pp = live[r] ? p : 0;
+    z[r] = A.R.z[pp];
+    lambda_start(A, pp, m, l1[r], sc[r]);
+    l2[r] = 0.0;
+    const int64_t o = (int64_t)m * A.R.npairs + pp;
+    fp[r] = live[r] ? Fp[o] : make_double2(0.0, 0.0);
+    fm[r] = live[r] ? Fm[o] : make_double2(0.0, 0.0);
+  }
+  for (int l0 = m; l0 <= lmax; l0 += kLChunk) {
+#pragma unroll
+    for (int j = 0; j < kLChunk; ++j) {
+      const int l = l0 + j;
+      double accr = 0.0, acci = 0.0;
+      if (l <= lmax) {
+        const double2 ab = l >= m + 1 ? A.AB[alm_idx(lmax, l, m)] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < kPairsPerThread; ++r) {
+          double lam;
+          if (l == m) {
+            lam = l1[r];
+          } else if (l == m + 1) {
+            lam = ab.x * z[r] * l1[r];
+            l2[r] = l1[r];
+            l1[r] = lam;
+          } else {
+            lam = ab.x * (z[r] * l1[r] - ab.y * l2[r]);
+            l2[r] = l1[r];
+            l1[r] = lam;
+          }
+          if (sc[r] < 0 && fabs(lam) > kBig) {
+            l1[r] *= kInvBig;
+            l2[r] *= kInvBig;
+            lam *= kInvBig;
+            sc[r] += 1;
+          }
+          if (sc[r] >= -1) {
+            const double lt = sc[r] == 0 ? lam : lam * kInvBig;
+            const double2 f = ((l - m) & 1) ? fm[r] : fp[r];
+            accr = fma(lt, f.x, accr);
+            acci = fma(lt, f.y, acci);
+          }
+        }
+      }
+      red[j][0][tid] = accr;
+      red[j][1][tid] = acci;
+    }
+    __syncthreads();
+    // 2 kLChunk outputs, 8 threads each: 32 values per thread, then a
+    // reduction over the 8 lanes of the group
+    {
+      const int o = tid >> 3, g = tid & 7;
+      const int j = o >> 1, c = o & 1;
+      double s = 0.0;
+#pragma unroll 8
+      for (int t = g * 32; t < g * 32 + 32; ++t) s += red[j][c][t];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      const int l = l0 + j;
+      if (g == 0 && l <= lmax) {
+        double* dst = (double*)(part + (int64_t)chunk * alm_idx(lmax, lmax + 1, lmax + 1) +
+                                alm_idx(lmax, l, m));
+        dst[c] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// alm (+)= w sum_chunk part[chunk]
+__global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ part,
+                                                    int nchunk, int64_t nalm, double w,
+                                                    int accumulate, double2* __restrict__ alm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nalm) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    const double2 v = part[(int64_t)c * nalm + i];
+    sr += v.x;
+    si += v.y;
+  }
+  double2 a = accumulate ? alm[i] : make_double2(0.0, 0.0);
+  a.x = fma(w, sr, a.x);
+  a.y = fma(w, si, a.y);
+  alm[i] = a;
+}
+
+// Synthesis: G[m][k] = sum_l a_lm lambda_lm(z_k) for the ring pair of each
+// thread (north: even + odd l+m parts, south: even - odd).
+__global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const double2* __restrict__ alm,
+                                                          double2* __restrict__ G) {
+  __shared__ double2 am[4096];
+  const int m = blockIdx.y;
+  const int lmax = A.R.lmax;
+  const int nl = lmax - m + 1;
+  const int64_t i0 = alm_idx(lmax, m, m);
+  for (int t = threadIdx.x; t < nl; t += blockDim.x) am[t] = alm[i0 + t];
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.R.npairs) return;
+  const double z = A.R.z[p];
+  double l1, l2 = 0.0;
+  int sc;
+  lambda_start(A, p, m, l1, sc);
+  double er = 0.0, ei = 0.0, orr = 0.0, oi = 0.0;
+  for (int l = m; l <= lmax; ++l) {
+    double lam;
+    if (l == m) {
+      lam = l1;
+    } else {
+      const double2 ab = A.AB[i0 + (l - m)];
+      lam = l == m + 1 ? ab.x * z * l1 : ab.x * (z * l1 - ab.y * l2);
+      l2 = l1;
+      l1 = lam;
+    }
+    if (sc < 0 && fabs(lam) > kBig) {
+      l1 *= kInvBig;
+      l2 *= kInvBig;
+      lam *= kInvBig;
+      sc += 1;
+    }
+    if (sc >= -1) {
+      const double lt = sc == 0 ? lam : lam * kInvBig;
+      const double2 a = am[l - m];
+      if ((l - m) & 1) {
+        orr = fma(lt, a.x, orr);
+        oi = fma(lt, a.y, oi);
+      } else {
+        er = fma(lt, a.x, er);
+        ei = fma(lt, a.y, ei);
+      }
+    }
+  }
+  const int kn = p, ks = A.R.nr - 1 - p;
+  G[(int64_t)m * A.R.nr + kn] = make_double2(er + orr, ei + oi);
+  if (ks != kn) G[(int64_t)m * A.R.nr + ks] = make_double2(er - orr, ei - oi);
+}
+
+// Hermitian half spectrum of each ring from G: Y[q] collects G_m e^{i m phi0}
+// for m = q mod n and conj(G_m e^{i m phi0}) for m = -q mod n (m >= 1); m = 0
+// contributes Re(G_0).
+__global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, const double2* __restrict__ G,
+                                                      double2* __restrict__ X) {
+  const int k = blockIdx.y;
+  const int n = R.n[k];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > n / 2) return;
+  const double phi0 = R.phi0[k];
+  double yr = 0.0, yi = 0.0;
+  for (int m = q; m <= R.lmax; m += n) {
+    const double2 g = G[(int64_t)m * R.nr + k];
+    if (m == 0) {
+      yr += g.x;
+      continue;
+    }
+    double s, c;
+    sincos((double)m * phi0, &s, &c);
+    yr += g.x * c - g.y * s;
+    yi += g.x * s + g.y * c;
+  }
+  for (int m = (q == 0 ? n : n - q); m <= R.lmax; m += n) {
+    const double2 g = G[(int64_t)m * R.nr + k];
+    double s, c;
+    sincos((double)m * phi0, &s, &c);
+    // conj(g e^{i m phi0})
+    yr += g.x * c - g.y * s;
+    yi -= g.x * s + g.y * c;
+  }
+  X[R.xoff[k] + q] = make_double2(yr, yi);
+}
+
+__global__ void __launch_bounds__(256) residual(const double* __restrict__ a,
+                                                const double* __restrict__ b, int64_t n,
+                                                double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+
+// ---- plans ----------------------------------------------------------------
+#define PSX_FFT(call)                                                            \
+  do {                                                                           \
+    rocfft_status s_ = (call);                                                   \
+    if (s_ != rocfft_status_success)                                             \
+      throw ::psx::Error{PSX_ERR_DEVICE, std::string(#call) + " failed (" +      \
+                                             std::to_string((int)s_) + ")"};     \
+  } while (0)
+
+struct ShtPlan {
+  Rings R;
+  int lmax = 0;
+  int64_t nalm = 0;
+  int nchunk = 0;
+  // device
+  int* dn = nullptr;
+  int64_t* dfirst = nullptr;
+  int64_t* dxoff = nullptr;
+  double *dz = nullptr, *dlogsin = nullptr, *dphi0 = nullptr, *dlogc = nullptr;
+  double2* dAB = nullptr;
+  rocfft_plan fwd = nullptr, inv = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  RingArgs ra() const {
+    return RingArgs{dn, dfirst, dxoff, dz, dlogsin, dphi0, R.nr, R.npairs, lmax};
+  }
+};
+
+std::mutex g_fft_mu;
+bool g_fft_ready = false;
+std::map<std::tuple<void*, int, int>, ShtPlan*> g_plans;
+
+ShtPlan* get_plan(Ctx* c, int nside, int lmax) {
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  auto key = std::make_tuple((void*)c, nside, lmax);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) return it->second;
+  if (!g_fft_ready) {
+    PSX_FFT(rocfft_setup());
+    g_fft_ready = true;
+  }
+  auto* P = new ShtPlan();
+  P->R = make_rings(nside);
+  P->lmax = lmax;
+  P->nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+  P->nchunk = (P->R.npairs + 256 * kPairsPerThread - 1) / (256 * kPairsPerThread);
+  const Rings& R = P->R;
+  P->dn = (int*)ctx_alloc(c, sizeof(int) * R.nr);
+  P->dfirst = (int64_t*)ctx_alloc(c, sizeof(int64_t) * R.nr);
+  P->dxoff = (int64_t*)ctx_alloc(c, sizeof(int64_t) * R.nr);
+  P->dz = (double*)ctx_alloc(c, sizeof(double) * R.nr);
+  P->dlogsin = (double*)ctx_alloc(c, sizeof(double) * R.nr);
+  P->dphi0 = (double*)ctx_alloc(c, sizeof(double) * R.nr);
+  P->dlogc = (double*)ctx_alloc(c, sizeof(double) * (lmax + 1));
+  P->dAB = (double2*)ctx_alloc(c, sizeof(double2) * P->nalm);
+  PSX_HIP(hipMemcpy(P->dn, R.n.data(), sizeof(int) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dfirst, R.first.data(), sizeof(int64_t) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dxoff, R.xoff.data(), sizeof(int64_t) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dz, R.z.data(), sizeof(double) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dlogsin, R.logsin.data(), sizeof(double) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dphi0, R.phi0.data(), sizeof(double) * R.nr, hipMemcpyHostToDevice));
+  std::vector<double> logc(lmax + 1);
+  long double acc = 0.0L;
+  for (int m = 0; m <= lmax; ++m) {
+    if (m > 0) acc += logl((2.0L * m - 1.0L) / (2.0L * m));
+    logc[m] = (double)(0.5L * (logl((2.0L * m + 1.0L) / (4.0L * (long double)kPi)) + acc));
+  }
+  PSX_HIP(hipMemcpy(P->dlogc, logc.data(), sizeof(double) * (lmax + 1), hipMemcpyHostToDevice));
+  {
+    dim3 grid((unsigned)((lmax + 1 + 255) / 256), (unsigned)(lmax + 1));
+    hipLaunchKernelGGL(rec_coefs, grid, dim3(256), 0, c->stream, lmax, P->dAB);
+    PSX_HIP(hipGetLastError());
+  }
+  // equatorial rings: 2 nside + 1 transforms of length 4 nside
+  const size_t len = (size_t)4 * nside, batch = (size_t)2 * nside + 1;
+  const size_t hl = len / 2 + 1;
+  {
+    rocfft_plan_description d = nullptr;
+    PSX_FFT(rocfft_plan_description_create(&d));
+    const size_t one = 1;
+    PSX_FFT(rocfft_plan_description_set_data_layout(d, rocfft_array_type_real,
+                                                    rocfft_array_type_hermitian_interleaved,
+                                                    nullptr, nullptr, 1, &one, len, 1, &one, hl));
+    PSX_FFT(rocfft_plan_create(&P->fwd, rocfft_placement_notinplace,
+                               rocfft_transform_type_real_forward, rocfft_precision_double, 1,
+                               &len, batch, d));
+    PSX_FFT(rocfft_plan_description_destroy(d));
+  }
+  {
+    rocfft_plan_description d = nullptr;
+    PSX_FFT(rocfft_plan_description_create(&d));
+    const size_t one = 1;
+    PSX_FFT(rocfft_plan_description_set_data_layout(d, rocfft_array_type_hermitian_interleaved,
+                                                    rocfft_array_type_real, nullptr, nullptr, 1,
+                                                    &one, hl, 1, &one, len));
+    PSX_FFT(rocfft_plan_create(&P->inv, rocfft_placement_notinplace,
+                               rocfft_transform_type_real_inverse, rocfft_precision_double, 1,
+                               &len, batch, d));
+    PSX_FFT(rocfft_plan_description_destroy(d));
+  }
+  size_t w1 = 0, w2 = 0;
+  PSX_FFT(rocfft_plan_get_work_buffer_size(P->fwd, &w1));
+  PSX_FFT(rocfft_plan_get_work_buffer_size(P->inv, &w2));
+  P->work_bytes = std::max(w1, w2);
+  if (P->work_bytes) P->work = ctx_alloc(c, P->work_bytes);
+  PSX_FFT(rocfft_execution_info_create(&P->info));
+  if (P->work) PSX_FFT(rocfft_execution_info_set_work_buffer(P->info, P->work, P->work_bytes));
+  PSX_FFT(rocfft_execution_info_set_stream(P->info, (void*)c->stream));
+  PSX_HIP(hipStreamSynchronize(c->stream));
+  g_plans[key] = P;
+  return P;
+}
+
+}  // namespace
+
+// Destroy the transform plans of a context that is being destroyed (their
+// device buffers go with the context's allocations).
+void release_sht_plans(Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_fft_mu);
+  for (auto it = g_plans.begin(); it != g_plans.end();) {
+    if (std::get<0>(it->first) == (void*)c) {
+      ShtPlan* P = it->second;
+      if (P->info) (void)rocfft_execution_info_destroy(P->info);
+      if (P->fwd) (void)rocfft_plan_destroy(P->fwd);
+      if (P->inv) (void)rocfft_plan_destroy(P->inv);
+      delete P;
+      it = g_plans.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+namespace {
+
+// Device working set for one map.
+struct ShtBufs {
+  double2 *X = nullptr, *Fp = nullptr, *Fm = nullptr, *part = nullptr, *G = nullptr;
+  double *syn = nullptr, *res = nullptr;
+};
+
+size_t sht_work_bytes(const ShtPlan& P) {
+  const Rings& R = P.R;
+  const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
+  return sizeof(double2) * ((size_t)R.xlen + 2 * fl + (size_t)P.nchunk * P.nalm +
+                            (size_t)(P.lmax + 1) * R.nr) +
+         2 * sizeof(double) * (size_t)R.npix + 8 * 256;  // 7 buffers, 256-aligned
+}
+
+ShtBufs carve(const ShtPlan& P, void* mem) {
+  const Rings& R = P.R;
+  char* p = (char*)mem;
+  auto take = [&](size_t b) {
+    char* q = p;
+    p += (b + 255) & ~size_t(255);
+    return q;
+  };
+  const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
+  ShtBufs B;
+  B.X = (double2*)take(sizeof(double2) * R.xlen);
+  B.Fp = (double2*)take(sizeof(double2) * fl);
+  B.Fm = (double2*)take(sizeof(double2) * fl);
+  B.part = (double2*)take(sizeof(double2) * P.nchunk * P.nalm);
+  B.G = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
+  B.syn = (double*)take(sizeof(double) * R.npix);
+  B.res = (double*)take(sizeof(double) * R.npix);
+  return B;
+}
+
+LegArgs leg(const ShtPlan& P) { return LegArgs{P.ra(), P.dAB, P.dlogc, P.nchunk}; }
+
+// alm (+)= A(map)
+void analysis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap, double2* dalm,
+              bool accumulate) {
+  const Rings& R = P.R;
+  const int ns = R.nside;
+  hipStream_t s = c->stream;
+  if (ns > 1) {
+    hipLaunchKernelGGL(polar_dft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), dmap, B.X, ns);
+    PSX_HIP(hipGetLastError());
+  }
+  {
+    void* in[1] = {(void*)(dmap + R.first[ns - 1])};
+    void* out[1] = {(void*)(B.X + R.xoff[ns - 1])};
+    PSX_FFT(rocfft_execute(P.fwd, in, out, P.info));
+  }
+  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, P.ra(), B.X, B.Fp, B.Fm);
+  PSX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(legendre_analysis, dim3((unsigned)P.nchunk, (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, leg(P), B.Fp, B.Fm, B.part);
+  PSX_HIP(hipGetLastError());
+  const double w = 4.0 * kPi / (double)R.npix;
+  hipLaunchKernelGGL(sum_partials, dim3((unsigned)((P.nalm + 255) / 256)), dim3(256), 0, s,
+                     B.part, P.nchunk, P.nalm, w, accumulate ? 1 : 0, dalm);
+  PSX_HIP(hipGetLastError());
+}
+
+// map = S(alm)
+void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double* dmap) {
+  const Rings& R = P.R;
+  const int ns = R.nside;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(legendre_synthesis,
+                     dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)), dim3(256), 0,
+                     s, leg(P), dalm, B.G);
+  PSX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
+                     dim3(256), 0, s, P.ra(), B.G, B.X);
+  PSX_HIP(hipGetLastError());
+  {
+    void* in[1] = {(void*)(B.X + R.xoff[ns - 1])};
+    void* out[1] = {(void*)(dmap + R.first[ns - 1])};
+    PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
+  }
+  if (ns > 1) {
+    hipLaunchKernelGGL(polar_idft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), B.X, dmap, ns);
+    PSX_HIP(hipGetLastError());
+  }
+}
+
+void check_sht_args(int nside, int lmax) {
+  // polar rings have at most 4 nside - 4 pixels: the DFT kernels' LDS
+  // cosine table holds 4096
+  PSX_REQUIRE(nside >= 1 && nside <= 1024, PSX_ERR_ARGUMENT, "nside must be 1..1024");
+  PSX_REQUIRE((nside & (nside - 1)) == 0, PSX_ERR_ARGUMENT, "nside must be a power of two");
+  PSX_REQUIRE(lmax >= 0 && lmax <= 4 * nside && lmax <= 4095, PSX_ERR_ARGUMENT,
+              "lmax must be 0..min(4 nside, 4095)");
+}
+
+}  // namespace
+
+// map2alm of nmaps device maps into device alms (complex interleaved), with
+// niter refinements, on the context stream.
+void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
+                 double* dalms) {
+  check_sht_args(nside, lmax);
+  ShtPlan& P = *get_plan(c, nside, lmax);
+  void* mem = ctx_alloc(c, sht_work_bytes(P));
+  try {
+    const ShtBufs B = carve(P, mem);
+    const int64_t npix = P.R.npix;
+    for (int i = 0; i < nmaps; ++i) {
+      const double* map = dmaps + i * npix;
+      double2* alm = (double2*)dalms + i * P.nalm;
+      analysis(c, P, B, map, alm, false);
+      for (int it = 0; it < niter; ++it) {
+        synthesis(c, P, B, alm, B.syn);
+        hipLaunchKernelGGL(residual, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+                           c->stream, map, B.syn, npix, B.res);
+        PSX_HIP(hipGetLastError());
+        analysis(c, P, B, B.res, alm, true);
+      }
+    }
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, mem);
+    throw;
+  }
+  ctx_free(c, mem);
+}
+
+void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps) {
+  check_sht_args(nside, lmax);
+  ShtPlan& P = *get_plan(c, nside, lmax);
+  void* mem = ctx_alloc(c, sht_work_bytes(P));
+  try {
+    const ShtBufs B = carve(P, mem);
+    for (int i = 0; i < nmaps; ++i)
+      synthesis(c, P, B, (const double2*)dalms + i * P.nalm, dmaps + i * P.R.npix);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, mem);
+    throw;
+  }
+  ctx_free(c, mem);
+}
+
+}  // namespace psx
+
+using namespace psx;
+
+extern "C" {
+
+int psx_map2alm(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const double* maps,
+                double* alms) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(maps && alms && nmaps >= 1 && niter >= 0, PSX_ERR_ARGUMENT, "bad arguments");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
+    double* da = nullptr;
+    try {
+      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
+      h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
+      map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
+      d2h_matrix(c, alms, 2 * nalm, da, 2 * nalm, 2 * nalm, nmaps);
+    } catch (...) {
+      ctx_free(c, dm);
+      if (da) ctx_free(c, da);
+      throw;
+    }
+    ctx_free(c, dm);
+    ctx_free(c, da);
+  });
+}
+
+int psx_alm2map(psx_ctx* h, int nside, int lmax, int nmaps, const double* alms, double* maps) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(maps && alms && nmaps >= 1, PSX_ERR_ARGUMENT, "bad arguments");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
+    double* da = nullptr;
+    try {
+      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
+      h2d_matrix(c, da, 2 * nalm, alms, 2 * nalm, 2 * nalm, nmaps);
+      alm2map_dev(c, nside, lmax, nmaps, da, dm);
+      d2h_matrix(c, maps, npix, dm, npix, npix, nmaps);
+    } catch (...) {
+      ctx_free(c, dm);
+      if (da) ctx_free(c, da);
+      throw;
+    }
+    ctx_free(c, dm);
+    ctx_free(c, da);
+  });
+}
+
+}  // extern "C"

@@ -1,12 +1,17 @@
-"""HEALPix ring geometry and spherical-harmonic transforms (numpy, host).
+"""HEALPix ring geometry and spherical-harmonic transforms.
 
-Upstream of the GPU path: the reference obtains mask spectra with
-Healpix.jl's `map2alm` (src/modecoupling.jl:250-251, src/workspace.jl:155,163)
-and `alm2cl`.  This is a direct ring-by-ring transform -- FFT along each
-iso-latitude ring, associated-Legendre recursion in l per m, vectorised over
-rings -- with the same defaults as Healpix.jl (lmax = 3 nside - 1, three
-Jacobi refinement iterations).  It is O(nside^3) on the host and meant for
-nside <= 512; pinned by round trips and Y_lm orthogonality in tests.
+The reference obtains mask spectra with Healpix.jl's `map2alm`
+(src/modecoupling.jl:250-251, src/workspace.jl:155,163) and `alm2cl`.
+`map2alm` / `alm2map` run on the GPU (psx_map2alm / psx_alm2map: rocFFT and
+direct ring DFTs, Legendre sums in HIP) with Healpix.jl's defaults
+(lmax = 3 nside - 1, three Jacobi refinement iterations).
+
+`map2alm_host` / `alm2map_host` are direct numpy transforms of the same
+definition (FFT along each iso-latitude ring, associated-Legendre recursion in
+l per m, vectorised over rings), O(nside^3) and meant for small maps; the
+polarised (spin-2) transforms `map2alm_pol` / `alm2map_pol` exist only in this
+host form.  All are pinned by round trips, Y_lm orthogonality and the
+analytic quadrupole in tests.
 """
 from __future__ import annotations
 
@@ -131,7 +136,8 @@ def _map2alm_once(pixels: np.ndarray, info: RingInfo, lmax: int) -> Alm:
     return Alm(lmax, lmax, out)
 
 
-def alm2map(alm: Alm, nside: int) -> HealpixMap:
+def alm2map_host(alm: Alm, nside: int) -> HealpixMap:
+    """Synthesis on the host (numpy)."""
     info = RingInfo(nside)
     lmax = alm.lmax
     nr = info.first.size
@@ -151,16 +157,35 @@ def alm2map(alm: Alm, nside: int) -> HealpixMap:
     return HealpixMap(pixels, nside)
 
 
-def map2alm(m, lmax: int | None = None, niter: int = 3) -> Alm:
-    """Spherical-harmonic analysis with `niter` Jacobi refinements (Healpix.jl default 3)."""
+def map2alm_host(m, lmax: int | None = None, niter: int = 3) -> Alm:
+    """Analysis on the host (numpy) with `niter` Jacobi refinements."""
     hm = m if isinstance(m, HealpixMap) else HealpixMap(m)
     lmax = nside2lmax(hm.nside) if lmax is None else int(lmax)
     info = RingInfo(hm.nside)
     alm = _map2alm_once(hm.pixels, info, lmax)
     for _ in range(niter):
-        resid = hm.pixels - alm2map(alm, hm.nside).pixels
+        resid = hm.pixels - alm2map_host(alm, hm.nside).pixels
         alm = Alm(lmax, lmax, alm.alm + _map2alm_once(resid, info, lmax).alm)
     return alm
+
+
+def map2alm(m, lmax: int | None = None, niter: int = 3, ctx=None) -> Alm:
+    """Spherical-harmonic analysis on the GPU with `niter` Jacobi refinements
+    (Healpix.jl's map2alm; default lmax = 3 nside - 1, niter = 3)."""
+    from ._abi import default_context
+    hm = m if isinstance(m, HealpixMap) else HealpixMap(m)
+    lmax = nside2lmax(hm.nside) if lmax is None else int(lmax)
+    ctx = ctx or default_context()
+    return Alm(lmax, lmax, ctx.map2alm(hm.pixels, hm.nside, lmax, niter)[0])
+
+
+def alm2map(alm: Alm, nside: int, ctx=None) -> HealpixMap:
+    """Spherical-harmonic synthesis on the GPU (Healpix.jl's alm2map)."""
+    from ._abi import default_context
+    if alm.mmax != alm.lmax:
+        raise ArgumentError("alm2map needs mmax = lmax")
+    ctx = ctx or default_context()
+    return HealpixMap(ctx.alm2map(alm.alm, nside, alm.lmax)[0], nside)
 
 
 # ---- spin 2 (polarisation) -----------------------------------------------------
@@ -250,7 +275,7 @@ def alm2map_pol(almT: Alm, almE: Alm, almB: Alm, nside: int):
     info = RingInfo(nside)
     lmax = almE.lmax
     nr = info.first.size
-    I = alm2map(almT, nside).pixels
+    I = alm2map_host(almT, nside).pixels
     Gp = np.zeros((nr, lmax + 1), dtype=np.complex128)
     Gm = np.zeros((nr, lmax + 1), dtype=np.complex128)
     norm = np.sqrt((2 * np.arange(lmax + 1) + 1) / (4 * np.pi))
@@ -285,7 +310,7 @@ def map2alm_pol(I, Q, U, lmax: int | None = None, niter: int = 3):
     ns = Im.nside
     lmax = nside2lmax(ns) if lmax is None else int(lmax)
     info = RingInfo(ns)
-    T = map2alm(Im, lmax=lmax, niter=niter)
+    T = map2alm_host(Im, lmax=lmax, niter=niter)
     aE, aB = _pol_analysis_once(Qp, Up, info, lmax)
     for _ in range(niter):
         _, q, u = alm2map_pol(Alm(lmax), Alm(lmax, lmax, aE), Alm(lmax, lmax, aB), ns)
