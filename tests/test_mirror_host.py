@@ -259,3 +259,31 @@ def test_binning_matrix(pkg):
     assert np.allclose(P[1].sum(), 1.0) and np.all(P[1, :5] == 0)
     P = binning_matrix([0, 3], [2, 5], lambda l: 1.0)
     assert np.allclose(P, [[1 / 3] * 3 + [0] * 3, [0] * 3 + [1 / 3] * 3])
+
+
+def test_cut_sky_mask(pkg):
+    """Seeded masks: 0 inside the cut and at hole centres, 1 away from both,
+    a cosine ramp over the apodisation width, reproducible by seed."""
+    from powerspectra_amd.masks import cut_sky_mask
+    from powerspectra_amd.sht import RingInfo
+    nside = 64
+    m = cut_sky_mask(nside, 10, seed=3)
+    assert np.array_equal(m, cut_sky_mask(nside, 10, seed=3))
+    assert not np.array_equal(m, cut_sky_mask(nside, 10, seed=4))
+    assert m.min() == 0.0 and m.max() == 1.0
+    info = RingInfo(nside)
+    b = np.degrees(np.pi / 2 - np.arccos(info.cth))
+    for k in range(info.first.size):
+        ring = m[info.first[k]: info.first[k] + info.count[k]]
+        if abs(b[k]) < 20.0:
+            assert np.all(ring == 0.0)
+        if 20.0 < abs(b[k]) < 21.0:          # ramp: 0.5 (1 - cos(pi (|b| - 20)))
+            assert ring.max() <= 0.5 * (1 - np.cos(np.pi * (abs(b[k]) - 20.0))) + 1e-12
+    # no holes: exactly the apodised cut
+    m0 = cut_sky_mask(nside, 0, seed=0)
+    for k in range(info.first.size):
+        want = 0.5 * (1 - np.cos(np.pi * np.clip(abs(b[k]) - 20.0, 0, 1)))
+        assert np.allclose(m0[info.first[k]: info.first[k] + info.count[k]], want)
+    # a hole removes sky only within 1.5 deg of its centre
+    assert (m0 - m).min() >= 0.0 and 0.0 < (m0 - m).sum() < 10 * 4 * np.pi * (1.5 * np.pi / 180) ** 2 \
+        / (4 * np.pi / m.size)
