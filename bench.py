@@ -304,6 +304,34 @@ def run_solve(args, pkg) -> dict:
                         "recovered_cl_max_rel_err_eebb": err_eb})
 
 
+def run_sht(args, pkg) -> dict:
+    """The mask -> W front end of the headline: map2alm (3 iterations) of the
+    four nside-1024 config masks at lmax 3071 and their ten cross-spectra
+    (host maps in, spectra out: PCIe included)."""
+    from powerspectra_amd.alm import Alm, alm2cl
+    from powerspectra_amd.masks import config_masks
+    ctx = pkg.Context(0)
+    nside, lmax = 1024, args.lmax
+    maps = np.stack(config_masks(nside, 200, (0, 1, 2, 3)))
+
+    def step():
+        A = ctx.map2alm(maps, nside, lmax, 3)
+        al = [Alm(lmax, lmax, a) for a in A]
+        return [alm2cl(al[i], al[j]) for i in range(4) for j in range(i, 4)]
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.close()
+    return _line("mask spectra: map2alm of four nside-1024 masks + cross-spectra", 1.0 / dt,
+                 "sets/s", args, dt * 1e3,
+                 {"workload": f"sht_nside1024_lmax{lmax}: map2alm (3 iterations) of four masks, "
+                              "ten alm2cl cross-spectra", "nside": nside, "lmax": lmax})
+
+
 def run_e2e(args, pkg) -> dict:
     """The headline workload through the host-pointer entry point psx_mcm_set:
     W copied in, five blocks computed and copied out each step (PCIe
@@ -476,7 +504,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
-                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve"],
+                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -491,7 +519,7 @@ def main() -> int:
     pkg = load_package()
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
-              "e2e": run_e2e, "serve": run_serve}[args.workload]
+              "e2e": run_e2e, "serve": run_serve, "sht": run_sht}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]
