@@ -12,9 +12,9 @@
 //      combined over the north/south ring pair into F+ = F_N + F_S and
 //      F- = F_N - F_S (lambda_lm(-z) = (-1)^{l+m} lambda_lm(z)).
 //   3. Legendre sums a_lm = (4 pi / Npix) sum_pairs lambda_lm(z) F+-(m), one
-//      workgroup per (m, chunk of ring pairs), the sum over rings reduced
-//      through LDS in chunks of 16 l, partial sums per chunk added by a
-//      second kernel.
+//      workgroup per (m, chunk of 1024 ring pairs), the sum over rings
+//      reduced through LDS in rounds of 8 l, partial sums per chunk added by
+//      a second kernel.
 // Synthesis runs the same steps backwards (one thread per ring pair
 // accumulating over l, no reduction), and map2alm refines with `niter` Jacobi
 // iterations a += A(map - S(a)) as Healpix.jl does.
@@ -40,8 +40,9 @@ namespace psx {
 
 namespace {
 
-constexpr int kLChunk = 16;                 // l values per LDS reduction round
-constexpr int kPairsPerThread = 2;          // ring pairs per analysis thread
+constexpr int kLChunk = 8;                  // l values per LDS reduction round
+constexpr int kPairsPerThread = 4;          // ring pairs per analysis thread
+constexpr int kSynPairs = 4;                // ring pairs per synthesis thread
 constexpr double kBig = 0x1p600;            // rescaling threshold
 constexpr double kInvBig = 0x1p-600;
 constexpr double kLnBig = 415.88830833596718565;  // 600 ln 2
@@ -242,8 +243,20 @@ __device__ inline void lambda_start(const LegArgs& A, int k, int m, double& mant
   s = sc;
 }
 
+// Uniform load through the constant address space: the value lands in SGPRs
+// (every lane reads the same address).
+__device__ __forceinline__ double sload(const double* p) {
+  return *((const __attribute__((address_space(4))) double*)p);
+}
+__device__ __forceinline__ double2 sload(const double2* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return make_double2(sload(q), sload(q + 1));
+}
+
 // Analysis partial sums: part[chunk][idx(l, m)] = sum over the chunk's pairs of
-// lambda_lm(z_p) F+-_p(m).
+// lambda_lm(z_p) F+-_p(m).  Each thread carries kPairsPerThread independent
+// recurrences; per round of kLChunk l values the 256 partial sums of every
+// (l, re/im) are reduced through LDS.
 __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double2* __restrict__ Fp,
                                                          const double2* __restrict__ Fm,
                                                          double2* __restrict__ part) {
@@ -252,21 +265,21 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
   const int chunk = blockIdx.x;
   const int lmax = A.R.lmax;
   const int tid = threadIdx.x;
+  const double2* ABm = A.AB + alm_idx(lmax, m, m);   // ABm[l - m]
   double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread];
   int sc[kPairsPerThread];
   double2 fp[kPairsPerThread], fm[kPairsPerThread];
-  bool live[kPairsPerThread];
 #pragma unroll
   for (int r = 0; r < kPairsPerThread; ++r) {
     const int p = (chunk * kPairsPerThread + r) * 256 + tid;
-    live[r] = p < A.R.npairs;
-    const int pp = live[r] ? p : 0;
+    const bool live = p < A.R.npairs;
+    const int pp = live ? p : 0;
     z[r] = A.R.z[pp];
     lambda_start(A, pp, m, l1[r], sc[r]);
     l2[r] = 0.0;
     const int64_t o = (int64_t)m * A.R.npairs + pp;
-    fp[r] = live[r] ? Fp[o] : make_double2(0.0, 0.0);
-    fm[r] = live[r] ? Fm[o] : make_double2(0.0, 0.0);
+    fp[r] = live ? Fp[o] : make_double2(0.0, 0.0);
+    fm[r] = live ? Fm[o] : make_double2(0.0, 0.0);
   }
   for (int l0 = m; l0 <= lmax; l0 += kLChunk) {
 #pragma unroll
@@ -274,18 +287,15 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
       const int l = l0 + j;
       double accr = 0.0, acci = 0.0;
       if (l <= lmax) {
-        const double2 ab = l >= m + 1 ? A.AB[alm_idx(lmax, l, m)] : make_double2(0.0, 0.0);
+        const double2 ab = l > m ? sload(ABm + (l - m)) : make_double2(0.0, 0.0);
+        const bool odd = (l - m) & 1;
 #pragma unroll
         for (int r = 0; r < kPairsPerThread; ++r) {
           double lam;
           if (l == m) {
             lam = l1[r];
-          } else if (l == m + 1) {
-            lam = ab.x * z[r] * l1[r];
-            l2[r] = l1[r];
-            l1[r] = lam;
           } else {
-            lam = ab.x * (z[r] * l1[r] - ab.y * l2[r]);
+            lam = l == m + 1 ? ab.x * z[r] * l1[r] : ab.x * fma(z[r], l1[r], -ab.y * l2[r]);
             l2[r] = l1[r];
             l1[r] = lam;
           }
@@ -297,7 +307,7 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
           }
           if (sc[r] >= -1) {
             const double lt = sc[r] == 0 ? lam : lam * kInvBig;
-            const double2 f = ((l - m) & 1) ? fm[r] : fp[r];
+            const double2 f = odd ? fm[r] : fp[r];
             accr = fma(lt, f.x, accr);
             acci = fma(lt, f.y, acci);
           }
@@ -307,17 +317,18 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
       red[j][1][tid] = acci;
     }
     __syncthreads();
-    // 2 kLChunk outputs, 8 threads each: 32 values per thread, then a
-    // reduction over the 8 lanes of the group
+    // 2 kLChunk = 16 outputs, 16 threads each: 16 values per thread, then a
+    // reduction over the 16 lanes of the group
     {
-      const int o = tid >> 3, g = tid & 7;
+      const int o = tid >> 4, g = tid & 15;
       const int j = o >> 1, c = o & 1;
       double s = 0.0;
-#pragma unroll 8
-      for (int t = g * 32; t < g * 32 + 32; ++t) s += red[j][c][t];
+#pragma unroll
+      for (int t = g * 16; t < g * 16 + 16; ++t) s += red[j][c][t];
       s += __shfl_xor(s, 1);
       s += __shfl_xor(s, 2);
       s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 8);
       const int l = l0 + j;
       if (g == 0 && l <= lmax) {
         double* dst = (double*)(part + (int64_t)chunk * alm_idx(lmax, lmax + 1, lmax + 1) +
@@ -347,55 +358,69 @@ __global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ 
   alm[i] = a;
 }
 
-// Synthesis: G[m][k] = sum_l a_lm lambda_lm(z_k) for the ring pair of each
-// thread (north: even + odd l+m parts, south: even - odd).
+// Synthesis: G[m][k] = sum_l a_lm lambda_lm(z_k) for kSynPairs ring pairs per
+// thread (north: even + odd l+m parts, south: even - odd); a_lm is uniform
+// across the workgroup and read with scalar loads.
 __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const double2* __restrict__ alm,
                                                           double2* __restrict__ G) {
-  __shared__ double2 am[4096];
   const int m = blockIdx.y;
   const int lmax = A.R.lmax;
-  const int nl = lmax - m + 1;
   const int64_t i0 = alm_idx(lmax, m, m);
-  for (int t = threadIdx.x; t < nl; t += blockDim.x) am[t] = alm[i0 + t];
-  __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.R.npairs) return;
-  const double z = A.R.z[p];
-  double l1, l2 = 0.0;
-  int sc;
-  lambda_start(A, p, m, l1, sc);
-  double er = 0.0, ei = 0.0, orr = 0.0, oi = 0.0;
+  const double2* am = alm + i0;
+  const double2* ABm = A.AB + i0;
+  double z[kSynPairs], l1[kSynPairs], l2[kSynPairs];
+  double er[kSynPairs], ei[kSynPairs], orr[kSynPairs], oi[kSynPairs];
+  int sc[kSynPairs], pr[kSynPairs];
+#pragma unroll
+  for (int r = 0; r < kSynPairs; ++r) {
+    const int p = (blockIdx.x * kSynPairs + r) * blockDim.x + threadIdx.x;
+    pr[r] = p;
+    const int pp = p < A.R.npairs ? p : 0;
+    z[r] = A.R.z[pp];
+    lambda_start(A, pp, m, l1[r], sc[r]);
+    l2[r] = 0.0;
+    er[r] = ei[r] = orr[r] = oi[r] = 0.0;
+  }
   for (int l = m; l <= lmax; ++l) {
-    double lam;
-    if (l == m) {
-      lam = l1;
-    } else {
-      const double2 ab = A.AB[i0 + (l - m)];
-      lam = l == m + 1 ? ab.x * z * l1 : ab.x * (z * l1 - ab.y * l2);
-      l2 = l1;
-      l1 = lam;
-    }
-    if (sc < 0 && fabs(lam) > kBig) {
-      l1 *= kInvBig;
-      l2 *= kInvBig;
-      lam *= kInvBig;
-      sc += 1;
-    }
-    if (sc >= -1) {
-      const double lt = sc == 0 ? lam : lam * kInvBig;
-      const double2 a = am[l - m];
-      if ((l - m) & 1) {
-        orr = fma(lt, a.x, orr);
-        oi = fma(lt, a.y, oi);
+    const double2 a = sload(am + (l - m));
+    const double2 ab = l > m ? sload(ABm + (l - m)) : make_double2(0.0, 0.0);
+    const bool odd = (l - m) & 1;
+#pragma unroll
+    for (int r = 0; r < kSynPairs; ++r) {
+      double lam;
+      if (l == m) {
+        lam = l1[r];
       } else {
-        er = fma(lt, a.x, er);
-        ei = fma(lt, a.y, ei);
+        lam = l == m + 1 ? ab.x * z[r] * l1[r] : ab.x * fma(z[r], l1[r], -ab.y * l2[r]);
+        l2[r] = l1[r];
+        l1[r] = lam;
+      }
+      if (sc[r] < 0 && fabs(lam) > kBig) {
+        l1[r] *= kInvBig;
+        l2[r] *= kInvBig;
+        lam *= kInvBig;
+        sc[r] += 1;
+      }
+      if (sc[r] >= -1) {
+        const double lt = sc[r] == 0 ? lam : lam * kInvBig;
+        if (odd) {
+          orr[r] = fma(lt, a.x, orr[r]);
+          oi[r] = fma(lt, a.y, oi[r]);
+        } else {
+          er[r] = fma(lt, a.x, er[r]);
+          ei[r] = fma(lt, a.y, ei[r]);
+        }
       }
     }
   }
-  const int kn = p, ks = A.R.nr - 1 - p;
-  G[(int64_t)m * A.R.nr + kn] = make_double2(er + orr, ei + oi);
-  if (ks != kn) G[(int64_t)m * A.R.nr + ks] = make_double2(er - orr, ei - oi);
+#pragma unroll
+  for (int r = 0; r < kSynPairs; ++r) {
+    const int p = pr[r];
+    if (p >= A.R.npairs) continue;
+    const int kn = p, ks = A.R.nr - 1 - p;
+    G[(int64_t)m * A.R.nr + kn] = make_double2(er[r] + orr[r], ei[r] + oi[r]);
+    if (ks != kn) G[(int64_t)m * A.R.nr + ks] = make_double2(er[r] - orr[r], ei[r] - oi[r]);
+  }
 }
 
 // Hermitian half spectrum of each ring from G: Y[q] collects G_m e^{i m phi0}
@@ -643,8 +668,9 @@ void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double
   const int ns = R.nside;
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(legendre_synthesis,
-                     dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)), dim3(256), 0,
-                     s, leg(P), dalm, B.G);
+                     dim3((unsigned)((R.npairs + 256 * kSynPairs - 1) / (256 * kSynPairs)),
+                          (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, leg(P), dalm, B.G);
   PSX_HIP(hipGetLastError());
   hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
                      dim3(256), 0, s, P.ra(), B.G, B.X);
