@@ -168,7 +168,8 @@ int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
  * (lmax+1)(lmax+2)/2 coefficients per map, with `niter` Jacobi refinements.
  * Replaces Healpix.jl's map2alm as called by mcm(spec, mask1, mask2) and the
  * covariance workspace (reference src/modecoupling.jl:250-251,
- * src/workspace.jl:155,163).  nside a power of two <= 1024, lmax <= 4 nside. */
+ * src/workspace.jl:155,163).  nside a power of two <= 2048,
+ * lmax <= min(4 nside, 8191). */
 int psx_map2alm(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
                 const double* maps, double* alms);
 /* Synthesis (Healpix.jl alm2map): the inverse of one analysis step. */

@@ -40,6 +40,7 @@ namespace psx {
 
 namespace {
 
+constexpr int kMaxPolarRing = 8192;         // pixels of the longest polar ring (nside <= 2048)
 constexpr int kLChunk = 8;                  // l values per LDS reduction round
 constexpr int kPairsPerThread = 4;          // ring pairs per analysis thread
 constexpr int kSynPairs = 4;                // ring pairs per synthesis thread
@@ -122,7 +123,7 @@ struct RingArgs {
 // sin(2 pi t / n) = cos(2 pi (t - n/4) / n).
 __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
                                                  double2* __restrict__ X, int ns) {
-  __shared__ double ctab[4096];
+  __shared__ double ctab[kMaxPolarRing];
   const int pr = blockIdx.x;                        // polar ring ordinal
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
   const int n = R.n[k];
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __res
 // f_j = Y_0 + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}) + Y_{n/2} (-1)^j.
 __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
                                                   double* __restrict__ map, int ns) {
-  __shared__ double ctab[4096];
+  __shared__ double ctab[kMaxPolarRing];
   const int pr = blockIdx.x;
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
   const int n = R.n[k];
@@ -688,11 +689,12 @@ void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double
 
 void check_sht_args(int nside, int lmax) {
   // polar rings have at most 4 nside - 4 pixels: the DFT kernels' LDS
-  // cosine table holds 4096
-  PSX_REQUIRE(nside >= 1 && nside <= 1024, PSX_ERR_ARGUMENT, "nside must be 1..1024");
+  // cosine table holds kMaxPolarRing
+  PSX_REQUIRE(nside >= 1 && 4 * (nside - 1) <= kMaxPolarRing, PSX_ERR_ARGUMENT,
+              "nside must be 1..2048");
   PSX_REQUIRE((nside & (nside - 1)) == 0, PSX_ERR_ARGUMENT, "nside must be a power of two");
-  PSX_REQUIRE(lmax >= 0 && lmax <= 4 * nside && lmax <= 4095, PSX_ERR_ARGUMENT,
-              "lmax must be 0..min(4 nside, 4095)");
+  PSX_REQUIRE(lmax >= 0 && lmax <= 4 * nside && lmax <= 8191, PSX_ERR_ARGUMENT,
+              "lmax must be 0..min(4 nside, 8191)");
 }
 
 }  // namespace

@@ -109,3 +109,22 @@ def test_mcm_from_masks_uses_gpu_transform(pkg, ctx, oracle):
     R = oracle.mcm(0, 0, lmax, V)
     from helpers import rel_err
     assert rel_err(M.parent, R) <= 1e-10
+
+
+def test_nside2048_roundtrip_and_lmax6143_masks(pkg, ctx):
+    """nside 2048 (polar rings up to 8188 pixels): a band-limited round trip
+    at lmax 4096, and the lmax-6143 workload's mask spectra (two nside-2048
+    masks with 500 holes) finite, with W_0 = 4 pi fsky^2-like and the steep
+    fall-off of an apodised mask."""
+    from powerspectra_amd.masks import config_masks, mask_spectra
+    nside, lmax = 2048, 4096
+    a = _rand_alm(pkg, lmax, 11)
+    m = ctx.alm2map(a.alm, nside, lmax)[0]
+    b = ctx.map2alm(m, nside, lmax, 10)[0]
+    assert np.abs(b - a.alm).max() <= 1e-9 * np.abs(a.alm).max()
+    t0 = time.perf_counter()
+    W = mask_spectra(config_masks(nside, 500, (0, 1)), 6143, ctx=ctx)[(0, 1)]
+    print(f"nside 2048 masks -> W (lmax 6143): {time.perf_counter() - t0:.2f} s")
+    assert W.size == 6144 and np.isfinite(W).all()
+    assert 0.5 * 4 * np.pi * 0.6 ** 2 < W[0] < 4 * np.pi
+    assert np.abs(W[3000:]).max() < 1e-4 * W[0]
