@@ -175,6 +175,17 @@ int psx_map2alm(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
 /* Synthesis (Healpix.jl alm2map): the inverse of one analysis step. */
 int psx_alm2map(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* alms,
                 double* maps);
+/* Polarised analysis (Healpix.jl map2alm of a PolarizedHealpixMap, as master
+ * calls it, reference src/modecoupling.jl:328-329): per map the I, Q, U rings
+ * (3 x 12 nside^2 doubles, consecutive) into T, E, B a_lm (3 x nalm complex,
+ * layout as psx_map2alm), HEALPix convention Q +- iU = sum _{+-2}a _{+-2}Y,
+ * a_E = -(_2a + _-2a)/2, a_B = i(_2a - _-2a)/2; `niter` refinements of both
+ * parts.  a_E, a_B are zero for l < 2.  Limits as psx_map2alm. */
+int psx_map2alm_pol(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
+                    const double* iqu, double* teb);
+/* Polarised synthesis: T, E, B a_lm to I, Q, U maps. */
+int psx_alm2map_pol(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* teb,
+                    double* iqu);
 
 /* ---- serving: a persistent request loop ------------------------------- */
 /* The reference has no serving layer (its callers run mcm / coupledcov one

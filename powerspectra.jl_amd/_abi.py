@@ -92,6 +92,8 @@ _int = ctypes.c_int
 SIGNATURES = {
     "psx_map2alm": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map": (_int, [_vp, _int, _int, _int, _dp, _dp]),
+    "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
+    "psx_alm2map_pol": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
     "psx_server_destroy": (_int, [_vp]),
     "psx_server_set_policy": (_int, [_vp, _int, ctypes.c_double]),
@@ -316,6 +318,35 @@ class Context:
         out = np.empty((A.shape[0], 12 * nside * nside))
         self.check(self.lib.psx_alm2map(self._h, nside, lmax, A.shape[0],
                                         A.ctypes.data_as(_dp), dptr(out)), "psx_alm2map")
+        return out
+
+    def map2alm_pol(self, iqu, nside: int, lmax: int, niter: int = 3) -> np.ndarray:
+        """(T, E, B) a_lm, shape (nmaps, 3, nalm), of (I, Q, U) ring-ordered
+        maps given as (3, npix) or (nmaps, 3, npix)."""
+        M = np.asarray(iqu, dtype=np.float64)
+        if M.ndim == 2:
+            M = M[None]
+        M = np.ascontiguousarray(M)
+        if M.ndim != 3 or M.shape[1] != 3 or M.shape[2] != 12 * nside * nside:
+            raise ArgumentError("maps must be (nmaps, 3, 12 nside^2)")
+        nalm = (lmax + 1) * (lmax + 2) // 2
+        out = np.empty((M.shape[0], 3, nalm), dtype=np.complex128)
+        self.check(self.lib.psx_map2alm_pol(self._h, nside, lmax, niter, M.shape[0], dptr(M),
+                                            out.ctypes.data_as(_dp)), "psx_map2alm_pol")
+        return out
+
+    def alm2map_pol(self, teb, nside: int, lmax: int) -> np.ndarray:
+        """(I, Q, U) maps, shape (nmaps, 3, npix), of (T, E, B) a_lm given as
+        (3, nalm) or (nmaps, 3, nalm)."""
+        A = np.asarray(teb, dtype=np.complex128)
+        if A.ndim == 2:
+            A = A[None]
+        A = np.ascontiguousarray(A)
+        if A.ndim != 3 or A.shape[1] != 3 or A.shape[2] != (lmax + 1) * (lmax + 2) // 2:
+            raise ArgumentError("alms must be (nmaps, 3, nalm) with nalm matching lmax")
+        out = np.empty((A.shape[0], 3, 12 * nside * nside))
+        self.check(self.lib.psx_alm2map_pol(self._h, nside, lmax, A.shape[0],
+                                            A.ctypes.data_as(_dp), dptr(out)), "psx_alm2map_pol")
         return out
 
     def decouple_covmat(self, Y, B1, B2):

@@ -457,6 +457,297 @@ __global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, const double2*
   X[R.xoff[k] + q] = make_double2(yr, yi);
 }
 
+// ---- spin 2 ---------------------------------------------------------------
+// With _sY_lm = (-1)^s N_l d^l_{m,-s}(theta) e^{i m phi}, N_l = sqrt((2l+1)/(4 pi)),
+// the two real functions per (l, m, ring) are X = N_l d^l_{m,-2} and
+// Y = N_l d^l_{m,2}; lambda+- = (X +- Y)/2.  For Fourier coefficients Q_m, U_m
+// of the real Q and U rings (HEALPix: Q +- iU = sum _{+-2}a _{+-2}Y,
+// a_E = -(_2a + _-2a)/2, a_B = i(_2a - _-2a)/2):
+//   a_E = -w sum_rings [lambda+ Q_m + i lambda- U_m],
+//   a_B = -w sum_rings [lambda+ U_m - i lambda- Q_m],
+// and synthesis is the transpose,
+//   Q_m = -sum_l [lambda+ a_E + i lambda- a_B],  U_m = -sum_l [lambda+ a_B - i lambda- a_E].
+// d^l_{m,m'}(-z) = (-1)^{l+m} d^l_{m,-m'}(z): lambda+ has the ring-pair
+// parity (-1)^{l+m} of the spin-0 case, lambda- the opposite one, so the
+// spin-0 fold (F+ = F_N + F_S, F- = F_N - F_S) serves Q and U unchanged.
+// X and Y each follow a three-term recurrence in l (they differ in the sign
+// of the m m' = -+2m term); run separately, each keeps full relative
+// precision, while a recurrence on lambda+- directly loses the small one near
+// the pole.
+constexpr int kLChunk2 = 4;                 // l values per LDS round, spin 2
+
+struct PolArgs {
+  RingArgs R;
+  const double* omz;      // 1 - z per ring, exact near the north pole
+  const double* abc;      // recurrence coefficients, 3 per (l, m)
+  const double* logc2;    // log(N_m K_m) - (m - 2) ln 2 per m (m >= 2)
+  int nchunk;
+};
+
+// Per (l, m), in a_lm order, the coefficients producing degree l from l - 1
+// and l - 2 (L = l - 1; zero for l <= max(m, 2)):
+//   X_l = (a z + b) X_{l-1} - c X_{l-2},   Y_l = (a z - b) Y_{l-1} - c Y_{l-2},
+//   a = sqrt((2L+3)(2L+1)) (L+1) / q,  b = 2 m a / (L (L+1)),
+//   c = sqrt((2L+3)/(2L-1)) (L+1) sqrt((L^2-m^2)(L^2-4)) / (L q),
+//   q = sqrt(((L+1)^2-m^2)((L+1)^2-4))
+// (the Wigner-d recurrence in l with m' = -+2, times N_l).  The integer
+// products stay below 2^53 for l <= 8191.
+__global__ void __launch_bounds__(256) rec_coefs_s2(int lmax, double* __restrict__ abc) {
+  const int m = blockIdx.y;
+  const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > lmax) return;
+  const int ls = m > 2 ? m : 2;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (l > ls) {
+    const double L = l - 1, L1 = l, M = m;
+    const double q = sqrt((L1 - M) * (L1 + M) * (L1 - 2.0) * (L1 + 2.0));
+    a = sqrt((2.0 * L + 3.0) * (2.0 * L + 1.0)) * L1 / q;
+    b = a * 2.0 * M / (L * L1);
+    c = sqrt((2.0 * L + 3.0) / (2.0 * L - 1.0)) * L1 *
+        sqrt((L - M) * (L + M) * (L - 2.0) * (L + 2.0)) / (L * q);
+  }
+  double* o = abc + 3 * alm_idx(lmax, l, m);
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+}
+
+// X and Y at l = max(m, 2) on north ring k, as mantissas of a common scale
+// 2^(600 s).  With c = cos(theta/2), s = sin(theta/2), K_m^2 = (2m)!/((m+2)!(m-2)!):
+//   m >= 2: X = (-1)^m N_m K_m (cs)^(m-2) s^4,  Y = ... c^4
+//           (s^4 = (1-z)^2/4, c^4 = (1+z)^2/4, cs = sin(theta)/2);
+//   m = 1:  X = -N_2 sin(theta) (1-z)/2,  Y = N_2 sin(theta) (1+z)/2;
+//   m = 0:  X = Y = N_2 (sqrt 6 / 4) sin^2(theta).
+__device__ inline void pol_start(const PolArgs& A, int k, int m, double& x, double& y, int& s) {
+  const double omz = A.omz[k];
+  const double opz = 1.0 + A.R.z[k];
+  if (m >= 2) {
+    const double L = A.logc2[m] + (double)(m - 2) * A.R.logsin[k];
+    const int sc = L >= 0.0 ? 0 : -(int)floor(-L / kLnBig);
+    double base = exp(L - (double)sc * kLnBig);
+    if (m & 1) base = -base;
+    x = 0.25 * base * omz * omz;
+    y = 0.25 * base * opz * opz;
+    s = sc;
+  } else {
+    const double n2 = 0.63078313050504001;   // sqrt(5 / (4 pi))
+    const double st = exp(A.R.logsin[k]);
+    if (m == 0) {
+      x = y = n2 * 0.61237243569579452 * omz * opz;   // sqrt(6)/4 sin^2
+    } else {
+      x = -0.5 * n2 * st * omz;
+      y = 0.5 * n2 * st * opz;
+    }
+    s = 0;
+  }
+}
+
+// One step of the X/Y recurrences of a ring pair, with rescaling; returns
+// 2 lambda+ and 2 lambda- at their true scale, or zero below 2^-1200.
+__device__ __forceinline__ bool pol_step(bool start, double a, double b, double c, double z,
+                                         double& x1, double& x2, double& y1, double& y2, int& sc,
+                                         double& tp, double& tm) {
+  double x, y;
+  if (start) {
+    x = x1;
+    y = y1;
+  } else {
+    const double az = a * z;
+    x = fma(az + b, x1, -c * x2);
+    y = fma(az - b, y1, -c * y2);
+    x2 = x1;
+    x1 = x;
+    y2 = y1;
+    y1 = y;
+  }
+  if (sc < 0 && fmax(fabs(x), fabs(y)) > kBig) {
+    x1 *= kInvBig;
+    x2 *= kInvBig;
+    y1 *= kInvBig;
+    y2 *= kInvBig;
+    x *= kInvBig;
+    y *= kInvBig;
+    sc += 1;
+  }
+  if (sc < -1) return false;
+  const double f = sc == 0 ? 1.0 : kInvBig;
+  tp = (x + y) * f;
+  tm = (x - y) * f;
+  return true;
+}
+
+// Spin-2 analysis partial sums: part[comp][chunk][idx(l, m)], comp 0 = E, 1 = B,
+// of sum over the chunk's pairs of [2 lambda+ Q + 2i lambda- U] (E) and
+// [2 lambda+ U - 2i lambda- Q] (B); F holds Q+, Q-, U+, U- at stride fs.
+__global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
+                                                            int64_t fs, double2* __restrict__ part) {
+  __shared__ double red[kLChunk2][4][256];
+  const int m = blockIdx.y;
+  const int chunk = blockIdx.x;
+  const int lmax = A.R.lmax;
+  const int tid = threadIdx.x;
+  const int ls = m > 2 ? m : 2;
+  const double* Cm = A.abc + 3 * alm_idx(lmax, m, m);   // Cm[3 (l - m) + 0..2]
+  const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+  double z[kPairsPerThread], x1[kPairsPerThread], x2[kPairsPerThread];
+  double y1[kPairsPerThread], y2[kPairsPerThread];
+  int sc[kPairsPerThread];
+  double2 qp[kPairsPerThread], qm[kPairsPerThread], up[kPairsPerThread], um[kPairsPerThread];
+#pragma unroll
+  for (int r = 0; r < kPairsPerThread; ++r) {
+    const int p = (chunk * kPairsPerThread + r) * 256 + tid;
+    const bool live = p < A.R.npairs;
+    const int pp = live ? p : 0;
+    z[r] = A.R.z[pp];
+    pol_start(A, pp, m, x1[r], y1[r], sc[r]);
+    x2[r] = y2[r] = 0.0;
+    const int64_t o = (int64_t)m * A.R.npairs + pp;   // in bounds for every lane
+    const double keep = live ? 1.0 : 0.0;
+    qp[r] = F[o];
+    qm[r] = F[fs + o];
+    up[r] = F[2 * fs + o];
+    um[r] = F[3 * fs + o];
+    qp[r].x *= keep;
+    qp[r].y *= keep;
+    qm[r].x *= keep;
+    qm[r].y *= keep;
+    up[r].x *= keep;
+    up[r].y *= keep;
+    um[r].x *= keep;
+    um[r].y *= keep;
+  }
+  for (int l0 = m; l0 <= lmax; l0 += kLChunk2) {
+#pragma unroll
+    for (int j = 0; j < kLChunk2; ++j) {
+      const int l = l0 + j;
+      double er = 0.0, ei = 0.0, br = 0.0, bi = 0.0;
+      if (l <= lmax && l >= ls) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        if (l > ls) {
+          const double* cf = Cm + 3 * (l - m);
+          a = sload(cf);
+          b = sload(cf + 1);
+          c = sload(cf + 2);
+        }
+        const bool odd = (l - m) & 1;
+#pragma unroll
+        for (int r = 0; r < kPairsPerThread; ++r) {
+          double tp, tm;
+          if (pol_step(l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], sc[r], tp, tm)) {
+            // lambda+ pairs with the (-1)^{l+m} fold, lambda- with the other
+            const double2 qa = odd ? qm[r] : qp[r], qb = odd ? qp[r] : qm[r];
+            const double2 ua = odd ? um[r] : up[r], ub = odd ? up[r] : um[r];
+            er = fma(tp, qa.x, fma(-tm, ub.y, er));
+            ei = fma(tp, qa.y, fma(tm, ub.x, ei));
+            br = fma(tp, ua.x, fma(tm, qb.y, br));
+            bi = fma(tp, ua.y, fma(-tm, qb.x, bi));
+          }
+        }
+      }
+      red[j][0][tid] = er;
+      red[j][1][tid] = ei;
+      red[j][2][tid] = br;
+      red[j][3][tid] = bi;
+    }
+    __syncthreads();
+    {
+      const int o = tid >> 4, g = tid & 15;
+      const int j = o >> 2, cc = o & 3;
+      double s = 0.0;
+#pragma unroll
+      for (int t = g * 16; t < g * 16 + 16; ++t) s += red[j][cc][t];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 8);
+      const int l = l0 + j;
+      if (g == 0 && l <= lmax) {
+        double* dst = (double*)(part + ((int64_t)(cc >> 1) * A.nchunk + chunk) * nalm +
+                                alm_idx(lmax, l, m));
+        dst[cc & 1] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Spin-2 synthesis: GQ[m][k], GU[m][k] = Q_m, U_m of every ring, kSynPairs
+// ring pairs per thread; lambda+ terms go to the part of the (-1)^{l+m}
+// parity, lambda- terms to the other (north = even + odd, south = even - odd).
+__global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
+                                                             const double2* __restrict__ aE,
+                                                             const double2* __restrict__ aB,
+                                                             double2* __restrict__ GQ,
+                                                             double2* __restrict__ GU) {
+  const int m = blockIdx.y;
+  const int lmax = A.R.lmax;
+  const int ls = m > 2 ? m : 2;
+  const int64_t i0 = alm_idx(lmax, m, m);
+  const double2* em = aE + i0;
+  const double2* bm = aB + i0;
+  const double* Cm = A.abc + 3 * i0;
+  double z[kSynPairs], x1[kSynPairs], x2[kSynPairs], y1[kSynPairs], y2[kSynPairs];
+  double2 qe[kSynPairs], qo[kSynPairs], ue[kSynPairs], uo[kSynPairs];
+  int sc[kSynPairs], pr[kSynPairs];
+#pragma unroll
+  for (int r = 0; r < kSynPairs; ++r) {
+    const int p = (blockIdx.x * kSynPairs + r) * blockDim.x + threadIdx.x;
+    pr[r] = p;
+    const int pp = p < A.R.npairs ? p : 0;
+    z[r] = A.R.z[pp];
+    pol_start(A, pp, m, x1[r], y1[r], sc[r]);
+    x2[r] = y2[r] = 0.0;
+    qe[r] = qo[r] = ue[r] = uo[r] = make_double2(0.0, 0.0);
+  }
+  for (int l = ls; l <= lmax; ++l) {
+    const double2 e = sload(em + (l - m));
+    const double2 b = sload(bm + (l - m));
+    double ca = 0.0, cb = 0.0, cc = 0.0;
+    if (l > ls) {
+      const double* cf = Cm + 3 * (l - m);
+      ca = sload(cf);
+      cb = sload(cf + 1);
+      cc = sload(cf + 2);
+    }
+    const bool odd = (l - m) & 1;
+#pragma unroll
+    for (int r = 0; r < kSynPairs; ++r) {
+      double tp, tm;
+      if (!pol_step(l == ls, ca, cb, cc, z[r], x1[r], x2[r], y1[r], y2[r], sc[r], tp, tm))
+        continue;
+      // lambda+ a_E and i lambda- a_B (Q); lambda+ a_B and -i lambda- a_E (U)
+      const double2 pq = make_double2(tp * e.x, tp * e.y);
+      const double2 mq = make_double2(-tm * b.y, tm * b.x);
+      const double2 pu = make_double2(tp * b.x, tp * b.y);
+      const double2 mu = make_double2(tm * e.y, -tm * e.x);
+      const double2 q_e = odd ? mq : pq, q_o = odd ? pq : mq;
+      const double2 u_e = odd ? mu : pu, u_o = odd ? pu : mu;
+      qe[r].x += q_e.x;
+      qe[r].y += q_e.y;
+      qo[r].x += q_o.x;
+      qo[r].y += q_o.y;
+      ue[r].x += u_e.x;
+      ue[r].y += u_e.y;
+      uo[r].x += u_o.x;
+      uo[r].y += u_o.y;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSynPairs; ++r) {
+    const int p = pr[r];
+    if (p >= A.R.npairs) continue;
+    const int kn = p, ks = A.R.nr - 1 - p;
+    const int64_t on = (int64_t)m * A.R.nr + kn, os = (int64_t)m * A.R.nr + ks;
+    // the factor -1/2 turns 2 lambda+- sums into Q_m, U_m
+    GQ[on] = make_double2(-0.5 * (qe[r].x + qo[r].x), -0.5 * (qe[r].y + qo[r].y));
+    GU[on] = make_double2(-0.5 * (ue[r].x + uo[r].x), -0.5 * (ue[r].y + uo[r].y));
+    if (ks != kn) {
+      GQ[os] = make_double2(-0.5 * (qe[r].x - qo[r].x), -0.5 * (qe[r].y - qo[r].y));
+      GU[os] = make_double2(-0.5 * (ue[r].x - uo[r].x), -0.5 * (ue[r].y - uo[r].y));
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) residual(const double* __restrict__ a,
                                                 const double* __restrict__ b, int64_t n,
                                                 double* __restrict__ out) {
@@ -484,6 +775,8 @@ struct ShtPlan {
   int64_t* dxoff = nullptr;
   double *dz = nullptr, *dlogsin = nullptr, *dphi0 = nullptr, *dlogc = nullptr;
   double2* dAB = nullptr;
+  // spin 2, built on first use
+  double *domz = nullptr, *dabc = nullptr, *dlogc2 = nullptr;
   rocfft_plan fwd = nullptr, inv = nullptr;
   rocfft_execution_info info = nullptr;
   void* work = nullptr;
@@ -497,11 +790,50 @@ std::mutex g_fft_mu;
 bool g_fft_ready = false;
 std::map<std::tuple<void*, int, int>, ShtPlan*> g_plans;
 
-ShtPlan* get_plan(Ctx* c, int nside, int lmax) {
+// Spin-2 tables of a plan: 1 - z per ring (long double, exact near the
+// pole), the recurrence coefficients and log(N_m K_m) - (m - 2) ln 2 with
+// log K_m = (1/2) sum_{j=2}^{m-1} log((2j+2)(2j+1) / ((j+3)(j-1))).
+void build_pol_tables(Ctx* c, ShtPlan* P) {
+  const Rings& R = P->R;
+  const int lmax = P->lmax;
+  const long double nsl = R.nside;
+  std::vector<double> omz(R.nr), logc2(lmax + 1, 0.0);
+  for (int k = 0; k < R.nr; ++k) {
+    const int i = k + 1;
+    long double v;
+    if (i < R.nside) v = (long double)i * i / (3.0L * nsl * nsl);
+    else if (i <= 3 * R.nside) v = 1.0L - (4.0L / 3.0L - 2.0L * i / (3.0L * nsl));
+    else v = 1.0L + (1.0L - (long double)(4 * R.nside - i) * (4 * R.nside - i) / (3.0L * nsl * nsl));
+    omz[k] = (double)v;
+  }
+  long double lk = 0.0L;
+  for (int m = 2; m <= lmax; ++m) {
+    if (m > 2) {
+      const long double j = m - 1;
+      lk += 0.5L * logl((2.0L * j + 2.0L) * (2.0L * j + 1.0L) / ((j + 3.0L) * (j - 1.0L)));
+    }
+    logc2[m] = (double)(0.5L * logl((2.0L * m + 1.0L) / (4.0L * (long double)kPi)) + lk -
+                        (long double)(m - 2) * logl(2.0L));
+  }
+  P->domz = (double*)ctx_alloc(c, sizeof(double) * R.nr);
+  P->dlogc2 = (double*)ctx_alloc(c, sizeof(double) * (lmax + 1));
+  P->dabc = (double*)ctx_alloc(c, 3 * sizeof(double) * P->nalm);
+  PSX_HIP(hipMemcpy(P->domz, omz.data(), sizeof(double) * R.nr, hipMemcpyHostToDevice));
+  PSX_HIP(hipMemcpy(P->dlogc2, logc2.data(), sizeof(double) * (lmax + 1), hipMemcpyHostToDevice));
+  dim3 grid((unsigned)((lmax + 1 + 255) / 256), (unsigned)(lmax + 1));
+  hipLaunchKernelGGL(rec_coefs_s2, grid, dim3(256), 0, c->stream, lmax, P->dabc);
+  PSX_HIP(hipGetLastError());
+  PSX_HIP(hipStreamSynchronize(c->stream));
+}
+
+ShtPlan* get_plan(Ctx* c, int nside, int lmax, bool pol = false) {
   std::lock_guard<std::mutex> lk(g_fft_mu);
   auto key = std::make_tuple((void*)c, nside, lmax);
   auto it = g_plans.find(key);
-  if (it != g_plans.end()) return it->second;
+  if (it != g_plans.end()) {
+    if (pol && !it->second->dabc) build_pol_tables(c, it->second);
+    return it->second;
+  }
   if (!g_fft_ready) {
     PSX_FFT(rocfft_setup());
     g_fft_ready = true;
@@ -575,6 +907,7 @@ ShtPlan* get_plan(Ctx* c, int nside, int lmax) {
   PSX_FFT(rocfft_execution_info_set_stream(P->info, (void*)c->stream));
   PSX_HIP(hipStreamSynchronize(c->stream));
   g_plans[key] = P;
+  if (pol) build_pol_tables(c, P);
   return P;
 }
 
@@ -687,6 +1020,119 @@ void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double
   }
 }
 
+// Device working set for one polarised (Q, U) pair.
+struct PolBufs {
+  double2 *X[2] = {nullptr, nullptr}, *F = nullptr, *part = nullptr;
+  double2* G[2] = {nullptr, nullptr};
+  double *syn = nullptr, *res = nullptr;   // Q then U, npix each
+  int64_t fs = 0;
+};
+
+size_t pol_work_bytes(const ShtPlan& P) {
+  const Rings& R = P.R;
+  const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
+  return sizeof(double2) * (2 * (size_t)R.xlen + 4 * fl + 2 * (size_t)P.nchunk * P.nalm +
+                            2 * (size_t)(P.lmax + 1) * R.nr) +
+         4 * sizeof(double) * (size_t)R.npix + 8 * 256;  // 7 buffers, 256-aligned
+}
+
+PolBufs carve_pol(const ShtPlan& P, void* mem) {
+  const Rings& R = P.R;
+  char* p = (char*)mem;
+  auto take = [&](size_t b) {
+    char* q = p;
+    p += (b + 255) & ~size_t(255);
+    return q;
+  };
+  const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
+  PolBufs B;
+  B.X[0] = (double2*)take(sizeof(double2) * R.xlen);
+  B.X[1] = (double2*)take(sizeof(double2) * R.xlen);
+  B.F = (double2*)take(4 * sizeof(double2) * fl);
+  B.fs = (int64_t)fl;
+  B.part = (double2*)take(2 * sizeof(double2) * P.nchunk * P.nalm);
+  B.G[0] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
+  B.G[1] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
+  B.syn = (double*)take(2 * sizeof(double) * R.npix);
+  B.res = (double*)take(2 * sizeof(double) * R.npix);
+  return B;
+}
+
+PolArgs pleg(const ShtPlan& P) { return PolArgs{P.ra(), P.domz, P.dabc, P.dlogc2, P.nchunk}; }
+
+// Ring Fourier coefficients of one real map, folded into F+ and F-.
+void ring_fold(Ctx* c, ShtPlan& P, const double* dmap, double2* X, double2* Fp, double2* Fm) {
+  const Rings& R = P.R;
+  const int ns = R.nside;
+  hipStream_t s = c->stream;
+  if (ns > 1) {
+    hipLaunchKernelGGL(polar_dft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), dmap, X, ns);
+    PSX_HIP(hipGetLastError());
+  }
+  {
+    void* in[1] = {(void*)(dmap + R.first[ns - 1])};
+    void* out[1] = {(void*)(X + R.xoff[ns - 1])};
+    PSX_FFT(rocfft_execute(P.fwd, in, out, P.info));
+  }
+  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, P.ra(), X, Fp, Fm);
+  PSX_HIP(hipGetLastError());
+}
+
+// Ring spectra G -> map (inverse fold, c2r rocFFT, polar inverse DFT).
+void ring_unfold(Ctx* c, ShtPlan& P, const double2* G, double2* X, double* dmap) {
+  const Rings& R = P.R;
+  const int ns = R.nside;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
+                     dim3(256), 0, s, P.ra(), G, X);
+  PSX_HIP(hipGetLastError());
+  {
+    void* in[1] = {(void*)(X + R.xoff[ns - 1])};
+    void* out[1] = {(void*)(dmap + R.first[ns - 1])};
+    PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
+  }
+  if (ns > 1) {
+    hipLaunchKernelGGL(polar_idft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), X, dmap, ns);
+    PSX_HIP(hipGetLastError());
+  }
+}
+
+// (a_E, a_B) (+)= A2(Q, U)
+void analysis_pol(Ctx* c, ShtPlan& P, const PolBufs& B, const double* dQ, const double* dU,
+                  double2* aE, double2* aB, bool accumulate) {
+  const Rings& R = P.R;
+  hipStream_t s = c->stream;
+  ring_fold(c, P, dQ, B.X[0], B.F, B.F + B.fs);
+  ring_fold(c, P, dU, B.X[1], B.F + 2 * B.fs, B.F + 3 * B.fs);
+  hipLaunchKernelGGL(legendre_analysis_s2, dim3((unsigned)P.nchunk, (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, pleg(P), (const double2*)B.F, B.fs, B.part);
+  PSX_HIP(hipGetLastError());
+  // the kernel sums 2 lambda+- terms; a_E, a_B carry -w
+  const double w = -0.5 * 4.0 * kPi / (double)R.npix;
+  const unsigned nb = (unsigned)((P.nalm + 255) / 256);
+  hipLaunchKernelGGL(sum_partials, dim3(nb), dim3(256), 0, s, (const double2*)B.part, P.nchunk,
+                     P.nalm, w, accumulate ? 1 : 0, aE);
+  PSX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sum_partials, dim3(nb), dim3(256), 0, s,
+                     (const double2*)(B.part + (int64_t)P.nchunk * P.nalm), P.nchunk, P.nalm, w,
+                     accumulate ? 1 : 0, aB);
+  PSX_HIP(hipGetLastError());
+}
+
+// (Q, U) = S2(a_E, a_B)
+void synthesis_pol(Ctx* c, ShtPlan& P, const PolBufs& B, const double2* aE, const double2* aB,
+                   double* dQ, double* dU) {
+  const Rings& R = P.R;
+  hipLaunchKernelGGL(legendre_synthesis_s2,
+                     dim3((unsigned)((R.npairs + 256 * kSynPairs - 1) / (256 * kSynPairs)),
+                          (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, c->stream, pleg(P), aE, aB, B.G[0], B.G[1]);
+  PSX_HIP(hipGetLastError());
+  ring_unfold(c, P, B.G[0], B.X[0], dQ);
+  ring_unfold(c, P, B.G[1], B.X[1], dU);
+}
+
 void check_sht_args(int nside, int lmax) {
   // polar rings have at most 4 nside - 4 pixels: the DFT kernels' LDS
   // cosine table holds kMaxPolarRing
@@ -747,6 +1193,66 @@ void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, do
   ctx_free(c, mem);
 }
 
+// Polarised map2alm: per map (I, Q, U) contiguous, npix each, into (T, E, B)
+// alms, nalm complex each; niter refinements of both the spin-0 and the
+// spin-2 parts.
+void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
+                     double* dteb) {
+  check_sht_args(nside, lmax);
+  ShtPlan& P = *get_plan(c, nside, lmax, true);
+  const int64_t npix = P.R.npix, nalm = P.nalm;
+  for (int i = 0; i < nmaps; ++i)
+    map2alm_dev(c, nside, lmax, niter, 1, diqu + 3 * i * npix, dteb + 6 * i * nalm);
+  void* mem = ctx_alloc(c, pol_work_bytes(P));
+  try {
+    const PolBufs B = carve_pol(P, mem);
+    for (int i = 0; i < nmaps; ++i) {
+      const double* Q = diqu + (3 * i + 1) * npix;
+      const double* U = Q + npix;
+      double2* aE = (double2*)dteb + (3 * i + 1) * nalm;
+      double2* aB = aE + nalm;
+      analysis_pol(c, P, B, Q, U, aE, aB, false);
+      for (int it = 0; it < niter; ++it) {
+        synthesis_pol(c, P, B, aE, aB, B.syn, B.syn + npix);
+        // Q and U are adjacent in the input, as are the two syntheses
+        hipLaunchKernelGGL(residual, dim3((unsigned)((2 * npix + 255) / 256)), dim3(256), 0,
+                           c->stream, Q, (const double*)B.syn, 2 * npix, B.res);
+        PSX_HIP(hipGetLastError());
+        analysis_pol(c, P, B, B.res, B.res + npix, aE, aB, true);
+      }
+    }
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, mem);
+    throw;
+  }
+  ctx_free(c, mem);
+}
+
+void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb, double* diqu) {
+  check_sht_args(nside, lmax);
+  ShtPlan& P = *get_plan(c, nside, lmax, true);
+  const int64_t npix = P.R.npix, nalm = P.nalm;
+  for (int i = 0; i < nmaps; ++i)
+    alm2map_dev(c, nside, lmax, 1, dteb + 6 * i * nalm, diqu + 3 * i * npix);
+  void* mem = ctx_alloc(c, pol_work_bytes(P));
+  try {
+    const PolBufs B = carve_pol(P, mem);
+    for (int i = 0; i < nmaps; ++i) {
+      const double2* aE = (const double2*)dteb + (3 * i + 1) * nalm;
+      double* Q = diqu + (3 * i + 1) * npix;
+      synthesis_pol(c, P, B, aE, aE + nalm, Q, Q + npix);
+    }
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, mem);
+    throw;
+  }
+  ctx_free(c, mem);
+}
+
 }  // namespace psx
 
 using namespace psx;
@@ -794,6 +1300,57 @@ int psx_alm2map(psx_ctx* h, int nside, int lmax, int nmaps, const double* alms, 
       h2d_matrix(c, da, 2 * nalm, alms, 2 * nalm, 2 * nalm, nmaps);
       alm2map_dev(c, nside, lmax, nmaps, da, dm);
       d2h_matrix(c, maps, npix, dm, npix, npix, nmaps);
+    } catch (...) {
+      ctx_free(c, dm);
+      if (da) ctx_free(c, da);
+      throw;
+    }
+    ctx_free(c, dm);
+    ctx_free(c, da);
+  });
+}
+
+int psx_map2alm_pol(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const double* iqu,
+                    double* teb) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(iqu && teb && nmaps >= 1 && niter >= 0, PSX_ERR_ARGUMENT, "bad arguments");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    double* dm = (double*)ctx_alloc(c, 3 * sizeof(double) * npix * nmaps);
+    double* da = nullptr;
+    try {
+      da = (double*)ctx_alloc(c, 6 * sizeof(double) * nalm * nmaps);
+      h2d_matrix(c, dm, npix, iqu, npix, npix, 3 * nmaps);
+      map2alm_pol_dev(c, nside, lmax, niter, nmaps, dm, da);
+      d2h_matrix(c, teb, 2 * nalm, da, 2 * nalm, 2 * nalm, 3 * nmaps);
+    } catch (...) {
+      ctx_free(c, dm);
+      if (da) ctx_free(c, da);
+      throw;
+    }
+    ctx_free(c, dm);
+    ctx_free(c, da);
+  });
+}
+
+int psx_alm2map_pol(psx_ctx* h, int nside, int lmax, int nmaps, const double* teb, double* iqu) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(iqu && teb && nmaps >= 1, PSX_ERR_ARGUMENT, "bad arguments");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    double* dm = (double*)ctx_alloc(c, 3 * sizeof(double) * npix * nmaps);
+    double* da = nullptr;
+    try {
+      da = (double*)ctx_alloc(c, 6 * sizeof(double) * nalm * nmaps);
+      h2d_matrix(c, da, 2 * nalm, teb, 2 * nalm, 2 * nalm, 3 * nmaps);
+      alm2map_pol_dev(c, nside, lmax, nmaps, da, dm);
+      d2h_matrix(c, iqu, npix, dm, npix, npix, 3 * nmaps);
     } catch (...) {
       ctx_free(c, dm);
       if (da) ctx_free(c, da);

@@ -56,8 +56,8 @@ def master(map1, maskT1, maskP1, map2, maskT2, maskP2, *, already_masked: bool =
            lmin: int = 0, transform=None, ctx: Context | None = None) -> dict:
     """Decoupled spectra of two polarised maps (I, Q, U) with their masks
     (src/modecoupling.jl:317-341).  `transform` = (pol_map2alm, map2alm):
-    the polarised and scalar analyses to use (defaults: the host spin-2
-    transform for the maps, the GPU transform for the masks)."""
+    the polarised and scalar analyses to use (defaults: the GPU transforms,
+    spin 0 and 2 for the maps, spin 0 for the masks)."""
     from .sht import HealpixMap, map2alm, map2alm_pol
     pol, scal = transform if transform is not None else (map2alm_pol, map2alm)
     px = lambda m: np.asarray(m.pixels if isinstance(m, HealpixMap) else m, dtype=np.float64)

@@ -2,16 +2,17 @@
 
 The reference obtains mask spectra with Healpix.jl's `map2alm`
 (src/modecoupling.jl:250-251, src/workspace.jl:155,163) and `alm2cl`.
-`map2alm` / `alm2map` run on the GPU (psx_map2alm / psx_alm2map: rocFFT and
-direct ring DFTs, Legendre sums in HIP) with Healpix.jl's defaults
-(lmax = 3 nside - 1, three Jacobi refinement iterations).
+`map2alm` / `alm2map` and the polarised `map2alm_pol` / `alm2map_pol` run on
+the GPU (psx_map2alm / psx_alm2map / psx_map2alm_pol / psx_alm2map_pol:
+rocFFT and direct ring DFTs, Legendre and Wigner-d sums in HIP) with
+Healpix.jl's defaults (lmax = 3 nside - 1, three Jacobi refinement
+iterations).
 
-`map2alm_host` / `alm2map_host` are direct numpy transforms of the same
-definition (FFT along each iso-latitude ring, associated-Legendre recursion in
-l per m, vectorised over rings), O(nside^3) and meant for small maps; the
-polarised (spin-2) transforms `map2alm_pol` / `alm2map_pol` exist only in this
-host form.  All are pinned by round trips, Y_lm orthogonality and the
-analytic quadrupole in tests.
+The `*_host` functions are direct numpy transforms of the same definitions
+(FFT along each iso-latitude ring, recursion in l per m, vectorised over
+rings), O(nside^3) and meant for small maps.  They are pinned by round trips,
+Y_lm orthogonality, the Wigner d against sympy and the analytic quadrupole in
+tests, and the GPU transforms are checked against them.
 """
 from __future__ import annotations
 
@@ -270,8 +271,8 @@ def _pol_analysis_once(Q, U, info: RingInfo, lmax: int):
     return aE, aB
 
 
-def alm2map_pol(almT: Alm, almE: Alm, almB: Alm, nside: int):
-    """(I, Q, U) maps from T, E, B coefficients (HEALPix convention)."""
+def alm2map_pol_host(almT: Alm, almE: Alm, almB: Alm, nside: int):
+    """(I, Q, U) maps from T, E, B coefficients (HEALPix convention), numpy."""
     info = RingInfo(nside)
     lmax = almE.lmax
     nr = info.first.size
@@ -301,9 +302,9 @@ def alm2map_pol(almT: Alm, almE: Alm, almB: Alm, nside: int):
     return HealpixMap(I, nside), HealpixMap(P.real.copy(), nside), HealpixMap(P.imag.copy(), nside)
 
 
-def map2alm_pol(I, Q, U, lmax: int | None = None, niter: int = 3):
+def map2alm_pol_host(I, Q, U, lmax: int | None = None, niter: int = 3):
     """(T, E, B) coefficients of an (I, Q, U) map with `niter` Jacobi
-    refinements (Healpix.jl's polarised map2alm)."""
+    refinements (Healpix.jl's polarised map2alm), numpy."""
     Im = I if isinstance(I, HealpixMap) else HealpixMap(I)
     Qp = np.asarray(Q.pixels if isinstance(Q, HealpixMap) else Q, dtype=np.float64)
     Up = np.asarray(U.pixels if isinstance(U, HealpixMap) else U, dtype=np.float64)
@@ -313,8 +314,37 @@ def map2alm_pol(I, Q, U, lmax: int | None = None, niter: int = 3):
     T = map2alm_host(Im, lmax=lmax, niter=niter)
     aE, aB = _pol_analysis_once(Qp, Up, info, lmax)
     for _ in range(niter):
-        _, q, u = alm2map_pol(Alm(lmax), Alm(lmax, lmax, aE), Alm(lmax, lmax, aB), ns)
+        _, q, u = alm2map_pol_host(Alm(lmax), Alm(lmax, lmax, aE), Alm(lmax, lmax, aB), ns)
         dE, dB = _pol_analysis_once(Qp - q.pixels, Up - u.pixels, info, lmax)
         aE = aE + dE
         aB = aB + dB
     return T, Alm(lmax, lmax, aE), Alm(lmax, lmax, aB)
+
+
+def _pixels(m) -> np.ndarray:
+    return np.asarray(m.pixels if isinstance(m, HealpixMap) else m, dtype=np.float64)
+
+
+def map2alm_pol(I, Q, U, lmax: int | None = None, niter: int = 3, ctx=None):
+    """(T, E, B) coefficients of an (I, Q, U) map on the GPU with `niter`
+    Jacobi refinements (Healpix.jl's map2alm of a PolarizedHealpixMap, as
+    master calls it, reference src/modecoupling.jl:328-329)."""
+    from ._abi import default_context
+    iqu = np.stack([_pixels(I), _pixels(Q), _pixels(U)])
+    ns = HealpixMap(iqu[0]).nside
+    lmax = nside2lmax(ns) if lmax is None else int(lmax)
+    ctx = ctx or default_context()
+    teb = ctx.map2alm_pol(iqu, ns, lmax, niter)[0]
+    return tuple(Alm(lmax, lmax, teb[i]) for i in range(3))
+
+
+def alm2map_pol(almT: Alm, almE: Alm, almB: Alm, nside: int, ctx=None):
+    """(I, Q, U) maps from T, E, B coefficients on the GPU."""
+    from ._abi import default_context
+    lmax = almT.lmax
+    for a in (almT, almE, almB):
+        if a.lmax != lmax or a.mmax != lmax:
+            raise ArgumentError("alm2map_pol needs three alms with mmax = lmax = lmax of T")
+    ctx = ctx or default_context()
+    iqu = ctx.alm2map_pol(np.stack([almT.alm, almE.alm, almB.alm]), nside, lmax)[0]
+    return tuple(HealpixMap(iqu[i], nside) for i in range(3))
