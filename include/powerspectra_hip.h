@@ -187,6 +187,27 @@ int psx_map2alm_pol(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
 int psx_alm2map_pol(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* teb,
                     double* iqu);
 
+/* ---- QuickPol beam matrix --------------------------------------------- */
+/* quickpolΞ! (reference src/beam.jl:72-101, with Ξsum :17-28 and
+ * specrowrange :57-61) on a BandedMatrix in BandedMatrices.jl band storage:
+ * (bl + bu + 1) x (lmax + 1), column-major, element (row l'', column l) at
+ * band[(bu + l'' - l) + l (bl + bu + 1)].  For 2 <= l'' <= lmax and
+ * max(2, l'' - bl) <= l <= min(lmax, l'' + bu):
+ *   band(l'', l) = (-1)^(s1+s2+nu1+nu2) sum_{l'} W[l'] f1(l') f2(l'),
+ *   f_k = WignerF(l, l'', -s_k, -nu_k) = (l' l l''; s_k+nu_k, -s_k, -nu_k);
+ * every other stored entry (rows or columns 0, 1) is multiplied by the same
+ * sign, as the reference's `𝚵 .*= sgn` does, so `band` is read and written.
+ * W is 0-based with LW entries (quickpolW of the two scan-weight alms);
+ * terms with l' >= LW count as zero (the reference reads W[l'] unchecked).
+ * A family with |s| > l or |nu| > l'' is zero.  lmax <= 65535, bandwidths
+ * 0..lmax, spins within -64..64. */
+int psx_quickpol_xi(psx_ctx* ctx, int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu,
+                    const double* W, int64_t LW, double* band);
+/* The same on device buffers, enqueued on the context stream (returns when
+ * done). */
+int psx_quickpol_xi_d(psx_ctx* ctx, int nu1, int nu2, int s1, int s2, int lmax, int bl,
+                      int bu, const double* dW, int64_t LW, double* dband);
+
 /* ---- serving: a persistent request loop ------------------------------- */
 /* The reference has no serving layer (its callers run mcm / coupledcov one
  * call at a time); these entry points batch those same calls for callers that

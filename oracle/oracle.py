@@ -50,6 +50,7 @@ def lib():
                                ctypes.c_int64, ctypes.c_int]
         L.psxo_decouple_covmat.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.psxo_lu_solve.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.psxo_quickpol_xi.argtypes = [ctypes.c_int] * 7 + [_dp, ctypes.c_int64, _dp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -143,3 +144,20 @@ def lu_solve(A, B) -> np.ndarray:
     if st != 0:
         raise RuntimeError(f"psxo_lu_solve status {st}")
     return B
+
+
+def quickpol_xi(nu1: int, nu2: int, s1: int, s2: int, lmax: int, bl: int, bu: int, W,
+                band=None, nthreads: int = 0) -> np.ndarray:
+    """QuickPol Ξ (src/beam.jl:72-101) in band storage, shape (bl + bu + 1, lmax + 1),
+    Fortran order; `band` (in/out) defaults to zeros."""
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    if band is None:
+        band = np.zeros((bl + bu + 1, lmax + 1), order="F")
+    if band.shape != (bl + bu + 1, lmax + 1) or band.dtype != np.float64 or \
+            not band.flags.f_contiguous:
+        raise ValueError("band must be a Fortran-ordered float64 (bl+bu+1, lmax+1) array")
+    st = lib().psxo_quickpol_xi(nu1, nu2, s1, s2, lmax, bl, bu, _ptr(W), W.size, _ptr(band),
+                                nthreads)
+    if st != 0:
+        raise RuntimeError(f"psxo_quickpol_xi: status {st}")
+    return band

@@ -22,6 +22,11 @@
  *   psxo_decouple_covmat   the reference KAT (test/test_covmat.jl:11-17) and
  *                          the C = M C_dec M^T invariant (:115-127);
  *   psxo_lu_solve          numpy.linalg.solve.
+ *   psxo_quickpol_xi       general-spin families against the sympy fixtures
+ *                          (spins up to 4); for s = nu = 0 it equals the
+ *                          pinned TT mode-coupling sums (W -> 4 pi W/(2l+1));
+ *                          brute force from sympy at small lmax; band layout
+ *                          and the (-1)^(s1+s2+nu1+nu2) sign by hand.
  * Spin-2 rows/columns with l < 2 are unpinned (the reference's value comes
  * from WignerFamilies.jl on invalid |m| > l); the oracle returns zeros there.
  */
@@ -62,6 +67,12 @@ int psxo_cov_rows(int kind, int lmin, int lmax, int l1_lo, int l1_hi,
                   const double* const* spectra, const double* const* ratios,
                   const double* const* W, int64_t LW, double* C, int64_t ldC,
                   int nthreads);
+
+/* QuickPol Ξ (reference src/beam.jl:72-101) into band storage
+ * band[(bu + i - j) + j (bl + bu + 1)], i = l'' (row), j = l (column);
+ * in/out: entries outside rows/columns >= 2 are only multiplied by the sign. */
+int psxo_quickpol_xi(int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu,
+                     const double* W, int64_t LW, double* band, int nthreads);
 
 /* out = B1^{-1} Y B2^{-T} (column-major n x n). */
 int psxo_decouple_covmat(int n, const double* Y, const double* B1,

@@ -397,3 +397,67 @@ int psxo_decouple_covmat(int n, const double* Y, const double* B1,
   free(T);
   return st;
 }
+
+/* ------------------------------------------------------------------------
+ * QuickPol Ξ, reference src/beam.jl:17-28 (Ξsum), :57-61 (specrowrange) and
+ * :72-101 (quickpolΞ!).  The matrix is a BandedMatrix (lower bandwidth bl,
+ * upper bu) in band storage: element (row i = l'', column j = l), 0-based,
+ * at band[(bu + i - j) + j * (bl + bu + 1)].  For every row l'' in 2..lmax
+ * and column l in max(2, l'' - bl)..min(lmax, l'' + bu):
+ *   Ξ[l'', l] = sum_{l'} W[l'] f1(l') f2(l'),
+ *   f1 = WignerF(l, l'', -s1, -nu1) = (l' l l''; s1+nu1, -s1, -nu1),
+ *   f2 = WignerF(l, l'', -s2, -nu2),
+ * l' over the intersection of the two families' ranges.  Then the whole band
+ * (the entries just written and all others) is multiplied by
+ * (-1)^(s1 + s2 + nu1 + nu2) (beam.jl:98-99), so entries the loop does not
+ * write (rows or columns 0 and 1) keep their input value times that sign.
+ * The reference indexes W at l' without bounds checks (@inbounds); here terms
+ * with l' >= LW are zero, identical whenever LW > 2 lmax.
+ * ---------------------------------------------------------------------- */
+int psxo_quickpol_xi(int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu,
+                     const double* W, int64_t LW, double* band, int nthreads) {
+  if (lmax < 0 || bl < 0 || bu < 0 || LW < 0) return PSX_ERR_ARGUMENT;
+  const int64_t ldb = (int64_t)bl + bu + 1;
+  const double sgn = ((s1 + s2 + nu1 + nu2) % 2 == 0) ? 1.0 : -1.0;
+  int nt = nthreads_or_default(nthreads);
+  int status = PSX_OK;
+#pragma omp parallel num_threads(nt)
+  {
+    double* w1 = (double*)malloc(sizeof(double) * (2 * (size_t)lmax + 2));
+    double* w2 = (double*)malloc(sizeof(double) * (2 * (size_t)lmax + 2));
+    if (!w1 || !w2) {
+#pragma omp atomic write
+      status = PSX_ERR_BUDGET;
+    }
+#pragma omp for schedule(dynamic)
+    for (int lpp = 2; lpp <= lmax; ++lpp) {
+      if (!w1 || !w2) continue;
+      const int lo = lpp - bl > 2 ? lpp - bl : 2;
+      const int hi = lpp + bu < lmax ? lpp + bu : lmax;
+      for (int l = lo; l <= hi; ++l) {
+        int n1min, n2min;
+        const int n1 = wigner_family_d(l, lpp, -s1, -nu1, w1, &n1min);
+        const int n2 = wigner_family_d(l, lpp, -s2, -nu2, w2, &n2min);
+        const int start = n1min > n2min ? n1min : n2min;
+        int64_t stop = l + lpp;                 /* both families end at l + l'' */
+        if (n1 <= 0 || n2 <= 0) stop = start - 1;
+        if (stop > LW - 1) stop = LW - 1;
+        double xi = 0.0;
+        for (int64_t lp = start; lp <= stop; ++lp)
+          xi += W[lp] * w1[lp - n1min] * w2[lp - n2min];
+        band[(bu + lpp - l) + (int64_t)l * ldb] = xi;
+      }
+    }
+    free(w1);
+    free(w2);
+  }
+  /* 𝚵 .*= sgn over the stored band */
+  const int n = lmax + 1;
+  for (int j = 0; j < n; ++j)
+    for (int d = -bl; d <= bu; ++d) {
+      const int i = j - d;                       /* row with j - i = d */
+      if (i < 0 || i >= n) continue;
+      band[(bu - d) + (int64_t)j * ldb] *= sgn;
+    }
+  return status;
+}

@@ -7,6 +7,7 @@ from . import _abi
 from ._abi import (ArgumentError, AxisError, BudgetError, Context, DeviceError, DomainError,
                    PowerSpectraError, Server, SingularException, default_context)
 from .alm import Alm, alm2cl
+from .beam import BandedMatrix, ku, quickpolW, quickpolXi
 from .blockspectralmatrix import (BlockSpectralMatrix, block_vector, getblock, hcat, hvcat, solve,
                                   spectra, vcat)
 from .covariance import coupledcov, coupledcovTEEE, decouple_covmat
@@ -19,7 +20,7 @@ from .workspace import (ConstantDict, CovarianceWorkspace, CovField, SpectralWor
                         SpectrumName)
 
 __all__ = [
-    "Alm", "ArgumentError", "AxisError", "BlockSpectralMatrix", "BudgetError", "ConstantDict",
+    "Alm", "ArgumentError", "AxisError", "BandedMatrix", "BlockSpectralMatrix", "BudgetError", "ConstantDict",
     "Context", "CovField", "CovarianceWorkspace", "DeviceError", "DomainError", "HealpixMap",
     "PowerSpectraError", "Server", "SingularException", "SpectralArray", "SpectralVector",
     "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "alm2map_pol", "binning_matrix",
@@ -27,5 +28,5 @@ __all__ = [
     "coupledcovTEEE", "decouple_covmat", "default_context", "getblock", "hcat", "hvcat", "inv",
     "map2alm", "map2alm_pol", "mask", "maskedalm2spectra", "master", "mcm", "mcm_from_spectrum",
     "mcm_set", "nside2lmax", "nside2npix",
-    "parent", "scale", "solve", "spectra", "spectralones", "spectralzeros", "vcat",
+    "ku", "parent", "quickpolW", "quickpolXi", "scale", "solve", "spectra", "spectralones", "spectralzeros", "vcat",
 ]

@@ -92,6 +92,8 @@ _int = ctypes.c_int
 SIGNATURES = {
     "psx_map2alm": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map": (_int, [_vp, _int, _int, _int, _dp, _dp]),
+    "psx_quickpol_xi": (_int, [_vp] + [_int] * 7 + [_dp, _i64, _dp]),
+    "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map_pol": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
@@ -348,6 +350,21 @@ class Context:
         self.check(self.lib.psx_alm2map_pol(self._h, nside, lmax, A.shape[0],
                                             A.ctypes.data_as(_dp), dptr(out)), "psx_alm2map_pol")
         return out
+
+    def quickpol_xi(self, nu1: int, nu2: int, s1: int, s2: int, lmax: int, bl: int, bu: int,
+                    W, band=None) -> np.ndarray:
+        """QuickPol Ξ in band storage (bl + bu + 1, lmax + 1), Fortran order;
+        `band` is read and written (zeros if not given)."""
+        W = np.ascontiguousarray(W, dtype=np.float64).ravel()
+        ldb = bl + bu + 1
+        if band is None:
+            band = np.zeros((ldb, lmax + 1), order="F")
+        if band.shape != (ldb, lmax + 1) or band.dtype != np.float64 or \
+                not band.flags.f_contiguous:
+            raise AxisError("band must be a Fortran-ordered float64 (bl+bu+1, lmax+1) array")
+        self.check(self.lib.psx_quickpol_xi(self._h, nu1, nu2, s1, s2, lmax, bl, bu, dptr(W),
+                                            W.size, dptr(band)), "psx_quickpol_xi")
+        return band
 
     def decouple_covmat(self, Y, B1, B2):
         Y = np.asfortranarray(Y, dtype=np.float64)

@@ -41,6 +41,17 @@ def main():
                            (700, 800, 157)):
         fams.append(family(l1, l2, 0, 0, step))
         fams.append(family(l1, l2, -2, 2, step))
+    # QuickPol families WignerF(l, l'', -s, -nu) (src/beam.jl:87-88): spins
+    # s in {0, +-2}, nu up to +-4
+    qp = ((-2, 0), (2, -1), (-2, -3), (0, 2), (2, 4), (-2, -4), (0, -3))
+    for l1 in range(0, 9):
+        for l2 in range(0, 9):
+            for m1, m2 in qp:
+                if abs(m1) <= l1 and abs(m2) <= l2:
+                    fams.append(family(l1, l2, m1, m2))
+    for (l1, l2, m1, m2, step) in ((40, 57, -2, -3, 5), (150, 203, 2, 4, 23), (5, 90, -2, -4, 1),
+                                   (300, 300, 0, -3, 37)):
+        fams.append(family(l1, l2, m1, m2, step))
     with open(OUT, "w") as f:
         json.dump({"generator": "sympy.physics.wigner.wigner_3j (exact), 25 digits",
                    "families": fams}, f, separators=(",", ":"))

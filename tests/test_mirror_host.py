@@ -287,3 +287,38 @@ def test_cut_sky_mask(pkg):
     # a hole removes sky only within 1.5 deg of its centre
     assert (m0 - m).min() >= 0.0 and 0.0 < (m0 - m).sum() < 10 * 4 * np.pi * (1.5 * np.pi / 180) ** 2 \
         / (4 * np.pi / m.size)
+
+
+# ---- QuickPol helpers (src/beam.jl) ------------------------------------------
+
+def test_quickpolW_bruteforce(pkg):
+    from powerspectra_amd.beam import quickpolW
+    rng = np.random.default_rng(5)
+    lmax = 9
+    a, b = pkg.Alm(lmax), pkg.Alm(lmax)
+    a.alm[:] = rng.standard_normal(a.alm.size) + 1j * rng.standard_normal(a.alm.size)
+    b.alm[:] = rng.standard_normal(b.alm.size) + 1j * rng.standard_normal(b.alm.size)
+    W = quickpolW(a, b).parent
+    for l in range(lmax + 1):
+        want = (a[l, 0] * np.conj(b[l, 0])).real + sum(
+            2 * (a[l, m] * np.conj(b[l, m])).real for m in range(1, l + 1))
+        assert abs(W[l] - want) <= 1e-13
+
+
+def test_ku_and_banded_layout(pkg):
+    from powerspectra_amd.beam import BandedMatrix, ku
+    assert ku(0) == 1.0 and ku(2) == 0.5 and ku(-2) == 0.5
+    with pytest.raises(pkg.ArgumentError):
+        ku(1)
+    B = BandedMatrix(6, 1, 2)
+    for i in range(6):
+        for j in B.rowrange(i):
+            B[i, j] = 10 * i + j
+    D = B.todense()
+    for i in range(6):
+        for j in range(6):
+            assert D[i, j] == ((10 * i + j) if -1 <= j - i <= 2 else 0.0)
+    # band storage: element (i, j) at data[u + i - j, j]
+    assert B.data[2 + 3 - 4, 4] == 34 and B[5, 1] == 0.0
+    with pytest.raises(pkg.ArgumentError):
+        B[0, 3] = 1.0

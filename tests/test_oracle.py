@@ -372,3 +372,53 @@ def test_lu_solve_vs_numpy(oracle):
     assert np.allclose(X, np.linalg.solve(A, B), rtol=1e-12, atol=1e-12)
     with pytest.raises(RuntimeError):
         oracle.lu_solve(np.zeros((3, 3)), np.ones(3))
+
+
+# ---------------------------------------------------------------- QuickPol --
+def _qp_cells(lmax, bl, bu):
+    for lpp in range(2, lmax + 1):
+        for l in range(max(2, lpp - bl), min(lmax, lpp + bu) + 1):
+            yield lpp, l
+
+
+def test_quickpol_oracle_tt_reduction(oracle):
+    """s = nu = 0: both families are (l' l l''; 0 0 0), so Ξ[l'', l] equals the
+    (independently pinned) TT mode-coupling sum M[l, l''] / (2 l'' + 1) with
+    V = 4 pi W / (2 l' + 1)."""
+    lmax, bl, bu = 40, 5, 7
+    rng = np.random.default_rng(3)
+    W = rng.uniform(0.1, 1.0, 2 * lmax + 1)
+    band = oracle.quickpol_xi(0, 0, 0, 0, lmax, bl, bu, W)
+    ell = np.arange(W.size)
+    M = oracle.mcm(0, 0, lmax, 4 * np.pi * W / (2 * ell + 1))
+    for lpp, l in _qp_cells(lmax, bl, bu):
+        want = M[l, lpp] / (2 * lpp + 1)
+        assert abs(band[bu + lpp - l, l] - want) <= 1e-14 * abs(want) + 1e-16
+
+
+@pytest.mark.parametrize("nu1,nu2,s1,s2", [(1, -2, 2, 0), (0, 3, -2, 2), (-4, 2, 2, -2),
+                                            (2, 2, 0, 2)])
+def test_quickpol_oracle_bruteforce_sympy(oracle, nu1, nu2, s1, s2):
+    """Every written cell against exact sympy 3j sums, the sign
+    (-1)^(s1+s2+nu1+nu2), and entries in rows/columns 0, 1 (not written by the
+    loop) scaled by that sign only (src/beam.jl:98-99)."""
+    from sympy.physics.wigner import wigner_3j
+    lmax, bl, bu = 7, 2, 3
+    rng = np.random.default_rng(nu1 + 10 * s1 + 100)
+    W = rng.uniform(-1.0, 1.0, 2 * lmax + 1)
+    start = np.asfortranarray(rng.uniform(1.0, 2.0, (bl + bu + 1, lmax + 1)))
+    band = oracle.quickpol_xi(nu1, nu2, s1, s2, lmax, bl, bu, W, band=start.copy(order="F"))
+    sgn = (-1) ** ((s1 + s2 + nu1 + nu2) % 2)
+    for lpp in range(0, lmax + 1):
+        for l in range(max(0, lpp - bl), min(lmax, lpp + bu) + 1):
+            got = band[bu + lpp - l, l]
+            if lpp < 2 or l < 2:
+                assert got == sgn * start[bu + lpp - l, l]
+                continue
+            want = 0.0
+            for lp in range(abs(l - lpp), l + lpp + 1):
+                a = wigner_3j(lp, l, lpp, s1 + nu1, -s1, -nu1)
+                b = wigner_3j(lp, l, lpp, s2 + nu2, -s2, -nu2)
+                if a != 0 and b != 0:
+                    want += W[lp] * float(a * b)
+            assert abs(got - sgn * want) <= 1e-14, (lpp, l, got, sgn * want)
