@@ -11,7 +11,8 @@ resident in HBM; the timed region is the device work only.
 
 Side workloads (--workload): the single-block kernels, the covariance
 kernels at lmax 2508, lmax 6143 with the LU solves, the headline with host
-buffers (PCIe included), and the serving stream.
+buffers (PCIe included), the serving stream, the mask -> W transforms and the
+QuickPol Ξ band.
 
 Prints ONE JSON line.  `value` is elements/s of the whole step (5 n^2 matrix
 entries per step); `roofline` is for the
@@ -341,6 +342,62 @@ def run_sht(args, pkg) -> dict:
                               "ten alm2cl cross-spectra", "nside": nside, "lmax": lmax})
 
 
+QP_SPINS = (2, -2, 2, 2)   # nu1, nu2, s1, s2
+
+
+def qp_terms(lmax: int, bl: int, bu: int, LW: int, spins) -> tuple[int, int]:
+    """(family terms of both families, contracted terms) of one QuickPol Ξ
+    band: every cell l'', l >= 2 inside the band runs two families over
+    l' = max(|l-l''|, |s+nu|) .. l+l'' and contracts their overlap below LW."""
+    nu1, nu2, s1, s2 = spins
+    fam = con = 0
+    lpp = np.arange(2, lmax + 1)
+    for d in range(-bl, bu + 1):
+        l = lpp + d
+        ok = (l >= 2) & (l <= lmax)
+        ok &= (abs(s1) <= l) & (abs(nu1) <= lpp) & (abs(s2) <= l) & (abs(nu2) <= lpp)
+        nmax = (l + lpp)[ok]
+        na = np.maximum(abs(d), abs(s1 + nu1))
+        nb = np.maximum(abs(d), abs(s2 + nu2))
+        fam += int(np.sum(nmax - na + 1) + np.sum(nmax - nb + 1))
+        con += int(np.sum(np.maximum(0, np.minimum(nmax, LW - 1) - max(na, nb) + 1)))
+    return fam, con
+
+
+def run_quickpol(args, pkg) -> dict:
+    """QuickPol Ξ (src/beam.jl:72-101) at lmax with 41 diagonals (bl = bu = 20),
+    spins nu = (2, -2), s = (2, 2), W of length 2 lmax + 1; device resident."""
+    ctx = pkg.Context(0)
+    L = ctx.lib
+    lmax, bl, bu = args.lmax, 20, 20
+    LW = 2 * lmax + 1
+    W = synthetic_W(LW, 5, False)
+    dW = ctx.alloc(8 * LW)
+    ctx.h2d(dW, W)
+    ldb = bl + bu + 1
+    dB = ctx.alloc(8 * ldb * (lmax + 1))
+    f = lambda: ctx.check(L.psx_quickpol_xi_d(ctx.handle, *QP_SPINS, lmax, bl, bu, dW, LW, dB),
+                          "psx_quickpol_xi_d")
+    dt, ks = _timed(ctx, f, ["psx_quickpol_xi"], args.steps, args.warmup)
+    ctx.free(dW)
+    ctx.free(dB)
+    ctx.close()
+    fam, con = qp_terms(lmax, bl, bu, LW, QP_SPINS)
+    ms = ks["psx_quickpol_xi"]
+    flops = 22 * fam + 3 * con
+    tf = flops / (ms * 1e-3) / 1e12
+    cells = sum(max(0, min(lmax, lpp + bu) - max(2, lpp - bl) + 1) for lpp in range(2, lmax + 1))
+    return _line(f"QuickPol Ξ band cells/s at lmax {lmax}", cells * args.steps / dt, "cells/s",
+                 args, dt / args.steps * 1e3,
+                 {"workload": f"quickpol_lmax{lmax}: 41-diagonal Ξ band, nu (2, -2), s (2, 2), "
+                              f"LW {LW}", "lmax": lmax, "bl": bl, "bu": bu},
+                 kernels={"psx_quickpol_xi": {"ms": ms, "tflops": tf,
+                                              "frac": tf / FP64_PEAK_TFLOPS,
+                                              "family_terms": fam, "contracted_terms": con,
+                                              "flops": "22 per family term + 3 per contracted "
+                                                       "term (sqrt and div count 1)"}})
+
+
 def run_e2e(args, pkg) -> dict:
     """The headline workload through the host-pointer entry point psx_mcm_set:
     W copied in, five blocks computed and copied out each step (PCIe
@@ -513,7 +570,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
-                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht"],
+                    choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
+                             "quickpol"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -528,7 +586,8 @@ def main() -> int:
     pkg = load_package()
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
-              "e2e": run_e2e, "serve": run_serve, "sht": run_sht}[args.workload]
+              "e2e": run_e2e, "serve": run_serve, "sht": run_sht,
+              "quickpol": run_quickpol}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]

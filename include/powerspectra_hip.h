@@ -200,7 +200,7 @@ int psx_alm2map_pol(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* 
  * W is 0-based with LW entries (quickpolW of the two scan-weight alms);
  * terms with l' >= LW count as zero (the reference reads W[l'] unchecked).
  * A family with |s| > l or |nu| > l'' is zero.  lmax <= 65535, bandwidths
- * 0..lmax, spins within -64..64. */
+ * 0..lmax, spins within -8..8. */
 int psx_quickpol_xi(psx_ctx* ctx, int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu,
                     const double* W, int64_t LW, double* band);
 /* The same on device buffers, enqueued on the context stream (returns when

@@ -13,7 +13,11 @@
 // start are uniform across the wavefront and only the end (l + l'') differs.
 // Each family runs the Schulten-Gordon three-term recurrence in j forward
 // from its lowest value (f(nmin) = 1), and the sum_j W f1 f2 and both
-// normalisation sums are accumulated on the fly, so nothing is stored:
+// normalisation sums are accumulated on the fly, so nothing is stored.  The
+// factor of A(j+1) that depends on j, d and M1 only is the same for every
+// lane: the wavefront computes 64 steps of it at a time into LDS (one step
+// per lane) together with those steps' W values, which leaves one reciprocal
+// square root per lane and step and no global load in the recurrence:
 //   j A(j+1) f(j+1) + Y(j) f(j) + (j+1) A(j) f(j-1) = 0,
 //   A(j) = sqrt((j^2 - d^2)((l+l''+1)^2 - j^2)(j^2 - M1^2)),
 //   Y(j) = -(2j+1)[M1 (l(l+1) - l''(l''+1)) - j(j+1)(M3 - M2)].
@@ -44,29 +48,37 @@ struct QpArgs {
 
 constexpr double kQpBig = 1e150;
 constexpr double kQpTiny = 1e-150;
+constexpr int kQpSteps = 64;      // recurrence steps per LDS table refill (= lanes)
 
 struct QpState {
   double f = 1.0, p = 0.0, A = 0.0, S = 0.0;
 };
 
-// f(j+1) from f(j), f(j-1); A holds A(j) on entry and A(j+1) on return.
-__device__ __forceinline__ void qp_step(QpState& s, int j, double jp2, double t, double rv,
-                                        double m2, double c, double e, double& D) {
-  const double jp = (double)j + 1.0;
-  const double An = sqrt(t * (jp2 - m2)) * rv;
-  double fn;
-  if (j == 0) {
-    // nmin = 0 (l == l'', M1 == 0): the recurrence divided by j gives
-    // A(1) f(1) + (M3 - M2) f(0) = 0
-    fn = -e / An;
-  } else {
-    const double Y = -(2.0 * j + 1.0) * (c - (double)j * jp * e);
-    fn = -(Y * s.f + jp * s.A * s.p) / ((double)j * An);
-  }
+// f(j+1) from f(j), f(j-1) for j >= 1.  au = sqrt(((j+1)^2-d^2)((j+1)^2-M1^2))
+// and rj = 1/(j au) are the wavefront-uniform parts of A(j+1) = au rv and of
+// 1/(j A(j+1)) = rj r, with rv = sqrt((l+l''+1)^2-(j+1)^2) and r = 1/rv per
+// lane; s.A holds A(j) on entry and A(j+1) on return.  jp = j + 1,
+// tj = 2j + 1, kj = (2j+1) j (j+1).
+__device__ __forceinline__ void qp_adv(QpState& s, double au, double rj, double rv, double r,
+                                       double jp, double tj, double kj, double c, double e) {
+  const double Y = fma(kj, e, -tj * c);        // -(2j+1)[c - j(j+1) e]
+  const double fn = -fma(Y, s.f, jp * s.A * s.p) * (rj * r);
   s.p = s.f;
   s.f = fn;
+  s.A = au * rv;
+}
+
+// The step out of j = 0 (nmin = 0: l == l'', M1 == 0), where the recurrence
+// divided by j gives A(1) f(1) + (M3 - M2) f(0) = 0.
+__device__ __forceinline__ void qp_adv0(QpState& s, double au, double rv, double e) {
+  const double An = au * rv;
+  s.p = s.f;
+  s.f = -e / An;
   s.A = An;
-  if (fabs(fn) > kQpBig) {
+}
+
+__device__ __forceinline__ void qp_rescale(QpState& s, double& D) {
+  if (fmax(fabs(s.f), fabs(s.p)) > kQpBig) {
     s.f *= kQpTiny;
     s.p *= kQpTiny;
     s.S *= kQpTiny * kQpTiny;
@@ -74,26 +86,45 @@ __device__ __forceinline__ void qp_step(QpState& s, int j, double jp2, double t,
   }
 }
 
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
 __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
-  const int d = (int)blockIdx.x - A.bl;                              // l - l''
-  const int lpp = A.lmax - ((int)blockIdx.y * 64 + (int)threadIdx.x);  // l'', descending
+  __shared__ double tab[5][kQpSteps];   // au_a, rj_a, au_b, rj_b, W per step
+  const int d = (int)blockIdx.x - A.bl;                                   // l - l''
+  const int lpp0 = A.lmax - (int)blockIdx.y * 64;                         // lane 0's l''
+  const int lane = (int)threadIdx.x;
+  const int lpp = lpp0 - lane;                                            // l'', descending
   const int l = lpp + d;
-  if (lpp < 0 || l < 0 || l > A.lmax) return;
-  double* cell = A.band + (A.bu - d) + (int64_t)l * A.ldb;
-  if (lpp < 2 || l < 2) {        // not written by the loop, only signed
-    *cell *= A.sgn;
+  if (lpp0 < 2 || lpp0 + d < 2) {
+    // no lane of this wavefront has l'' >= 2 and l >= 2 (lane 0 has the largest)
+    if (lpp >= 0 && l >= 0 && l <= A.lmax) A.band[(A.bu - d) + (int64_t)l * A.ldb] *= A.sgn;
     return;
   }
-  const int nmax = l + lpp;
+  const bool inmat = lpp >= 0 && l >= 0 && l <= A.lmax;
+  double* cell = A.band + (A.bu - d) + (int64_t)(inmat ? l : 0) * A.ldb;
+  bool live = false;
+  if (inmat) {
+    if (lpp < 2 || l < 2) {
+      *cell *= A.sgn;                             // not written by the loop, only signed
+    } else if (abs(A.a.M2) > l || abs(A.a.M3) > lpp || abs(A.b.M2) > l || abs(A.b.M3) > lpp) {
+      *cell = 0.0 * A.sgn;                        // no valid 3j symbol: the family is zero
+    } else {
+      live = true;
+    }
+  }
   const int ad = d < 0 ? -d : d;
   const int na = max(ad, abs(A.a.M1)), nb = max(ad, abs(A.b.M1));
-  const bool va = abs(A.a.M2) <= l && abs(A.a.M3) <= lpp;
-  const bool vb = abs(A.b.M2) <= l && abs(A.b.M3) <= lpp;
-  if (!va || !vb) {              // no valid 3j symbol: the family is zero
-    *cell = 0.0 * A.sgn;
-    return;
-  }
-  const double L2 = (double)l * (l + 1) - (double)lpp * (lpp + 1);
+  const int jc = max(na, nb);
+  const int jend = 2 * lpp0 + d;                  // largest l + l'' of the wavefront
+  // lanes without a cell run the recurrence of lane 0's cell and discard it
+  const int nmax = live ? l + lpp : jend;
+  const int ell = live ? l : lpp0 + d, ellpp = live ? lpp : lpp0;
+  const int nmin_end = wave_min(live ? nmax : jend);   // every lane is active below it
+  const double L2 = (double)ell * (ell + 1) - (double)ellpp * (ellpp + 1);
   const double ca = A.a.M1 * L2, cb = A.b.M1 * L2;
   const double ea = A.a.M3 - A.a.M2, eb = A.b.M3 - A.b.M2;
   const double ma2 = (double)A.a.M1 * A.a.M1, mb2 = (double)A.b.M1 * A.b.M1;
@@ -101,21 +132,92 @@ __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
   const double d2 = (double)d * d;
   QpState fa, fb;
   double D = 0.0;
-  const int j0 = min(na, nb), jc = max(na, nb);
-  const int64_t wend = A.LW;
-  for (int j = j0;; ++j) {
-    const double tj = 2.0 * j + 1.0;
-    if (j >= na) fa.S = fma(tj * fa.f, fa.f, fa.S);
-    if (j >= nb) fb.S = fma(tj * fb.f, fb.f, fb.S);
-    if (j >= jc && j < wend) D = fma(A.W[j] * fa.f, fb.f, D);
-    if (j == nmax) break;
-    const double jp = (double)j + 1.0;
-    const double jp2 = jp * jp;
-    const double rv = sqrt(s2 - jp2);
-    const double t = jp2 - d2;
-    if (j >= na) qp_step(fa, j, jp2, t, rv, ma2, ca, ea, D);
-    if (j >= nb) qp_step(fb, j, jp2, t, rv, mb2, cb, eb, D);
+
+  // prologue: the family with the lower start runs alone up to jc (at most
+  // a few steps, |M1a| - |M1b| apart), then both stand at j = jc
+  {
+    QpState& f1 = na < nb ? fa : fb;
+    const double m2 = na < nb ? ma2 : mb2;
+    const double c1 = na < nb ? ca : cb, e1 = na < nb ? ea : eb;
+    for (int j = min(na, nb); j < jc; ++j) {
+      const double jp = j + 1.0, tj = 2.0 * j + 1.0;
+      f1.S = fma(tj * f1.f, f1.f, f1.S);
+      const double v = s2 - jp * jp;
+      const double rv = sqrt(v);
+      const double au = sqrt((jp * jp - d2) * (jp * jp - m2));
+      if (j == 0) qp_adv0(f1, au, rv, e1);
+      else qp_adv(f1, au, 1.0 / ((double)j * au), rv, 1.0 / rv, jp, tj, tj * j * jp, c1, e1);
+    }
   }
+  int j1 = jc;
+  if (jc == 0) {
+    // both families start at 0 (l == l'', M1 = 0 for both): peel the j = 0 step
+    fa.S = fa.f * fa.f;
+    fb.S = fb.f * fb.f;
+    if (A.LW > 0) D = A.W[0] * fa.f * fb.f;
+    const double rv = sqrt(s2 - 1.0);
+    qp_adv0(fa, sqrt((1.0 - d2) * (1.0 - ma2)), rv, ea);
+    qp_adv0(fb, sqrt((1.0 - d2) * (1.0 - mb2)), rv, eb);
+    j1 = 1;
+  }
+
+  // main part: both families step together, uniform j; steps below
+  // nmin_end need no per-lane test
+  for (int jb = j1; jb <= jend; jb += kQpSteps) {
+    {
+      const int j = jb + lane;
+      const double jp2 = ((double)j + 1.0) * ((double)j + 1.0);
+      const double t = jp2 - d2;
+      double aa = 0.0, ra = 0.0, ab = 0.0, rb = 0.0;
+      if (j < jend) {
+        aa = sqrt(t * (jp2 - ma2));
+        ra = 1.0 / ((double)j * aa);
+        ab = sqrt(t * (jp2 - mb2));
+        rb = 1.0 / ((double)j * ab);
+      }
+      const double w = j <= jend && j < A.LW ? A.W[j] : 0.0;
+      __syncthreads();
+      tab[0][lane] = aa;
+      tab[1][lane] = ra;
+      tab[2][lane] = ab;
+      tab[3][lane] = rb;
+      tab[4][lane] = w;
+      __syncthreads();
+    }
+    const int tend = min(kQpSteps, jend - jb + 1);
+    const int tm = max(0, min(tend, nmin_end - jb));   // steps with j < nmin_end
+    for (int t = 0; t < tm; ++t) {
+      const int j = jb + t;
+      const double jp = j + 1.0, tj = 2.0 * j + 1.0;
+      fa.S = fma(tj * fa.f, fa.f, fa.S);
+      fb.S = fma(tj * fb.f, fb.f, fb.S);
+      D = fma(tab[4][t] * fa.f, fb.f, D);      // W is zero beyond LW
+      const double v = s2 - jp * jp;            // (l+l''+1)^2 - (j+1)^2 > 0
+      const double r = rsqrt(v);
+      const double rv = v * r;
+      const double kj = tj * (double)j * jp;
+      qp_adv(fa, tab[0][t], tab[1][t], rv, r, jp, tj, kj, ca, ea);
+      qp_adv(fb, tab[2][t], tab[3][t], rv, r, jp, tj, kj, cb, eb);
+    }
+    for (int t = tm; t < tend; ++t) {          // the last steps, lane by lane
+      const int j = jb + t;
+      if (j > nmax) break;
+      const double jp = j + 1.0, tj = 2.0 * j + 1.0;
+      fa.S = fma(tj * fa.f, fa.f, fa.S);
+      fb.S = fma(tj * fb.f, fb.f, fb.S);
+      D = fma(tab[4][t] * fa.f, fb.f, D);
+      if (j == nmax) break;
+      const double v = s2 - jp * jp;
+      const double r = rsqrt(v);
+      const double rv = v * r;
+      const double kj = tj * (double)j * jp;
+      qp_adv(fa, tab[0][t], tab[1][t], rv, r, jp, tj, kj, ca, ea);
+      qp_adv(fb, tab[2][t], tab[3][t], rv, r, jp, tj, kj, cb, eb);
+    }
+    qp_rescale(fa, D);
+    qp_rescale(fb, D);
+  }
+  if (!live) return;
   // normalise, fixing the sign of f(l + l'') to (-1)^(l - l'' + M1)
   double ka = 1.0 / sqrt(fa.S), kb = 1.0 / sqrt(fb.S);
   if ((fa.f < 0.0) != (((l - lpp + A.a.M1) & 1) != 0)) ka = -ka;
@@ -129,8 +231,10 @@ void check_qp(int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu, int64_
               "bandwidths must be 0..lmax");
   PSX_REQUIRE(LW >= 0, PSX_ERR_ARGUMENT, "W length must be >= 0");
   const int sp[4] = {nu1, nu2, s1, s2};
+  // the forward recurrence and its once-per-64-steps overflow check are
+  // sized for scan-pattern spins
   for (int v : sp)
-    PSX_REQUIRE(v >= -64 && v <= 64, PSX_ERR_ARGUMENT, "spins must be within -64..64");
+    PSX_REQUIRE(v >= -8 && v <= 8, PSX_ERR_ARGUMENT, "spins must be within -8..8");
 }
 
 void quickpol_xi_dev(Ctx* c, int nu1, int nu2, int s1, int s2, int lmax, int bl, int bu,
