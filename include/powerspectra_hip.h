@@ -160,6 +160,15 @@ int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
  * (n x nrhs, overwritten with the solution).  A is not modified. */
 int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
                  double* B, int64_t ldb, int nrhs);
+/* `lu(A)` (reference src/blockspectralmatrix.jl:89-99, Julia's LAPACK getrf
+ * with partial pivoting): LU (n x n, leading dimension ldlu) receives the
+ * unit-lower L below the diagonal and U on and above it, ipiv[n] the 1-based
+ * row interchanges (row j was swapped with row ipiv[j] at step j).  A zero
+ * pivot returns PSX_ERR_SINGULAR (Julia SingularException) after writing both.
+ * Ties in |pivot| go to the lower row index.  The factorisation behind
+ * psx_lu_solve and psx_decouple_covmat. */
+int psx_lu_factor(psx_ctx* ctx, int n, const double* A, int64_t lda, double* LU,
+                  int64_t ldlu, int* ipiv);
 
 /* ---- spherical-harmonic transforms (mask -> a_lm -> W_l front end) ----- */
 /* Analysis of `nmaps` ring-ordered HEALPix maps (12 nside^2 doubles each,

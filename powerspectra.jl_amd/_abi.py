@@ -92,6 +92,7 @@ _int = ctypes.c_int
 SIGNATURES = {
     "psx_map2alm": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map": (_int, [_vp, _int, _int, _int, _dp, _dp]),
+    "psx_lu_factor": (_int, [_vp, _int, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_int)]),
     "psx_quickpol_xi": (_int, [_vp] + [_int] * 7 + [_dp, _i64, _dp]),
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
@@ -365,6 +366,19 @@ class Context:
         self.check(self.lib.psx_quickpol_xi(self._h, nu1, nu2, s1, s2, lmax, bl, bu, dptr(W),
                                             W.size, dptr(band)), "psx_quickpol_xi")
         return band
+
+    def lu_factor(self, A):
+        """(LU, ipiv) of a square matrix: LAPACK getrf layout, ipiv 1-based."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        if A.shape != (n, n):
+            raise AxisError("lu_factor needs a square matrix")
+        LU = np.empty((n, n), order="F")
+        ipiv = np.empty(n, dtype=np.int32)
+        self.check(self.lib.psx_lu_factor(self._h, n, dptr(A), n, dptr(LU), n,
+                                          ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                   "psx_lu_factor")
+        return LU, ipiv
 
     def decouple_covmat(self, Y, B1, B2):
         Y = np.asfortranarray(Y, dtype=np.float64)

@@ -1,8 +1,10 @@
-// FP64 LU decoupling on rocSOLVER: `A \ B` (reference
+// FP64 LU decoupling: `lu` and `A \ B` (reference
 // src/blockspectralmatrix.jl:89-129) and decouple_covmat
 // (src/covariance.jl:8-14).  Julia's `lu` is LAPACK getrf with partial
-// pivoting and throws SingularException on an exact zero pivot; rocSOLVER's
-// getrf reports the same condition through `info`.
+// pivoting and throws SingularException on an exact zero pivot.  The
+// factorisation is this file's blocked getrf (HIP leaf kernels + rocBLAS
+// updates), the triangular solves rocSOLVER's getrs on its LAPACK-layout
+// output.
 #include "psx_internal.h"
 
 #include <rocblas/rocblas.h>
@@ -35,23 +37,335 @@ void release_blas(Ctx* c) {
                 std::string(#call) + " failed: " + rocblas_status_to_string(s_)); \
   } while (0)
 
+// ---- LU with partial pivoting ---------------------------------------------
+// Right-looking blocked getrf.  Outer blocks of kLuPanel columns end with a
+// trailing update by rocBLAS (dtrsm + dgemm of depth kLuPanel).  Inside a
+// block, leaves of kLuLeaf columns are factored one column per launch: the
+// kernel picks the pivot from the candidates the previous launch published
+// (each with the candidate row's remaining leaf entries, so the pivot row
+// needs no dependent read), eliminates it from every other unflagged row of
+// the leaf and publishes the next column's candidates.  The rest of the
+// block is then updated by a dtrsm + dgemm of depth kLuLeaf.  Within a leaf
+// rows are not moved: the pivot row is flagged and left in place, so no
+// launch reads a row another workgroup writes.  After the leaf the pivot
+// sequence is turned into LAPACK's row interchanges (ipiv, 1-based) and
+// applied to every column, which leaves the matrix exactly as getrf's
+// physical swaps would.  Ties in |pivot| go to the smallest row index.  A
+// zero pivot column is skipped and reported (first one, 1-based), as getrf
+// does.  (A single cooperative launch per leaf with a grid barrier per
+// column measured slower: the cross-XCD barrier costs more than a launch.)
+constexpr int kLuLeaf = 32;
+constexpr int kLuPanel = 256;
+constexpr int kLuRows = 256;     // rows per workgroup in the leaf kernels (one per thread)
+
+struct LuWork {
+  // per-workgroup pivot candidates, two buffers of cstride entries: the
+  // launch for column j reads buffer j & 1 and writes the other, since the
+  // workgroups of one launch read every candidate while others write theirs.
+  // Each candidate is |value|, its row, and that row's entries from the
+  // candidate's column to the leaf's end (kLuLeaf slots).
+  double* candv;
+  int* candi;
+  double* candrow;
+  int cstride;
+  unsigned char* done;// rows already chosen as pivots in the current leaf
+  int* piv;           // the leaf's pivot rows (original positions)
+  int* ipiv;          // LAPACK interchanges, 1-based, n entries
+  int* info;          // first zero pivot column (1-based) or 0
+};
+
+// |x| for the pivot search; NaN ranks above everything, so a NaN column
+// still yields a row (and propagates) and the search never comes back empty.
+__device__ inline double lu_mag(double x) { return x != x ? INFINITY : fabs(x); }
+
+__device__ inline void lu_better(double& v, int& r, double ov, int orr) {
+  if (ov > v || (ov == v && orr < r)) {
+    v = ov;
+    r = orr;
+  }
+}
+
+// (value, row) maximum with ties to the smaller row over the workgroup; every
+// thread returns with the result.
+__device__ inline void lu_argmax_block(double& v, int& r) {
+  __shared__ double sv[kLuRows / 64];
+  __shared__ int sr[kLuRows / 64];
+  for (int o = 32; o >= 1; o >>= 1) lu_better(v, r, __shfl_xor(v, o), __shfl_xor(r, o));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sv[w] = v;
+    sr[w] = r;
+  }
+  __syncthreads();
+  v = sv[0];
+  r = sr[0];
+  for (int k = 1; k < kLuRows / 64; ++k) lu_better(v, r, sv[k], sr[k]);
+}
+
+// Publish the workgroup's candidate into buffer `buf`: the winning thread
+// copies its row entries vals[first .. first+count).
+__device__ inline void lu_publish(const LuWork& w, int buf, double v, int r, int i,
+                                  const double* vals, int first, int count) {
+  const int slot = buf * w.cstride + (int)blockIdx.x;
+  if (threadIdx.x == 0) {
+    w.candv[slot] = v;
+    w.candi[slot] = r;
+  }
+  if (i == r) {
+    double* dst = w.candrow + (int64_t)slot * kLuLeaf;
+#pragma unroll
+    for (int k = 0; k < kLuLeaf; ++k)
+      if (k < count) dst[k] = vals[first + k];
+  }
+}
+
+// Rows [c, n): clear the pivot flags and publish the candidates of column c.
+__global__ void __launch_bounds__(kLuRows) lu_leaf_init(double* A, int64_t lda, int n, int c,
+                                                        int cend, LuWork w) {
+  const int i = c + (int)blockIdx.x * kLuRows + (int)threadIdx.x;
+  const int nc = cend - c;
+  double vals[kLuLeaf];
+  double v = -1.0;
+  int r = 0x7fffffff;
+  if (i < n) {
+    w.done[i] = 0;
+#pragma unroll
+    for (int k = 0; k < kLuLeaf; ++k)
+      if (k < nc) vals[k] = A[i + (int64_t)(c + k) * lda];
+    v = lu_mag(vals[0]);
+    r = i;
+  }
+  lu_argmax_block(v, r);
+  lu_publish(w, 0, v, r, i, vals, 0, nc);
+}
+
+// Column cj = c + j of the leaf [c, cend): choose the pivot among the
+// candidates, eliminate it from every other unflagged row of the leaf, and
+// publish the candidates of column cj + 1.
+__global__ void __launch_bounds__(kLuRows) lu_leaf_col(double* A, int64_t lda, int n, int c,
+                                                       int j, int cend, int G, LuWork w) {
+  __shared__ double u[kLuLeaf];
+  __shared__ int ps, gs;
+  const int cj = c + j;
+  const int ncol = cend - cj;                      // columns cj .. cend-1
+  const int i = c + (int)blockIdx.x * kLuRows + (int)threadIdx.x;
+  // independent loads first: this thread's row and flag, the candidates
+  double vals[kLuLeaf];
+  bool mine = false;
+  if (i < n) {
+    mine = !w.done[i];
+#pragma unroll
+    for (int k = 0; k < kLuLeaf; ++k)
+      if (k < ncol) vals[k] = A[i + (int64_t)(cj + k) * lda];
+  }
+  const int rb = (j & 1) * w.cstride;
+  if (threadIdx.x < 64) {
+    double v = -1.0;
+    int r = 0x7fffffff, g = 0;
+    for (int t = (int)threadIdx.x; t < G; t += 64) {
+      const double tv = w.candv[rb + t];
+      const int tr = w.candi[rb + t];
+      if (tv > v || (tv == v && tr < r)) {
+        v = tv;
+        r = tr;
+        g = t;
+      }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double ov = __shfl_xor(v, o);
+      const int orr = __shfl_xor(r, o), og = __shfl_xor(g, o);
+      if (ov > v || (ov == v && orr < r)) {
+        v = ov;
+        r = orr;
+        g = og;
+      }
+    }
+    if (threadIdx.x == 0) {
+      ps = r;
+      gs = g;
+    }
+  }
+  __syncthreads();
+  // rows [cj, n) hold n - cj >= 1 unflagged rows, so the search always
+  // returns one; the clamp only keeps the accesses below in bounds regardless
+  const int p = ps < n ? ps : cj;
+  if ((int)threadIdx.x < ncol)
+    u[threadIdx.x] = w.candrow[(int64_t)(rb + gs) * kLuLeaf + threadIdx.x];
+  __syncthreads();
+  const double piv = u[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.piv[j] = p;
+    if (piv == 0.0 && *w.info == 0) *w.info = cj + 1;
+  }
+  double v = -1.0;
+  int r = 0x7fffffff;
+  if (i < n && mine) {
+    if (i == p) {
+      w.done[i] = 1;
+    } else {
+      if (piv != 0.0) {
+        const double l = vals[0] / piv;
+        vals[0] = l;
+        A[i + (int64_t)cj * lda] = l;
+#pragma unroll
+        for (int k = 1; k < kLuLeaf; ++k)
+          if (k < ncol) {
+            vals[k] = fma(-l, u[k], vals[k]);
+            A[i + (int64_t)(cj + k) * lda] = vals[k];
+          }
+      }
+      if (ncol > 1) {
+        v = lu_mag(vals[1]);
+        r = i;
+      }
+    }
+  }
+  if (ncol > 1) {
+    lu_argmax_block(v, r);
+    lu_publish(w, (j + 1) & 1, v, r, i, vals, 1, ncol - 1);
+  }
+}
+
+// LAPACK interchanges of the leaf: at step j the row chosen (original
+// position piv[j]) is swapped with whatever row is at position c + j.  Only
+// positions touched by this leaf's swaps differ from the identity; one
+// wavefront keeps them as (position, row) pairs, one pair per lane, and finds
+// entries by ballot.
+__global__ void __launch_bounds__(64) lu_leaf_ipiv(int c, int nc, LuWork w) {
+  const int lane = (int)threadIdx.x;
+  int key = -1, val = -1, m = 0;
+  for (int j = 0; j < nc; ++j) {
+    const int want = w.piv[j];
+    const int pj = c + j;
+    const unsigned long long hv = __ballot(key >= 0 && val == want);
+    const int q = hv ? __shfl(key, __ffsll((long long)hv) - 1) : want;
+    const unsigned long long hp = __ballot(key == pj);
+    const int at_pj = hp ? __shfl(val, __ffsll((long long)hp) - 1) : pj;
+    if (lane == 0) w.ipiv[pj] = q + 1;
+    // position pj now holds `want`, position q the row that was at pj
+    if (hp) {
+      if (key == pj) val = want;
+    } else {
+      if (lane == m) {
+        key = pj;
+        val = want;
+      }
+      ++m;
+    }
+    if (q != pj) {
+      const unsigned long long hq = __ballot(key == q);
+      if (hq) {
+        if (key == q) val = at_pj;
+      } else {
+        if (lane == m) {
+          key = q;
+          val = at_pj;
+        }
+        ++m;
+      }
+    }
+  }
+}
+
+// Apply the leaf's interchanges (positions c .. c+nc-1) to every column.
+__global__ void __launch_bounds__(256) lu_laswp(double* A, int64_t lda, int ncols, int c, int nc,
+                                                const int* ipiv) {
+  const int col = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (col >= ncols) return;
+  double* a = A + (int64_t)col * lda;
+  for (int j = 0; j < nc; ++j) {
+    const int q = ipiv[c + j] - 1;
+    if (q != c + j) {
+      const double t = a[c + j];
+      a[c + j] = a[q];
+      a[q] = t;
+    }
+  }
+}
+
+void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinfo) {
+  rocblas_handle h = blas(c);
+  hipStream_t s = c->stream;
+  const int Gmax = (n + kLuRows - 1) / kLuRows;
+  const size_t rows_bytes = 2 * (size_t)Gmax * kLuLeaf * sizeof(double);
+  char* base = (char*)ctx_scratch(c, rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf + (size_t)n +
+                                         256);
+  LuWork w;
+  w.cstride = Gmax;
+  w.candrow = (double*)base;                                  // 2 Gmax kLuLeaf doubles
+  w.candv = (double*)(base + rows_bytes);                     // 2 Gmax doubles
+  w.candi = (int*)(base + rows_bytes + 16 * (size_t)Gmax);    // 2 Gmax ints
+  w.piv = (int*)(base + rows_bytes + 24 * (size_t)Gmax);      // kLuLeaf ints
+  w.done = (unsigned char*)(base + rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf);  // n flags
+  w.ipiv = dipiv;
+  w.info = dinfo;
+  PSX_HIP(hipMemsetAsync(dinfo, 0, sizeof(int), s));
+  const double one = 1.0, mone = -1.0;
+  for (int k = 0; k < n; k += kLuPanel) {
+    const int kend = std::min(n, k + kLuPanel);
+    for (int cl = k; cl < kend; cl += kLuLeaf) {
+      const int cend = std::min(kend, cl + kLuLeaf);
+      const int nc = cend - cl;
+      const int G = (n - cl + kLuRows - 1) / kLuRows;
+      hipLaunchKernelGGL(lu_leaf_init, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, cend, w);
+      for (int j = 0; j < nc; ++j)
+        hipLaunchKernelGGL(lu_leaf_col, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, j, cend, G,
+                           w);
+      hipLaunchKernelGGL(lu_leaf_ipiv, dim3(1), dim3(64), 0, s, cl, nc, w);
+      hipLaunchKernelGGL(lu_laswp, dim3((n + 255) / 256), dim3(256), 0, s, dA, lda, n, cl, nc,
+                         (const int*)dipiv);
+      PSX_HIP(hipGetLastError());
+      if (cend < kend) {   // rest of the block: U = L11^-1 A, then the update
+        double* L11 = dA + cl + (int64_t)cl * lda;
+        double* A12 = dA + cl + (int64_t)cend * lda;
+        PSX_ROC(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                              rocblas_diagonal_unit, nc, kend - cend, &one, L11, (rocblas_int)lda,
+                              A12, (rocblas_int)lda));
+        if (cend < n)
+          PSX_ROC(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n - cend,
+                                kend - cend, nc, &mone, dA + cend + (int64_t)cl * lda,
+                                (rocblas_int)lda, A12, (rocblas_int)lda, &one,
+                                dA + cend + (int64_t)cend * lda, (rocblas_int)lda));
+      }
+    }
+    if (kend < n) {        // trailing matrix
+      double* L11 = dA + k + (int64_t)k * lda;
+      double* A12 = dA + k + (int64_t)kend * lda;
+      PSX_ROC(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                            rocblas_diagonal_unit, kend - k, n - kend, &one, L11, (rocblas_int)lda,
+                            A12, (rocblas_int)lda));
+      PSX_ROC(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n - kend, n - kend,
+                            kend - k, &mone, dA + kend + (int64_t)k * lda, (rocblas_int)lda, A12,
+                            (rocblas_int)lda, &one, dA + kend + (int64_t)kend * lda,
+                            (rocblas_int)lda));
+    }
+  }
+}
+
 // dA (n x n, lda) is overwritten by its LU factors; dB (n x nrhs) by X.
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                   int nrhs) {
   rocblas_handle h = blas(c);
-  char* s = (char*)ctx_scratch(c, sizeof(rocblas_int) * (n + 64));
-  rocblas_int* ipiv = (rocblas_int*)s;
-  rocblas_int* info = ipiv + n + 32;
-  PSX_ROC(rocsolver_dgetrf(h, n, n, dA, (rocblas_int)lda, ipiv, info));
-  rocblas_int hinfo = 0;
-  PSX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c->stream));
-  PSX_HIP(hipStreamSynchronize(c->stream));
-  PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
-              "LU: zero pivot at position " + std::to_string(hinfo) +
-                  " (Julia SingularException)");
-  if (nrhs > 0)
-    PSX_ROC(rocsolver_dgetrs(h, rocblas_operation_none, n, nrhs, dA, (rocblas_int)lda,
-                             ipiv, dB, (rocblas_int)ldb));
+  int* ipiv = (int*)ctx_alloc(c, sizeof(int) * ((size_t)n + 64));
+  try {
+    int* info = ipiv + n + 32;
+    lu_factor_dev(c, n, dA, lda, ipiv, info);
+    int hinfo = 0;
+    PSX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
+                "LU: zero pivot at position " + std::to_string(hinfo) +
+                    " (Julia SingularException)");
+    if (nrhs > 0)
+      PSX_ROC(rocsolver_dgetrs(h, rocblas_operation_none, n, nrhs, dA, (rocblas_int)lda,
+                               (const rocblas_int*)ipiv, dB, (rocblas_int)ldb));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, ipiv);
+    throw;
+  }
+  ctx_free(c, ipiv);
 }
 
 static void transpose(Ctx* c, int n, const double* dIn, double* dOut) {
@@ -114,6 +428,39 @@ int psx_lu_solve(psx_ctx* hctx, int n, const double* A, int64_t lda, double* B,
     }
     ctx_free(c, dA);
     ctx_free(c, dB);
+  });
+}
+
+int psx_lu_factor(psx_ctx* hctx, int n, const double* A, int64_t lda, double* LU, int64_t ldlu,
+                  int* ipiv) {
+  Ctx* c = reinterpret_cast<Ctx*>(hctx);
+  return guarded(c, [&] {
+    PSX_REQUIRE(n >= 1 && lda >= n && ldlu >= n, PSX_ERR_ARGUMENT, "bad sizes for lu_factor");
+    PSX_REQUIRE(A && LU && ipiv, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const size_t nn = (size_t)n * n;
+    double* dA = (double*)ctx_alloc(c, sizeof(double) * nn);
+    int* dp = nullptr;
+    try {
+      dp = (int*)ctx_alloc(c, sizeof(int) * ((size_t)n + 64));
+      h2d_matrix(c, dA, n, A, lda, n, n);
+      lu_factor_dev(c, n, dA, n, dp, dp + n + 32);
+      int hinfo = 0;
+      PSX_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+      PSX_HIP(hipMemcpyAsync(&hinfo, dp + n + 32, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      d2h_matrix(c, LU, ldlu, dA, n, n, n);
+      PSX_HIP(hipStreamSynchronize(c->stream));
+      PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
+                  "LU: zero pivot at position " + std::to_string(hinfo) +
+                      " (Julia SingularException)");
+    } catch (...) {
+      (void)hipStreamSynchronize(c->stream);
+      ctx_free(c, dA);
+      if (dp) ctx_free(c, dp);
+      throw;
+    }
+    ctx_free(c, dA);
+    ctx_free(c, dp);
   });
 }
 

@@ -66,3 +66,36 @@ def test_singular(ctx, pkg):
         ctx.lu_solve(A, np.ones(4))
     with pytest.raises(pkg.SingularException):
         ctx.decouple_covmat(np.eye(4), np.zeros((4, 4)), np.eye(4))
+
+
+@pytest.mark.parametrize("n", [1, 5, 33, 300, 513, 1000, 2100])
+def test_lu_factor_vs_lapack(ctx, n):
+    """The blocked getrf (leaf kernels + rocBLAS updates) against LAPACK's
+    getrf (scipy): the same row interchanges and factors to rounding."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    LU, ipiv = ctx.lu_factor(A)
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert np.array_equal(ipiv - 1, piv_ref)
+    assert np.linalg.norm(LU - lu_ref) / np.linalg.norm(lu_ref) < 1e-12
+
+
+def test_lu_factor_mcm_and_solve(ctx):
+    """A TT mode-coupling matrix at lmax 1500 (no row interchanges expected
+    beyond LAPACK's): factors as LAPACK, and `M \\ b` through psx_lu_solve."""
+    import scipy.linalg as sl
+    lmax = 1500
+    M = ctx.mcm(0, 0, lmax, mask_like_W(lmax + 1, seed=3))
+    LU, ipiv = ctx.lu_factor(M)
+    lu_ref, piv_ref = sl.lu_factor(M)
+    assert np.array_equal(ipiv - 1, piv_ref)
+    assert np.linalg.norm(LU - lu_ref) / np.linalg.norm(lu_ref) < 1e-12
+    b = M @ (1.0 / (np.arange(lmax + 1) + 1.0) ** 2)
+    x = ctx.lu_solve(M, b)
+    assert np.linalg.norm(x - sl.lu_solve((lu_ref, piv_ref), b)) / np.linalg.norm(x) < 1e-12
+
+
+def test_lu_factor_singular(ctx, pkg):
+    with pytest.raises(pkg.SingularException):
+        ctx.lu_factor(np.ones((40, 40)))
