@@ -44,6 +44,7 @@ constexpr int kMaxPolarRing = 8192;         // pixels of the longest polar ring 
 constexpr int kLChunk = 8;                  // l values per LDS reduction round
 constexpr int kPairsPerThread = 4;          // ring pairs per analysis thread
 constexpr int kSynPairs = 4;                // ring pairs per synthesis thread
+constexpr int kSynChunk = 4;                // l values read ahead per synthesis round
 constexpr double kBig = 0x1p600;            // rescaling threshold
 constexpr double kInvBig = 0x1p-600;
 constexpr double kLnBig = 415.88830833596718565;  // 600 ln 2
@@ -254,6 +255,26 @@ __device__ __forceinline__ double2 sload(const double2* p) {
   return make_double2(sload(q), sload(q + 1));
 }
 
+// Weight of a scaled value mant 2^(600 s) in a sum: 1, 2^-600, or 0 below
+// 2^-1200 (too small to matter next to O(1) terms).
+__device__ __forceinline__ double scale_weight(int s) {
+  return s == 0 ? 1.0 : (s == -1 ? kInvBig : 0.0);
+}
+
+// Rescaling runs once per chunk of kLChunk steps, so the steps themselves
+// carry no branch: from a mantissa below 2^600 the recurrence grows by less
+// than 2^48 over 8 steps (sqrt(2m+3) sqrt(m) sqrt(2m/3) ... < 2^48 for
+// m <= 8191), far from double overflow, and a value that crosses 2^600 inside
+// a chunk is still weighted exactly (or dropped when it is below 2^-600).
+__device__ __forceinline__ void rescale_pair(double& a, double& b, int& s, double& wgt) {
+  if (s < 0 && fabs(a) > kBig) {
+    a *= kInvBig;
+    b *= kInvBig;
+    s += 1;
+    wgt = scale_weight(s);
+  }
+}
+
 // Analysis partial sums: part[chunk][idx(l, m)] = sum over the chunk's pairs of
 // lambda_lm(z_p) F+-_p(m).  Each thread carries kPairsPerThread independent
 // recurrences; per round of kLChunk l values the 256 partial sums of every
@@ -267,7 +288,7 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
   const int lmax = A.R.lmax;
   const int tid = threadIdx.x;
   const double2* ABm = A.AB + alm_idx(lmax, m, m);   // ABm[l - m]
-  double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread];
+  double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread], wgt[kPairsPerThread];
   int sc[kPairsPerThread];
   double2 fp[kPairsPerThread], fm[kPairsPerThread];
 #pragma unroll
@@ -277,6 +298,7 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
     const int pp = live ? p : 0;
     z[r] = A.R.z[pp];
     lambda_start(A, pp, m, l1[r], sc[r]);
+    wgt[r] = scale_weight(sc[r]);
     l2[r] = 0.0;
     const int64_t o = (int64_t)m * A.R.npairs + pp;
     fp[r] = live ? Fp[o] : make_double2(0.0, 0.0);
@@ -288,8 +310,8 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
       const int l = l0 + j;
       double accr = 0.0, acci = 0.0;
       if (l <= lmax) {
-        const double2 ab = l > m ? sload(ABm + (l - m)) : make_double2(0.0, 0.0);
-        const bool odd = (l - m) & 1;
+        const double2 ab = sload(ABm + (l - m));   // (0, 0) at l = m
+        const bool odd = j & 1;                    // l0 - m is a multiple of kLChunk
 #pragma unroll
         for (int r = 0; r < kPairsPerThread; ++r) {
           double lam;
@@ -300,23 +322,17 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
             l2[r] = l1[r];
             l1[r] = lam;
           }
-          if (sc[r] < 0 && fabs(lam) > kBig) {
-            l1[r] *= kInvBig;
-            l2[r] *= kInvBig;
-            lam *= kInvBig;
-            sc[r] += 1;
-          }
-          if (sc[r] >= -1) {
-            const double lt = sc[r] == 0 ? lam : lam * kInvBig;
-            const double2 f = odd ? fm[r] : fp[r];
-            accr = fma(lt, f.x, accr);
-            acci = fma(lt, f.y, acci);
-          }
+          const double lt = lam * wgt[r];
+          const double2 f = odd ? fm[r] : fp[r];
+          accr = fma(lt, f.x, accr);
+          acci = fma(lt, f.y, acci);
         }
       }
       red[j][0][tid] = accr;
       red[j][1][tid] = acci;
     }
+#pragma unroll
+    for (int r = 0; r < kPairsPerThread; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
     __syncthreads();
     // 2 kLChunk = 16 outputs, 16 threads each: 16 values per thread, then a
     // reduction over the 16 lanes of the group
@@ -360,8 +376,10 @@ __global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ 
 }
 
 // Synthesis: G[m][k] = sum_l a_lm lambda_lm(z_k) for kSynPairs ring pairs per
-// thread (north: even + odd l+m parts, south: even - odd); a_lm is uniform
-// across the workgroup and read with scalar loads.
+// thread (north: even + odd l+m parts, south: even - odd); a_lm and the
+// recurrence coefficients are uniform across the workgroup and read with
+// scalar loads, kSynChunk at a time ahead of the steps that use them (a
+// rescale check per such round: growth over 4 steps stays below 2^26).
 __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const double2* __restrict__ alm,
                                                           double2* __restrict__ G) {
   const int m = blockIdx.y;
@@ -369,7 +387,7 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const doubl
   const int64_t i0 = alm_idx(lmax, m, m);
   const double2* am = alm + i0;
   const double2* ABm = A.AB + i0;
-  double z[kSynPairs], l1[kSynPairs], l2[kSynPairs];
+  double z[kSynPairs], l1[kSynPairs], l2[kSynPairs], wgt[kSynPairs];
   double er[kSynPairs], ei[kSynPairs], orr[kSynPairs], oi[kSynPairs];
   int sc[kSynPairs], pr[kSynPairs];
 #pragma unroll
@@ -379,40 +397,46 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const doubl
     const int pp = p < A.R.npairs ? p : 0;
     z[r] = A.R.z[pp];
     lambda_start(A, pp, m, l1[r], sc[r]);
+    wgt[r] = scale_weight(sc[r]);
     l2[r] = 0.0;
     er[r] = ei[r] = orr[r] = oi[r] = 0.0;
   }
-  for (int l = m; l <= lmax; ++l) {
-    const double2 a = sload(am + (l - m));
-    const double2 ab = l > m ? sload(ABm + (l - m)) : make_double2(0.0, 0.0);
-    const bool odd = (l - m) & 1;
+  for (int l0 = m; l0 <= lmax; l0 += kSynChunk) {
+    double2 a[kSynChunk], ab[kSynChunk];
 #pragma unroll
-    for (int r = 0; r < kSynPairs; ++r) {
-      double lam;
-      if (l == m) {
-        lam = l1[r];
-      } else {
-        lam = l == m + 1 ? ab.x * z[r] * l1[r] : ab.x * fma(z[r], l1[r], -ab.y * l2[r]);
-        l2[r] = l1[r];
-        l1[r] = lam;
-      }
-      if (sc[r] < 0 && fabs(lam) > kBig) {
-        l1[r] *= kInvBig;
-        l2[r] *= kInvBig;
-        lam *= kInvBig;
-        sc[r] += 1;
-      }
-      if (sc[r] >= -1) {
-        const double lt = sc[r] == 0 ? lam : lam * kInvBig;
-        if (odd) {
-          orr[r] = fma(lt, a.x, orr[r]);
-          oi[r] = fma(lt, a.y, oi[r]);
+    for (int j = 0; j < kSynChunk; ++j) {
+      const bool in = l0 + j <= lmax;
+      a[j] = in ? sload(am + (l0 + j - m)) : make_double2(0.0, 0.0);
+      ab[j] = in ? sload(ABm + (l0 + j - m)) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int j = 0; j < kSynChunk; ++j) {
+      const int l = l0 + j;
+      if (l > lmax) break;
+      const bool odd = j & 1;                      // l0 - m is a multiple of kSynChunk
+#pragma unroll
+      for (int r = 0; r < kSynPairs; ++r) {
+        double lam;
+        if (l == m) {
+          lam = l1[r];
         } else {
-          er[r] = fma(lt, a.x, er[r]);
-          ei[r] = fma(lt, a.y, ei[r]);
+          lam = l == m + 1 ? ab[j].x * z[r] * l1[r]
+                           : ab[j].x * fma(z[r], l1[r], -ab[j].y * l2[r]);
+          l2[r] = l1[r];
+          l1[r] = lam;
+        }
+        const double lt = lam * wgt[r];
+        if (odd) {
+          orr[r] = fma(lt, a[j].x, orr[r]);
+          oi[r] = fma(lt, a[j].y, oi[r]);
+        } else {
+          er[r] = fma(lt, a[j].x, er[r]);
+          ei[r] = fma(lt, a[j].y, ei[r]);
         }
       }
     }
+#pragma unroll
+    for (int r = 0; r < kSynPairs; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
   }
 #pragma unroll
   for (int r = 0; r < kSynPairs; ++r) {
@@ -542,11 +566,12 @@ __device__ inline void pol_start(const PolArgs& A, int k, int m, double& x, doub
   }
 }
 
-// One step of the X/Y recurrences of a ring pair, with rescaling; returns
-// 2 lambda+ and 2 lambda- at their true scale, or zero below 2^-1200.
-__device__ __forceinline__ bool pol_step(bool start, double a, double b, double c, double z,
-                                         double& x1, double& x2, double& y1, double& y2, int& sc,
-                                         double& tp, double& tm) {
+// One step of the X/Y recurrences of a ring pair (the start value at l = ls);
+// returns 2 lambda+ and 2 lambda- weighted by the pair's scale (see
+// rescale_pair: rescaling runs once per chunk of steps).
+__device__ __forceinline__ void pol_adv(bool start, double a, double b, double c, double z,
+                                        double& x1, double& x2, double& y1, double& y2,
+                                        double wgt, double& tp, double& tm) {
   double x, y;
   if (start) {
     x = x1;
@@ -560,26 +585,26 @@ __device__ __forceinline__ bool pol_step(bool start, double a, double b, double 
     y2 = y1;
     y1 = y;
   }
-  if (sc < 0 && fmax(fabs(x), fabs(y)) > kBig) {
+  tp = (x + y) * wgt;
+  tm = (x - y) * wgt;
+}
+
+__device__ __forceinline__ void pol_rescale(double& x1, double& x2, double& y1, double& y2,
+                                            int& s, double& wgt) {
+  if (s < 0 && fmax(fabs(x1), fabs(y1)) > kBig) {
     x1 *= kInvBig;
     x2 *= kInvBig;
     y1 *= kInvBig;
     y2 *= kInvBig;
-    x *= kInvBig;
-    y *= kInvBig;
-    sc += 1;
+    s += 1;
+    wgt = scale_weight(s);
   }
-  if (sc < -1) return false;
-  const double f = sc == 0 ? 1.0 : kInvBig;
-  tp = (x + y) * f;
-  tm = (x - y) * f;
-  return true;
 }
 
 // Spin-2 analysis partial sums: part[comp][chunk][idx(l, m)], comp 0 = E, 1 = B,
 // of sum over the chunk's pairs of [2 lambda+ Q + 2i lambda- U] (E) and
 // [2 lambda+ U - 2i lambda- Q] (B); F holds Q+, Q-, U+, U- at stride fs.
-__global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
                                                             int64_t fs, double2* __restrict__ part) {
   __shared__ double red[kLChunk2][4][256];
   const int m = blockIdx.y;
@@ -590,7 +615,7 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
   const double* Cm = A.abc + 3 * alm_idx(lmax, m, m);   // Cm[3 (l - m) + 0..2]
   const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
   double z[kPairsPerThread], x1[kPairsPerThread], x2[kPairsPerThread];
-  double y1[kPairsPerThread], y2[kPairsPerThread];
+  double y1[kPairsPerThread], y2[kPairsPerThread], wgt[kPairsPerThread];
   int sc[kPairsPerThread];
   double2 qp[kPairsPerThread], qm[kPairsPerThread], up[kPairsPerThread], um[kPairsPerThread];
 #pragma unroll
@@ -600,6 +625,7 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
     const int pp = live ? p : 0;
     z[r] = A.R.z[pp];
     pol_start(A, pp, m, x1[r], y1[r], sc[r]);
+    wgt[r] = scale_weight(sc[r]);
     x2[r] = y2[r] = 0.0;
     const int64_t o = (int64_t)m * A.R.npairs + pp;   // in bounds for every lane
     const double keep = live ? 1.0 : 0.0;
@@ -629,19 +655,18 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
           b = sload(cf + 1);
           c = sload(cf + 2);
         }
-        const bool odd = (l - m) & 1;
+        const bool odd = j & 1;                    // l0 - m is a multiple of kLChunk2
 #pragma unroll
         for (int r = 0; r < kPairsPerThread; ++r) {
           double tp, tm;
-          if (pol_step(l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], sc[r], tp, tm)) {
-            // lambda+ pairs with the (-1)^{l+m} fold, lambda- with the other
-            const double2 qa = odd ? qm[r] : qp[r], qb = odd ? qp[r] : qm[r];
-            const double2 ua = odd ? um[r] : up[r], ub = odd ? up[r] : um[r];
-            er = fma(tp, qa.x, fma(-tm, ub.y, er));
-            ei = fma(tp, qa.y, fma(tm, ub.x, ei));
-            br = fma(tp, ua.x, fma(tm, qb.y, br));
-            bi = fma(tp, ua.y, fma(-tm, qb.x, bi));
-          }
+          pol_adv(l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
+          // lambda+ pairs with the (-1)^{l+m} fold, lambda- with the other
+          const double2 qa = odd ? qm[r] : qp[r], qb = odd ? qp[r] : qm[r];
+          const double2 ua = odd ? um[r] : up[r], ub = odd ? up[r] : um[r];
+          er = fma(tp, qa.x, fma(-tm, ub.y, er));
+          ei = fma(tp, qa.y, fma(tm, ub.x, ei));
+          br = fma(tp, ua.x, fma(tm, qb.y, br));
+          bi = fma(tp, ua.y, fma(-tm, qb.x, bi));
         }
       }
       red[j][0][tid] = er;
@@ -649,6 +674,9 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
       red[j][2][tid] = br;
       red[j][3][tid] = bi;
     }
+#pragma unroll
+    for (int r = 0; r < kPairsPerThread; ++r)
+      if (l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
     __syncthreads();
     {
       const int o = tid >> 4, g = tid & 15;
@@ -674,7 +702,9 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
 // Spin-2 synthesis: GQ[m][k], GU[m][k] = Q_m, U_m of every ring, kSynPairs
 // ring pairs per thread; lambda+ terms go to the part of the (-1)^{l+m}
 // parity, lambda- terms to the other (north = even + odd, south = even - odd).
-__global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
+// a_E, a_B and the coefficients of kLChunk2 steps are read ahead with scalar
+// loads.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) legendre_synthesis_s2(PolArgs A,
                                                              const double2* __restrict__ aE,
                                                              const double2* __restrict__ aB,
                                                              double2* __restrict__ GQ,
@@ -687,6 +717,7 @@ __global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
   const double2* bm = aB + i0;
   const double* Cm = A.abc + 3 * i0;
   double z[kSynPairs], x1[kSynPairs], x2[kSynPairs], y1[kSynPairs], y2[kSynPairs];
+  double wgt[kSynPairs];
   double2 qe[kSynPairs], qo[kSynPairs], ue[kSynPairs], uo[kSynPairs];
   int sc[kSynPairs], pr[kSynPairs];
 #pragma unroll
@@ -696,41 +727,54 @@ __global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
     const int pp = p < A.R.npairs ? p : 0;
     z[r] = A.R.z[pp];
     pol_start(A, pp, m, x1[r], y1[r], sc[r]);
+    wgt[r] = scale_weight(sc[r]);
     x2[r] = y2[r] = 0.0;
     qe[r] = qo[r] = ue[r] = uo[r] = make_double2(0.0, 0.0);
   }
-  for (int l = ls; l <= lmax; ++l) {
-    const double2 e = sload(em + (l - m));
-    const double2 b = sload(bm + (l - m));
-    double ca = 0.0, cb = 0.0, cc = 0.0;
-    if (l > ls) {
-      const double* cf = Cm + 3 * (l - m);
-      ca = sload(cf);
-      cb = sload(cf + 1);
-      cc = sload(cf + 2);
-    }
-    const bool odd = (l - m) & 1;
+  for (int l0 = m; l0 <= lmax; l0 += kLChunk2) {
+    double2 e[kLChunk2], b[kLChunk2];
+    double ca[kLChunk2], cb[kLChunk2], cc[kLChunk2];
 #pragma unroll
-    for (int r = 0; r < kSynPairs; ++r) {
-      double tp, tm;
-      if (!pol_step(l == ls, ca, cb, cc, z[r], x1[r], x2[r], y1[r], y2[r], sc[r], tp, tm))
-        continue;
-      // lambda+ a_E and i lambda- a_B (Q); lambda+ a_B and -i lambda- a_E (U)
-      const double2 pq = make_double2(tp * e.x, tp * e.y);
-      const double2 mq = make_double2(-tm * b.y, tm * b.x);
-      const double2 pu = make_double2(tp * b.x, tp * b.y);
-      const double2 mu = make_double2(tm * e.y, -tm * e.x);
-      const double2 q_e = odd ? mq : pq, q_o = odd ? pq : mq;
-      const double2 u_e = odd ? mu : pu, u_o = odd ? pu : mu;
-      qe[r].x += q_e.x;
-      qe[r].y += q_e.y;
-      qo[r].x += q_o.x;
-      qo[r].y += q_o.y;
-      ue[r].x += u_e.x;
-      ue[r].y += u_e.y;
-      uo[r].x += u_o.x;
-      uo[r].y += u_o.y;
+    for (int j = 0; j < kLChunk2; ++j) {
+      const int l = l0 + j;
+      const bool in = l >= ls && l <= lmax;
+      e[j] = in ? sload(em + (l - m)) : make_double2(0.0, 0.0);
+      b[j] = in ? sload(bm + (l - m)) : make_double2(0.0, 0.0);
+      const double* cf = Cm + 3 * (l - m);
+      ca[j] = in && l > ls ? sload(cf) : 0.0;
+      cb[j] = in && l > ls ? sload(cf + 1) : 0.0;
+      cc[j] = in && l > ls ? sload(cf + 2) : 0.0;
     }
+#pragma unroll
+    for (int j = 0; j < kLChunk2; ++j) {
+      const int l = l0 + j;
+      if (l > lmax) break;
+      if (l < ls) continue;
+      const bool odd = j & 1;                      // l0 - m is a multiple of kLChunk2
+#pragma unroll
+      for (int r = 0; r < kSynPairs; ++r) {
+        double tp, tm;
+        pol_adv(l == ls, ca[j], cb[j], cc[j], z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
+        // lambda+ a_E and i lambda- a_B (Q); lambda+ a_B and -i lambda- a_E (U)
+        const double2 pq = make_double2(tp * e[j].x, tp * e[j].y);
+        const double2 mq = make_double2(-tm * b[j].y, tm * b[j].x);
+        const double2 pu = make_double2(tp * b[j].x, tp * b[j].y);
+        const double2 mu = make_double2(tm * e[j].y, -tm * e[j].x);
+        const double2 q_e = odd ? mq : pq, q_o = odd ? pq : mq;
+        const double2 u_e = odd ? mu : pu, u_o = odd ? pu : mu;
+        qe[r].x += q_e.x;
+        qe[r].y += q_e.y;
+        qo[r].x += q_o.x;
+        qo[r].y += q_o.y;
+        ue[r].x += u_e.x;
+        ue[r].y += u_e.y;
+        uo[r].x += u_o.x;
+        uo[r].y += u_o.y;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kSynPairs; ++r)
+      if (l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
   }
 #pragma unroll
   for (int r = 0; r < kSynPairs; ++r) {
