@@ -347,6 +347,28 @@ def run_sht(args, pkg) -> dict:
                               "ten alm2cl cross-spectra", "nside": nside, "lmax": lmax})
 
 
+def run_sht_pol(args, pkg) -> dict:
+    """The polarised transform master uses: map2alm_pol (3 iterations) of one
+    (I, Q, U) nside-1024 map at lmax 3071, the three fields taken from config
+    masks (seeds 0-2); device time of the whole call, host map in and alms out."""
+    from powerspectra_amd.masks import config_masks
+    ctx = pkg.Context(0)
+    nside, lmax = 1024, args.lmax
+    iqu = np.stack(config_masks(nside, 200, (0, 1, 2)))
+    step = lambda: ctx.map2alm_pol(iqu, nside, lmax, 3)
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.close()
+    return _line("polarised map2alm: (I, Q, U) nside-1024 map, T/E/B to lmax 3071", 1.0 / dt,
+                 "maps/s", args, dt * 1e3,
+                 {"workload": f"sht_pol_nside1024_lmax{lmax}: map2alm_pol (3 iterations) of one "
+                              "(I, Q, U) map", "nside": nside, "lmax": lmax})
+
+
 QP_SPINS = (2, -2, 2, 2)   # nu1, nu2, s1, s2
 
 
@@ -576,7 +598,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
                     choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
-                             "quickpol"],
+                             "sht_pol", "quickpol"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -591,7 +613,7 @@ def main() -> int:
     pkg = load_package()
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
-              "e2e": run_e2e, "serve": run_serve, "sht": run_sht,
+              "e2e": run_e2e, "serve": run_serve, "sht": run_sht, "sht_pol": run_sht_pol,
               "quickpol": run_quickpol}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0

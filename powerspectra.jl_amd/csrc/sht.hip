@@ -120,60 +120,81 @@ struct RingArgs {
 };
 
 // Forward DFT of one polar ring (n = 4i <= 4 nside - 4 pixels):
-// X[q] = sum_j x_j e^{-2 pi i q j / n}, q = 0 .. n/2.  Cosine table in LDS;
-// sin(2 pi t / n) = cos(2 pi (t - n/4) / n).
+// X[q] = sum_j x_j e^{-2 pi i q j / n}, q = 0 .. n/2.  With the real input
+// folded as s_j = x_j + x_{n-j}, d_j = x_j - x_{n-j} (cos is even, sin odd):
+//   X[q] = x_0 + (-1)^q x_{n/2} + sum_{j=1}^{n/2-1} [s_j cos(2 pi q j/n) - i d_j sin(...)],
+// half the terms of the direct sum.  Cosine table, s and d in LDS (NMAX is
+// the longest ring the launch allows); sin(2 pi t / n) = cos(2 pi (t - n/4) / n).
+template <int NMAX>
 __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
                                                  double2* __restrict__ X, int ns) {
-  __shared__ double ctab[kMaxPolarRing];
+  __shared__ double ctab[NMAX];
+  __shared__ double sd[NMAX];                       // s_j at [j], d_j at [NMAX/2 + j]
   const int pr = blockIdx.x;                        // polar ring ordinal
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
-  const int n = R.n[k];
-  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
-  __syncthreads();
+  const int n = R.n[k], h = n / 2;
   const double* x = map + R.first[k];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  for (int j = threadIdx.x; j < h; j += blockDim.x) {
+    const double a = j > 0 ? x[j] : 0.0, b = j > 0 ? x[n - j] : 0.0;
+    sd[j] = a + b;
+    sd[NMAX / 2 + j] = a - b;
+  }
+  const double x0 = x[0], xh = x[h];
+  __syncthreads();
   const int q4 = n / 4;
-  for (int q = threadIdx.x; q <= n / 2; q += blockDim.x) {
-    double re = 0.0, im = 0.0;
+  for (int q = threadIdx.x; q <= h; q += blockDim.x) {
+    double re = (q & 1) ? x0 - xh : x0 + xh, im = 0.0;
     int t = 0;
-    for (int j = 0; j < n; ++j) {
-      const double v = x[j];
-      int ts = t - q4;
-      ts += ts < 0 ? n : 0;
-      re = fma(v, ctab[t], re);
-      im = fma(-v, ctab[ts], im);
+    for (int j = 1; j < h; ++j) {
       t += q;
       t -= t >= n ? n : 0;
+      int ts = t - q4;
+      ts += ts < 0 ? n : 0;
+      re = fma(sd[j], ctab[t], re);
+      im = fma(-sd[NMAX / 2 + j], ctab[ts], im);
     }
     X[R.xoff[k] + q] = make_double2(re, im);
   }
 }
 
 // Inverse DFT of one polar ring from its Hermitian half Y[0 .. n/2]:
-// f_j = Y_0 + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}) + Y_{n/2} (-1)^j.
+// f_j = Y_0 + (-1)^j Y_{n/2} + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}).
+// Outputs j and n - j share the cosine sum A and see the sine sum B with
+// opposite signs (f_j = base + A - B, f_{n-j} = base + A + B), so each step
+// serves two pixels.
+template <int NMAX>
 __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
                                                   double* __restrict__ map, int ns) {
-  __shared__ double ctab[kMaxPolarRing];
+  __shared__ double ctab[NMAX];
+  __shared__ double2 ys[NMAX / 2 + 1];              // 2 Y_q
   const int pr = blockIdx.x;
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
-  const int n = R.n[k];
+  const int n = R.n[k], h = n / 2;
   for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
-  __syncthreads();
   const double2* Y = X + R.xoff[k];
+  for (int q = threadIdx.x; q <= h; q += blockDim.x) {
+    const double2 y = Y[q];
+    ys[q] = make_double2(2.0 * y.x, 2.0 * y.y);
+  }
+  __syncthreads();
   const int q4 = n / 4;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double acc = Y[0].x + ((j & 1) ? -Y[n / 2].x : Y[n / 2].x);
+  double* f = map + R.first[k];
+  for (int j = threadIdx.x; j <= h; j += blockDim.x) {
+    const double base = 0.5 * (ys[0].x + ((j & 1) ? -ys[h].x : ys[h].x));
+    double A = 0.0, B = 0.0;
     int t = 0;
-    for (int q = 1; q < n / 2; ++q) {
+    for (int q = 1; q < h; ++q) {
       t += j;
       t -= t >= n ? n : 0;
       int ts = t - q4;
       ts += ts < 0 ? n : 0;
-      const double2 y = Y[q];
-      // Re(y e^{i a}) = y.x cos a - y.y sin a
-      acc = fma(2.0 * y.x, ctab[t], acc);
-      acc = fma(-2.0 * y.y, ctab[ts], acc);
+      const double2 y = ys[q];
+      A = fma(y.x, ctab[t], A);
+      B = fma(y.y, ctab[ts], B);
     }
-    map[R.first[k] + j] = acc;
+    f[j] = base + A - B;
+    if (j > 0 && j < h) f[n - j] = base + A + B;
   }
 }
 
@@ -604,7 +625,7 @@ __device__ __forceinline__ void pol_rescale(double& x1, double& x2, double& y1, 
 // Spin-2 analysis partial sums: part[comp][chunk][idx(l, m)], comp 0 = E, 1 = B,
 // of sum over the chunk's pairs of [2 lambda+ Q + 2i lambda- U] (E) and
 // [2 lambda+ U - 2i lambda- Q] (B); F holds Q+, Q-, U+, U- at stride fs.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
+__global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
                                                             int64_t fs, double2* __restrict__ part) {
   __shared__ double red[kLChunk2][4][256];
   const int m = blockIdx.y;
@@ -704,7 +725,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) l
 // parity, lambda- terms to the other (north = even + odd, south = even - odd).
 // a_E, a_B and the coefficients of kLChunk2 steps are read ahead with scalar
 // loads.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) legendre_synthesis_s2(PolArgs A,
+__global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
                                                              const double2* __restrict__ aE,
                                                              const double2* __restrict__ aB,
                                                              double2* __restrict__ GQ,
@@ -797,6 +818,26 @@ __global__ void __launch_bounds__(256) residual(const double* __restrict__ a,
                                                 double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] - b[i];
+}
+
+// The polar DFTs with LDS tables sized for the longest ring of the grid
+// (4 nside - 4 pixels): 64 KB for nside <= 1024, 128 KB up to 2048.
+void launch_polar_dft(int ns, hipStream_t s, RingArgs ra, const double* map, double2* X) {
+  if (4 * (ns - 1) <= kMaxPolarRing / 2)
+    hipLaunchKernelGGL(polar_dft<kMaxPolarRing / 2>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, map,
+                       X, ns);
+  else
+    hipLaunchKernelGGL(polar_dft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, map, X,
+                       ns);
+}
+
+void launch_polar_idft(int ns, hipStream_t s, RingArgs ra, const double2* X, double* map) {
+  if (4 * (ns - 1) <= kMaxPolarRing / 2)
+    hipLaunchKernelGGL(polar_idft<kMaxPolarRing / 2>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, X,
+                       map, ns);
+  else
+    hipLaunchKernelGGL(polar_idft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, X, map,
+                       ns);
 }
 
 // ---- plans ----------------------------------------------------------------
@@ -1020,7 +1061,7 @@ void analysis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap, double2*
   const int ns = R.nside;
   hipStream_t s = c->stream;
   if (ns > 1) {
-    hipLaunchKernelGGL(polar_dft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), dmap, B.X, ns);
+    launch_polar_dft(ns, s, P.ra(), dmap, B.X);
     PSX_HIP(hipGetLastError());
   }
   {
@@ -1059,7 +1100,7 @@ void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double
     PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
   }
   if (ns > 1) {
-    hipLaunchKernelGGL(polar_idft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), B.X, dmap, ns);
+    launch_polar_idft(ns, s, P.ra(), B.X, dmap);
     PSX_HIP(hipGetLastError());
   }
 }
@@ -1110,7 +1151,7 @@ void ring_fold(Ctx* c, ShtPlan& P, const double* dmap, double2* X, double2* Fp, 
   const int ns = R.nside;
   hipStream_t s = c->stream;
   if (ns > 1) {
-    hipLaunchKernelGGL(polar_dft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), dmap, X, ns);
+    launch_polar_dft(ns, s, P.ra(), dmap, X);
     PSX_HIP(hipGetLastError());
   }
   {
@@ -1137,7 +1178,7 @@ void ring_unfold(Ctx* c, ShtPlan& P, const double2* G, double2* X, double* dmap)
     PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
   }
   if (ns > 1) {
-    hipLaunchKernelGGL(polar_idft, dim3(2 * (ns - 1)), dim3(256), 0, s, P.ra(), X, dmap, ns);
+    launch_polar_idft(ns, s, P.ra(), X, dmap);
     PSX_HIP(hipGetLastError());
   }
 }
