@@ -343,9 +343,9 @@ def run_sht(args, pkg) -> dict:
     ctx.close()
     return _line("mask spectra: map2alm of four nside-1024 masks + cross-spectra", 1.0 / dt,
                  "sets/s", args, dt * 1e3,
-                 data="seeded masks: four nside-1024 cut-sky masks (seeds 0-3, 200 holes)",
                  {"workload": f"sht_nside1024_lmax{lmax}: map2alm (3 iterations) of four masks, "
-                              "ten alm2cl cross-spectra", "nside": nside, "lmax": lmax})
+                              "ten alm2cl cross-spectra", "nside": nside, "lmax": lmax},
+                 data="seeded masks: four nside-1024 cut-sky masks (seeds 0-3, 200 holes)")
 
 
 def run_sht_pol(args, pkg) -> dict:
@@ -366,9 +366,9 @@ def run_sht_pol(args, pkg) -> dict:
     ctx.close()
     return _line("polarised map2alm: (I, Q, U) nside-1024 map, T/E/B to lmax 3071", 1.0 / dt,
                  "maps/s", args, dt * 1e3,
-                 data="seeded masks as fields: I, Q, U = nside-1024 cut-sky masks, seeds 0-2",
                  {"workload": f"sht_pol_nside1024_lmax{lmax}: map2alm_pol (3 iterations) of one "
-                              "(I, Q, U) map", "nside": nside, "lmax": lmax})
+                              "(I, Q, U) map", "nside": nside, "lmax": lmax},
+                 data="seeded masks as fields: I, Q, U = nside-1024 cut-sky masks, seeds 0-2")
 
 
 QP_SPINS = (2, -2, 2, 2)   # nu1, nu2, s1, s2
