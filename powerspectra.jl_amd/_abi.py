@@ -71,7 +71,7 @@ class BudgetError(PowerSpectraError, MemoryError):
 
 
 class DeviceError(PowerSpectraError):
-    """HIP / rocSOLVER failure."""
+    """HIP, rocBLAS or rocSOLVER failure."""
 
 
 _ERRORS = {

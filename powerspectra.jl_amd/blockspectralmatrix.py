@@ -1,8 +1,9 @@
 """BlockSpectralMatrix: dense block matrices of spectra and the FP64 LU solve.
 
 Mirrors src/blockspectralmatrix.jl: block assembly (hvcat/vcat/hcat, :17-45),
-`\\` with LU and partial pivoting (:89-129, here rocSOLVER getrf/getrs through
-psx_lu_solve), getblock and the @spectra unpack (:144-184).
+`\\` with LU and partial pivoting (:89-129, here the library's blocked getrf
+and rocSOLVER getrs through psx_lu_solve), getblock and the @spectra unpack
+(:144-184).
 """
 from __future__ import annotations
 
@@ -77,7 +78,7 @@ def spectra(A: BlockSpectralMatrix) -> tuple:
 
 
 def solve(A, B, ctx: Context | None = None):
-    """`A \\ B`: FP64 LU with partial pivoting on the GPU (rocSOLVER).
+    """`A \\ B`: FP64 LU with partial pivoting on the GPU (psx_lu_solve).
 
     SpectralArray \\ SpectralVector asserts firstindex(A,2) == firstindex(B,1)
     and returns a SpectralVector with B's offsets (src/blockspectralmatrix.jl:124-129);
