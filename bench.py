@@ -321,18 +321,15 @@ def run_solve(args, pkg) -> dict:
 
 def run_sht(args, pkg) -> dict:
     """The mask -> W front end of the headline: map2alm (3 iterations) of the
-    four nside-1024 config masks at lmax 3071 and their ten cross-spectra
-    (host maps in, spectra out: PCIe included)."""
-    from powerspectra_amd.alm import Alm, alm2cl
+    four nside-1024 config masks at lmax 3071 and their ten cross-spectra in
+    one psx_mask_spectra call (host maps in, spectra out: PCIe included)."""
     from powerspectra_amd.masks import config_masks
     ctx = pkg.Context(0)
     nside, lmax = 1024, args.lmax
     maps = np.stack(config_masks(nside, 200, (0, 1, 2, 3)))
 
     def step():
-        A = ctx.map2alm(maps, nside, lmax, 3)
-        al = [Alm(lmax, lmax, a) for a in A]
-        return [alm2cl(al[i], al[j]) for i in range(4) for j in range(i, 4)]
+        return ctx.mask_spectra(maps, nside, lmax, 3)
 
     for _ in range(args.warmup):
         step()

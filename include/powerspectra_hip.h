@@ -184,6 +184,16 @@ int psx_map2alm(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
 /* Synthesis (Healpix.jl alm2map): the inverse of one analysis step. */
 int psx_alm2map(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* alms,
                 double* maps);
+/* The mask -> W front end in one call: map2alm (niter refinements) of nmaps
+ * ring-ordered maps (1..8, consecutive) and all their cross-spectra
+ * cls[(i nmaps + j)(lmax+1) + l] = alm2cl(a_i, a_j)[l]
+ *   = (2l+1)^-1 [Re(a_l0 b_l0*) + 2 sum_{m>=1} Re(a_lm b_lm*)],
+ * i.e. `alm2cl(map2alm(m_i), map2alm(m_j))` of mcm(spec, mask1, mask2) and the
+ * covariance workspace (reference src/modecoupling.jl:197,251,255,
+ * src/workspace.jl:202), with the alms kept on the device.  Limits as
+ * psx_map2alm. */
+int psx_mask_spectra(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
+                     const double* maps, double* cls);
 /* Polarised analysis (Healpix.jl map2alm of a PolarizedHealpixMap, as master
  * calls it, reference src/modecoupling.jl:328-329): per map the I, Q, U rings
  * (3 x 12 nside^2 doubles, consecutive) into T, E, B a_lm (3 x nalm complex,

@@ -95,6 +95,7 @@ SIGNATURES = {
     "psx_lu_factor": (_int, [_vp, _int, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_int)]),
     "psx_quickpol_xi": (_int, [_vp] + [_int] * 7 + [_dp, _i64, _dp]),
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
+    "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map_pol": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
@@ -321,6 +322,18 @@ class Context:
         out = np.empty((A.shape[0], 12 * nside * nside))
         self.check(self.lib.psx_alm2map(self._h, nside, lmax, A.shape[0],
                                         A.ctypes.data_as(_dp), dptr(out)), "psx_alm2map")
+        return out
+
+    def mask_spectra(self, maps, nside: int, lmax: int, niter: int = 3) -> np.ndarray:
+        """cl[i, j, l] = alm2cl(map2alm(m_i), map2alm(m_j))[l] of 1..8 ring-ordered
+        maps, alms kept on the device."""
+        M = np.ascontiguousarray(np.atleast_2d(maps), dtype=np.float64)
+        if M.shape[1] != 12 * nside * nside:
+            raise ArgumentError("maps must have 12 nside^2 pixels")
+        n = M.shape[0]
+        out = np.empty((n, n, lmax + 1))
+        self.check(self.lib.psx_mask_spectra(self._h, nside, lmax, niter, n, dptr(M),
+                                             dptr(out)), "psx_mask_spectra")
         return out
 
     def map2alm_pol(self, iqu, nside: int, lmax: int, niter: int = 3) -> np.ndarray:

@@ -813,6 +813,66 @@ __global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
   }
 }
 
+// ---- cross-spectra ----------------------------------------------------------
+// C^{ij}_l = (2l+1)^-1 [Re(a^i_l0 a^j_l0*) + 2 sum_{m>=1} Re(a^i_lm a^j_lm*)]
+// for every pair i <= j of NM maps (Healpix alm2cl as the reference applies it
+// to mask alms, src/modecoupling.jl:197, src/workspace.jl:202).  Thread l of
+// chunk c sums m in [c mchunk, (c+1) mchunk) (the a_lm of consecutive l are
+// adjacent, so the loads coalesce); chunk partials part[c][pair][l] are then
+// added in chunk order by cross_spectra_sum, so the result does not depend on
+// scheduling.
+template <int NM>
+__global__ void __launch_bounds__(256) cross_spectra_partial(const double2* __restrict__ alms,
+                                                             int64_t nalm, int lmax, int mchunk,
+                                                             double* __restrict__ part) {
+  constexpr int NP = NM * (NM + 1) / 2;
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const int c = blockIdx.y;
+  if (l > lmax) return;
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  const int m0 = c * mchunk, m1 = min(m0 + mchunk, l + 1);
+  for (int m = m0; m < m1; ++m) {
+    const int64_t idx = alm_idx(lmax, l, m);
+    const double w = m == 0 ? 1.0 : 2.0;
+    double2 a[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) a[i] = alms[i * nalm + idx];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+#pragma unroll
+      for (int j = i; j < NM; ++j, ++q)
+        acc[q] = fma(w, fma(a[i].x, a[j].x, a[i].y * a[j].y), acc[q]);
+  }
+  const int64_t L = lmax + 1;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) part[((int64_t)c * NP + q) * L + l] = acc[q];
+}
+
+// cl[i][j][l] = cl[j][i][l] = sum_c part[c][pair(i,j)][l] / (2l+1)
+__global__ void __launch_bounds__(256) cross_spectra_sum(const double* __restrict__ part,
+                                                         int nchunk, int nm, int lmax,
+                                                         double* __restrict__ cl) {
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const int q = blockIdx.y;                      // pair index, i <= j in row order
+  if (l > lmax) return;
+  const int np = nm * (nm + 1) / 2;
+  const int64_t L = lmax + 1;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * np + q) * L + l];
+  s /= 2.0 * l + 1.0;
+  int i = 0, r = q;
+  while (r >= nm - i) {
+    r -= nm - i;
+    ++i;
+  }
+  const int j = i + r;
+  cl[((int64_t)i * nm + j) * L + l] = s;
+  cl[((int64_t)j * nm + i) * L + l] = s;
+}
+
 __global__ void __launch_bounds__(256) residual(const double* __restrict__ a,
                                                 const double* __restrict__ b, int64_t n,
                                                 double* __restrict__ out) {
@@ -1338,6 +1398,44 @@ void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb,
   ctx_free(c, mem);
 }
 
+constexpr int kMaxSpectraMaps = 8;
+
+// All cross-spectra of nmaps device alm sets (nalm complex each) into
+// cl[nmaps][nmaps][lmax+1], on the context stream.
+void cross_spectra_dev(Ctx* c, int lmax, int nmaps, const double* dalms, double* dcl) {
+  PSX_REQUIRE(nmaps >= 1 && nmaps <= kMaxSpectraMaps, PSX_ERR_ARGUMENT,
+              "cross-spectra of 1..8 maps per call");
+  const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+  const int np = nmaps * (nmaps + 1) / 2;
+  const int nchunk = std::min(64, lmax + 1);
+  const int mchunk = (lmax + 1 + nchunk - 1) / nchunk;
+  double* part = (double*)ctx_alloc(c, sizeof(double) * (size_t)nchunk * np * (lmax + 1));
+  try {
+    const dim3 grid((unsigned)((lmax + 1 + 255) / 256), (unsigned)nchunk);
+    const double2* a = (const double2*)dalms;
+    hipStream_t s = c->stream;
+    switch (nmaps) {
+#define PSX_CS(NMV)                                                                       \
+  case NMV:                                                                               \
+    hipLaunchKernelGGL(cross_spectra_partial<NMV>, grid, dim3(256), 0, s, a, nalm, lmax, \
+                       mchunk, part);                                                     \
+    break;
+      PSX_CS(1) PSX_CS(2) PSX_CS(3) PSX_CS(4) PSX_CS(5) PSX_CS(6) PSX_CS(7) PSX_CS(8)
+#undef PSX_CS
+    }
+    PSX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(cross_spectra_sum, dim3((unsigned)((lmax + 1 + 255) / 256), (unsigned)np),
+                       dim3(256), 0, s, (const double*)part, nchunk, nmaps, lmax, dcl);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c, part);
+    throw;
+  }
+  ctx_free(c, part);
+}
+
 }  // namespace psx
 
 using namespace psx;
@@ -1392,6 +1490,40 @@ int psx_alm2map(psx_ctx* h, int nside, int lmax, int nmaps, const double* alms, 
     }
     ctx_free(c, dm);
     ctx_free(c, da);
+  });
+}
+
+int psx_mask_spectra(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const double* maps,
+                     double* cls) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(maps && cls && niter >= 0, PSX_ERR_ARGUMENT, "bad arguments");
+    PSX_REQUIRE(nmaps >= 1 && nmaps <= kMaxSpectraMaps, PSX_ERR_ARGUMENT,
+                "mask spectra of 1..8 maps per call");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    const int64_t ncl = (int64_t)nmaps * nmaps * (lmax + 1);
+    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
+    double* da = nullptr;
+    double* dc = nullptr;
+    try {
+      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
+      dc = (double*)ctx_alloc(c, sizeof(double) * ncl);
+      h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
+      map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
+      cross_spectra_dev(c, lmax, nmaps, da, dc);
+      d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
+    } catch (...) {
+      ctx_free(c, dm);
+      if (da) ctx_free(c, da);
+      if (dc) ctx_free(c, dc);
+      throw;
+    }
+    ctx_free(c, dm);
+    ctx_free(c, da);
+    ctx_free(c, dc);
   });
 }
 

@@ -57,12 +57,17 @@ def config_masks(nside: int, nholes: int, seeds) -> list:
 
 def mask_spectra(masks, lmax: int, ctx=None, niter: int = 3) -> dict:
     """Cross-spectra W[i, j] = alm2cl(map2alm(m_i), map2alm(m_j))[0:lmax] of
-    a list of masks (GPU transform, nside taken from the masks)."""
+    a list of masks (GPU transform and cross-spectra in one call per 8 masks,
+    the alms stay on the device; nside taken from the masks)."""
     from ._abi import default_context
-    from .alm import Alm, alm2cl
     ctx = ctx or default_context()
     M = np.stack([np.asarray(m, dtype=np.float64) for m in masks])
     nside = int(round(np.sqrt(M.shape[1] / 12)))
+    n = M.shape[0]
+    if n <= 8:
+        C = ctx.mask_spectra(M, nside, lmax, niter)
+        return {(i, j): C[i, j] for i in range(n) for j in range(n)}
+    from .alm import Alm, alm2cl
     A = ctx.map2alm(M, nside, lmax, niter)
     alms = [Alm(lmax, lmax, a) for a in A]
-    return {(i, j): alm2cl(alms[i], alms[j]) for i in range(len(alms)) for j in range(len(alms))}
+    return {(i, j): alm2cl(alms[i], alms[j]) for i in range(n) for j in range(n)}

@@ -220,3 +220,34 @@ def test_pol_roundtrip_nside1024(pkg, ctx):
     maps = ctx.alm2map_pol(np.stack([Z.alm, E.alm, Z.alm]), nside, lmax)[0]
     teb = ctx.map2alm_pol(maps, nside, lmax, 10)[0]
     assert np.abs(teb[2]).max() <= 1e-10 * np.abs(E.alm).max()
+
+
+# ---- mask spectra on the device ----------------------------------------------
+
+@pytest.mark.parametrize("nside,lmax,nmaps", [(16, 47, 1), (32, 95, 3), (8, 20, 8)])
+def test_mask_spectra_vs_host_alm2cl(pkg, ctx, nside, lmax, nmaps):
+    """psx_mask_spectra (map2alm + all cross-spectra on the device) against the
+    host alm2cl of the same GPU alms."""
+    maps = np.stack([_smooth_map(nside, 40 + s) for s in range(nmaps)])
+    C = ctx.mask_spectra(maps, nside, lmax, 2)
+    A = ctx.map2alm(maps, nside, lmax, 2)
+    alms = [pkg.Alm(lmax, lmax, a) for a in A]
+    for i in range(nmaps):
+        for j in range(nmaps):
+            want = pkg.alm2cl(alms[i], alms[j])
+            assert np.abs(C[i, j] - want).max() <= 1e-13 * np.abs(want).max(), (i, j)
+
+
+def test_mask_spectra_config_masks_and_limits(pkg, ctx):
+    """Two nside-256 config masks at lmax 767 through masks.mask_spectra, and
+    more than 8 maps per device call refused."""
+    from powerspectra_amd.masks import config_masks, mask_spectra
+    m = config_masks(256, 50, (0, 1))
+    W = mask_spectra(m, 767, ctx=ctx)
+    A = ctx.map2alm(np.stack(m), 256, 767, 3)
+    a0, a1 = pkg.Alm(767, 767, A[0]), pkg.Alm(767, 767, A[1])
+    want = pkg.alm2cl(a0, a1)
+    assert np.abs(W[(0, 1)] - want).max() <= 1e-13 * np.abs(want).max()
+    assert np.array_equal(W[(0, 1)], W[(1, 0)])
+    with pytest.raises(pkg.ArgumentError):
+        ctx.mask_spectra(np.zeros((9, 12 * 4 * 4)), 4, 8, 0)
