@@ -83,9 +83,19 @@ def mcm(spec, a1, a2=None, *, lmin: int = 0, lmax: int | None = None,
         m2 = a2.maskT if name[1] == "T" else a2.maskP
         return mcm(spec, m1, m2, lmin=lmin, lmax=lmax, ctx=ctx)
     if not isinstance(a1, Alm):
-        from .sht import HealpixMap, map2alm
-        a1 = map2alm(a1 if isinstance(a1, HealpixMap) else HealpixMap(a1))
-        a2 = map2alm(a2 if isinstance(a2, HealpixMap) else HealpixMap(a2))
+        # masks: map2alm (Healpix.jl defaults: lmax = 3 nside - 1, 3 iterations)
+        # and their cross-spectrum in one device call, the alms never leave it
+        from ._abi import default_context
+        from .sht import HealpixMap, nside2lmax
+        m1 = a1 if isinstance(a1, HealpixMap) else HealpixMap(a1)
+        m2 = a2 if isinstance(a2, HealpixMap) else HealpixMap(a2)
+        if m1.nside != m2.nside:
+            raise ArgumentError("the two masks need the same nside")
+        ctx = ctx or default_context()
+        la = nside2lmax(m1.nside)
+        C = ctx.mask_spectra(np.stack([m1.pixels, m2.pixels]), m1.nside, la, 3)
+        lmax = la if lmax is None else lmax
+        return mcm_from_spectrum(spec, C[0, 1][: lmax + 1], lmin=lmin, lmax=lmax, ctx=ctx)
     if lmax is None:
         lmax = min(a1.lmax, a2.lmax)
     V = alm2cl(a1, a2)[: lmax + 1]
