@@ -68,7 +68,7 @@ def test_singular(ctx, pkg):
         ctx.decouple_covmat(np.eye(4), np.zeros((4, 4)), np.eye(4))
 
 
-@pytest.mark.parametrize("n", [1, 5, 33, 300, 513, 1000, 2100])
+@pytest.mark.parametrize("n", [1, 5, 32, 33, 256, 257, 289, 300, 513, 1000, 2100])
 def test_lu_factor_vs_lapack(ctx, n):
     """The blocked getrf (leaf kernels + rocBLAS updates) against LAPACK's
     getrf (scipy): the same row interchanges and factors to rounding."""
@@ -99,3 +99,18 @@ def test_lu_factor_mcm_and_solve(ctx):
 def test_lu_factor_singular(ctx, pkg):
     with pytest.raises(pkg.SingularException):
         ctx.lu_factor(np.ones((40, 40)))
+
+
+def test_lu_factor_structured_pivots(ctx):
+    """Pivoting that moves rows across leaves and blocks: a matrix whose
+    largest entry of every column sits in the bottom rows (LAPACK swaps it up
+    at every step), against scipy."""
+    import scipy.linalg as sl
+    n = 600
+    rng = np.random.default_rng(1)
+    A = rng.uniform(0.0, 1.0, (n, n))
+    A[n - 64:, :] += 10.0 * np.arange(1, 65)[:, None]
+    LU, ipiv = ctx.lu_factor(A)
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert np.array_equal(ipiv - 1, piv_ref)
+    assert np.linalg.norm(LU - lu_ref) / np.linalg.norm(lu_ref) < 1e-12

@@ -70,3 +70,17 @@ def test_quickpolXi_mirror(pkg, ctx, oracle):
     quickpolXi(Xi, 2, 0, 2, -2, w1, w2, ctx=ctx)
     want = oracle.quickpol_xi(2, 0, 2, -2, 80, 4, 4, quickpolW(w1, w2).parent)
     assert _cmp(Xi.data, want) <= 1e-12
+
+
+@pytest.mark.parametrize("lmax,bl,bu", [(0, 0, 0), (1, 1, 0), (2, 0, 0), (3, 1, 2), (5, 5, 5)])
+def test_quickpol_xi_small(ctx, oracle, lmax, bl, bu):
+    """Tiny matrices: single diagonals, bands wider than the matrix, lmax < 2
+    (only the sign is applied)."""
+    rng = np.random.default_rng(lmax + 10 * bl)
+    W = rng.uniform(-1.0, 1.0, 2 * lmax + 1)
+    start = np.asfortranarray(rng.uniform(1.0, 2.0, (bl + bu + 1, lmax + 1)))
+    for spins in ((0, 0, 0, 0), (1, 0, 2, 0)):
+        got = ctx.quickpol_xi(*spins, lmax, bl, bu, W, band=start.copy(order="F"))
+        want = oracle.quickpol_xi(*spins, lmax, bl, bu, W, band=start.copy(order="F"))
+        # cells of the storage outside the matrix are untouched by both
+        assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
