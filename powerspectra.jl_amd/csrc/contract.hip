@@ -166,9 +166,10 @@ __host__ __device__ constexpr int rec_stride(int nf) {
 }
 
 // EPI: EPI_MCM or EPI_COV; COVK: covariance kind for EPI_COV; WPE: minimum
-// waves per SIMD the register allocation must allow.
-template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+// waves per SIMD the register allocation must allow; TPB: threads per
+// workgroup (64 per tile).
+template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE, int TPB>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE)))
 contract_kernel(KArgs A) {
   constexpr int NV = N00 + N02 + NPE + NPO;
   constexpr int NE = N00 + N02 + NPE;
@@ -574,9 +575,11 @@ struct Config {
   const char* name;
 };
 
-template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE = 1>
-Config cfg(int waves, const char* name) {
-  return Config{R, waves, contract_kernel<R, N00, N02, NPE, NPO, EPI, COVK, WPE>, name};
+template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE = 1,
+          int WAVES = 4>
+Config cfg(const char* name) {
+  return Config{R, WAVES, contract_kernel<R, N00, N02, NPE, NPO, EPI, COVK, WPE, 64 * WAVES>,
+                name};
 }
 
 // Anti-diagonals per tile (R) and minimum waves per SIMD for each kernel,
@@ -617,35 +620,38 @@ Config cfg(int waves, const char* name) {
 #ifndef PSX_R_COV
 #define PSX_R_COV 4
 #endif
+#ifndef PSX_WAVES_SET
+#define PSX_WAVES_SET 4
+#endif
 
 Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
   if (rq.epi == EPI_MCM) {
     if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
-      return cfg<PSX_R_00, 1, 0, 0, 0, EPI_MCM, 0, PSX_WPE_00>(4, "psx_mcm_00");
+      return cfg<PSX_R_00, 1, 0, 0, 0, EPI_MCM, 0, PSX_WPE_00>("psx_mcm_00");
     if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
-      return cfg<PSX_R_02, 0, 1, 0, 0, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_02");
+      return cfg<PSX_R_02, 0, 1, 0, 0, EPI_MCM, 0, PSX_WPE_02>("psx_mcm_02");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
-      return cfg<PSX_R_PP, 0, 0, 1, 0, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_pp");
+      return cfg<PSX_R_PP, 0, 0, 1, 0, EPI_MCM, 0, PSX_WPE_02>("psx_mcm_pp");
     if (f[0] == 0 && f[1] == 0 && f[2] == 0 && f[3] == 1)
-      return cfg<PSX_R_MM, 0, 0, 0, 1, EPI_MCM, 0, PSX_WPE_02>(4, "psx_mcm_mm");
+      return cfg<PSX_R_MM, 0, 0, 0, 1, EPI_MCM, 0, PSX_WPE_02>("psx_mcm_mm");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 1)
-      return cfg<PSX_R_PM, 0, 0, 1, 1, EPI_MCM, 0, PSX_WPE_PM>(4, "psx_mcm_pm");
+      return cfg<PSX_R_PM, 0, 0, 1, 1, EPI_MCM, 0, PSX_WPE_PM>("psx_mcm_pm");
     if (f[0] == 1 && f[1] == 2 && f[2] == 1 && f[3] == 1)
-      return cfg<PSX_R_SET, 1, 2, 1, 1, EPI_MCM, 0, PSX_WPE_SET>(4, "psx_mcm_set");
+      return cfg<PSX_R_SET, 1, 2, 1, 1, EPI_MCM, 0, PSX_WPE_SET, PSX_WAVES_SET>("psx_mcm_set");
     if (f[0] == 1 && f[1] == 1 && f[2] == 1 && f[3] == 1)
-      return cfg<PSX_R_SET, 1, 1, 1, 1, EPI_MCM, 0, PSX_WPE_SET>(4, "psx_mcm_set1");
+      return cfg<PSX_R_SET, 1, 1, 1, 1, EPI_MCM, 0, PSX_WPE_SET>("psx_mcm_set1");
   } else {
     constexpr int RC = PSX_R_COV;
     switch (rq.cov_kind) {
-      case PSX_COV_TTTT: return cfg<RC, 8, 0, 0, 0, EPI_COV, PSX_COV_TTTT, PSX_WPE_COV>(4, "psx_cov_tttt");
-      case PSX_COV_EEEE: return cfg<RC, 0, 0, 8, 0, EPI_COV, PSX_COV_EEEE, PSX_WPE_COV>(4, "psx_cov_eeee");
-      case PSX_COV_TTTE: return cfg<RC, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE, PSX_WPE_COV>(4, "psx_cov_ttte");
-      case PSX_COV_TETE: return cfg<RC, 1, 4, 0, 0, EPI_COV, PSX_COV_TETE, PSX_WPE_COV>(4, "psx_cov_tete");
-      case PSX_COV_TEEE: return cfg<RC, 0, 4, 0, 0, EPI_COV, PSX_COV_TEEE, PSX_WPE_COV>(4, "psx_cov_teee");
+      case PSX_COV_TTTT: return cfg<RC, 8, 0, 0, 0, EPI_COV, PSX_COV_TTTT, PSX_WPE_COV>("psx_cov_tttt");
+      case PSX_COV_EEEE: return cfg<RC, 0, 0, 8, 0, EPI_COV, PSX_COV_EEEE, PSX_WPE_COV>("psx_cov_eeee");
+      case PSX_COV_TTTE: return cfg<RC, 4, 0, 0, 0, EPI_COV, PSX_COV_TTTE, PSX_WPE_COV>("psx_cov_ttte");
+      case PSX_COV_TETE: return cfg<RC, 1, 4, 0, 0, EPI_COV, PSX_COV_TETE, PSX_WPE_COV>("psx_cov_tete");
+      case PSX_COV_TEEE: return cfg<RC, 0, 4, 0, 0, EPI_COV, PSX_COV_TEEE, PSX_WPE_COV>("psx_cov_teee");
       case PSX_COV_TEEE_PLANCK:
-        return cfg<RC, 0, 0, 4, 0, EPI_COV, PSX_COV_TEEE_PLANCK, PSX_WPE_COV>(4, "psx_cov_teee_planck");
-      case PSX_COV_TTEE: return cfg<RC, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE, PSX_WPE_COV>(4, "psx_cov_ttee");
+        return cfg<RC, 0, 0, 4, 0, EPI_COV, PSX_COV_TEEE_PLANCK, PSX_WPE_COV>("psx_cov_teee_planck");
+      case PSX_COV_TTEE: return cfg<RC, 2, 0, 0, 0, EPI_COV, PSX_COV_TTEE, PSX_WPE_COV>("psx_cov_ttee");
     }
   }
   throw Error{PSX_ERR_ARGUMENT, "unsupported contraction configuration"};
