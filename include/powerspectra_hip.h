@@ -160,6 +160,10 @@ int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
  * (n x nrhs, overwritten with the solution).  A is not modified. */
 int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
                  double* B, int64_t ldb, int nrhs);
+/* The same on device buffers: dA is overwritten by its LU factors and dB by
+ * the solution; enqueued on the context stream, returns when done. */
+int psx_lu_solve_d(psx_ctx* ctx, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
+                   int nrhs);
 /* `lu(A)` (reference src/blockspectralmatrix.jl:89-99, Julia's LAPACK getrf
  * with partial pivoting): LU (n x n, leading dimension ldlu) receives the
  * unit-lower L below the diagonal and U on and above it, ipiv[n] the 1-based

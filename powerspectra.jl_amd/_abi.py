@@ -93,6 +93,7 @@ SIGNATURES = {
     "psx_map2alm": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_lu_factor": (_int, [_vp, _int, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_int)]),
+    "psx_lu_solve_d": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _int]),
     "psx_quickpol_xi": (_int, [_vp] + [_int] * 7 + [_dp, _i64, _dp]),
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
@@ -227,7 +228,11 @@ class Context:
         self.check(self.lib.psx_dev_free(self._h, _vp(p)), "dev_free")
 
     def h2d(self, dst: int, src: np.ndarray):
-        src = np.ascontiguousarray(src)
+        """Copy the bytes of src as laid out (C- or Fortran-contiguous arrays are
+        sent as they are; anything else is made C-contiguous first)."""
+        src = np.asarray(src)
+        if not (src.flags.c_contiguous or src.flags.f_contiguous):
+            src = np.ascontiguousarray(src)
         self.check(self.lib.psx_memcpy_h2d(self._h, _vp(dst), src.ctypes.data_as(_vp), src.nbytes),
                    "memcpy_h2d")
 

@@ -431,6 +431,18 @@ int psx_lu_solve(psx_ctx* hctx, int n, const double* A, int64_t lda, double* B,
   });
 }
 
+int psx_lu_solve_d(psx_ctx* hctx, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
+                   int nrhs) {
+  Ctx* c = reinterpret_cast<Ctx*>(hctx);
+  return guarded(c, [&] {
+    PSX_REQUIRE(n >= 1 && nrhs >= 0 && lda >= n && ldb >= n, PSX_ERR_ARGUMENT,
+                "bad sizes for lu_solve");
+    PSX_REQUIRE(dA && (dB || nrhs == 0), PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    lu_solve_dev(c, n, dA, lda, dB, ldb, nrhs);
+  });
+}
+
 int psx_lu_factor(psx_ctx* hctx, int n, const double* A, int64_t lda, double* LU, int64_t ldlu,
                   int* ipiv) {
   Ctx* c = reinterpret_cast<Ctx*>(hctx);

@@ -114,3 +114,23 @@ def test_lu_factor_structured_pivots(ctx):
     lu_ref, piv_ref = sl.lu_factor(A)
     assert np.array_equal(ipiv - 1, piv_ref)
     assert np.linalg.norm(LU - lu_ref) / np.linalg.norm(lu_ref) < 1e-12
+
+
+def test_lu_solve_device_buffers(ctx):
+    """psx_lu_solve_d on device buffers equals psx_lu_solve on host buffers."""
+    rng = np.random.default_rng(4)
+    n = 700
+    A = np.asfortranarray(np.eye(n) * 2 + rng.standard_normal((n, n)))
+    b = rng.standard_normal(n)
+    want = ctx.lu_solve(A, b)
+    dA, dB = ctx.alloc(8 * n * n), ctx.alloc(8 * n)
+    try:
+        ctx.h2d(dA, A.ravel(order="F"))
+        ctx.h2d(dB, b)
+        ctx.check(ctx.lib.psx_lu_solve_d(ctx.handle, n, dA, n, dB, n, 1), "lu_solve_d")
+        got = np.empty(n)
+        ctx.d2h(got, dB)
+    finally:
+        ctx.free(dA)
+        ctx.free(dB)
+    assert np.array_equal(got, want)
