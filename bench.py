@@ -585,7 +585,8 @@ def run_serve(args, pkg) -> dict:
                 "p99_over_alone_p99": pct(lat, 99) / pct(alone, 99)}
 
     runs = [poisson(0.8 * sat, abi.SCHED_FIFO), poisson(0.8 * sat, abi.SCHED_DEADLINE, 2.0),
-            poisson(0.8 * sat, abi.SCHED_DEADLINE, 4.0), poisson(0.5 * sat, abi.SCHED_DEADLINE, 2.0)]
+            poisson(0.8 * sat, abi.SCHED_DEADLINE, 4.0), poisson(0.5 * sat, abi.SCHED_DEADLINE, 2.0),
+            poisson(0.4 * sat, abi.SCHED_DEADLINE, 2.0), poisson(0.3 * sat, abi.SCHED_DEADLINE, 2.0)]
     srv.close()
     ctx.close()
     head = runs[1]
