@@ -42,7 +42,10 @@ namespace {
 
 constexpr int kMaxPolarRing = 8192;         // pixels of the longest polar ring (nside <= 2048)
 constexpr int kLChunk = 8;                  // l values per LDS reduction round
-constexpr int kPairsPerThread = 4;          // ring pairs per analysis thread
+#ifndef PSX_SHT_PAIRS
+#define PSX_SHT_PAIRS 4
+#endif
+constexpr int kPairsPerThread = PSX_SHT_PAIRS;   // ring pairs per analysis thread
 constexpr int kSynPairs = 4;                // ring pairs per synthesis thread
 constexpr int kSynChunk = 4;                // l values read ahead per synthesis round
 constexpr double kBig = 0x1p600;            // rescaling threshold
@@ -123,8 +126,20 @@ struct RingArgs {
 // X[q] = sum_j x_j e^{-2 pi i q j / n}, q = 0 .. n/2.  With the real input
 // folded as s_j = x_j + x_{n-j}, d_j = x_j - x_{n-j} (cos is even, sin odd):
 //   X[q] = x_0 + (-1)^q x_{n/2} + sum_{j=1}^{n/2-1} [s_j cos(2 pi q j/n) - i d_j sin(...)],
-// half the terms of the direct sum.  Cosine table, s and d in LDS (NMAX is
-// the longest ring the launch allows); sin(2 pi t / n) = cos(2 pi (t - n/4) / n).
+// half the terms of the direct sum.  Each thread takes two frequencies; along
+// j their (cos, sin) advance by the angle-addition recurrence and are reset
+// from the LDS cosine table every kRotAnchor steps, so rounding drift stays
+// at ~kRotAnchor ulp (sin(2 pi t / n) = cos(2 pi (t - n/4) / n)).
+constexpr int kRotAnchor = 32;
+
+__device__ __forceinline__ void trig_at(const double* ctab, int n, int t, double& c,
+                                        double& s) {
+  int ts = t - n / 4;
+  ts += ts < 0 ? n : 0;
+  c = ctab[t];
+  s = ctab[ts];
+}
+
 template <int NMAX>
 __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
                                                  double2* __restrict__ X, int ns) {
@@ -142,19 +157,43 @@ __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __res
   }
   const double x0 = x[0], xh = x[h];
   __syncthreads();
-  const int q4 = n / 4;
-  for (int q = threadIdx.x; q <= h; q += blockDim.x) {
-    double re = (q & 1) ? x0 - xh : x0 + xh, im = 0.0;
-    int t = 0;
-    for (int j = 1; j < h; ++j) {
-      t += q;
-      t -= t >= n ? n : 0;
-      int ts = t - q4;
-      ts += ts < 0 ? n : 0;
-      re = fma(sd[j], ctab[t], re);
-      im = fma(-sd[NMAX / 2 + j], ctab[ts], im);
+  // frequencies q and q + half2 per thread, half2 = ceil((h + 1) / 2)
+  const int half2 = (h + 2) / 2;
+  for (int qa = threadIdx.x; qa < half2; qa += blockDim.x) {
+    const int qb = qa + half2;                      // <= h + 1; used only if <= h
+    const bool hasb = qb <= h;
+    double ra = (qa & 1) ? x0 - xh : x0 + xh, ia = 0.0;
+    double rb = (qb & 1) ? x0 - xh : x0 + xh, ib = 0.0;
+    double cda, sda, cdb, sdb;                      // one-step rotations
+    trig_at(ctab, n, qa % n, cda, sda);
+    trig_at(ctab, n, hasb ? qb % n : 0, cdb, sdb);
+    double ca = 1.0, sa = 0.0, cb = 1.0, sb = 0.0;  // cos, sin at j = 0
+    int ta = 0, tb = 0;                             // q j mod n at the last anchor
+    for (int j0 = 1; j0 < h; j0 += kRotAnchor) {
+      // anchor at j0 from the table
+      ta += qa * (j0 == 1 ? 1 : kRotAnchor);
+      ta %= n;
+      tb += (hasb ? qb : 0) * (j0 == 1 ? 1 : kRotAnchor);
+      tb %= n;
+      trig_at(ctab, n, ta, ca, sa);
+      trig_at(ctab, n, tb, cb, sb);
+      const int jend = min(h, j0 + kRotAnchor);
+      for (int j = j0; j < jend; ++j) {
+        const double sj = sd[j], dj = sd[NMAX / 2 + j];
+        ra = fma(sj, ca, ra);
+        ia = fma(-dj, sa, ia);
+        rb = fma(sj, cb, rb);
+        ib = fma(-dj, sb, ib);
+        const double ca2 = fma(ca, cda, -sa * sda), sa2 = fma(sa, cda, ca * sda);
+        const double cb2 = fma(cb, cdb, -sb * sdb), sb2 = fma(sb, cdb, cb * sdb);
+        ca = ca2;
+        sa = sa2;
+        cb = cb2;
+        sb = sb2;
+      }
     }
-    X[R.xoff[k] + q] = make_double2(re, im);
+    X[R.xoff[k] + qa] = make_double2(ra, ia);
+    if (hasb) X[R.xoff[k] + qb] = make_double2(rb, ib);
   }
 }
 
@@ -162,7 +201,8 @@ __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __res
 // f_j = Y_0 + (-1)^j Y_{n/2} + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}).
 // Outputs j and n - j share the cosine sum A and see the sine sum B with
 // opposite signs (f_j = base + A - B, f_{n-j} = base + A + B), so each step
-// serves two pixels.
+// serves two pixels; each thread takes two such j, with (cos, sin) rotated
+// along q as in the forward transform.
 template <int NMAX>
 __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
                                                   double* __restrict__ map, int ns) {
@@ -178,23 +218,47 @@ __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __r
     ys[q] = make_double2(2.0 * y.x, 2.0 * y.y);
   }
   __syncthreads();
-  const int q4 = n / 4;
   double* f = map + R.first[k];
-  for (int j = threadIdx.x; j <= h; j += blockDim.x) {
-    const double base = 0.5 * (ys[0].x + ((j & 1) ? -ys[h].x : ys[h].x));
-    double A = 0.0, B = 0.0;
-    int t = 0;
-    for (int q = 1; q < h; ++q) {
-      t += j;
-      t -= t >= n ? n : 0;
-      int ts = t - q4;
-      ts += ts < 0 ? n : 0;
-      const double2 y = ys[q];
-      A = fma(y.x, ctab[t], A);
-      B = fma(y.y, ctab[ts], B);
+  const int half2 = (h + 2) / 2;
+  for (int ja = threadIdx.x; ja < half2; ja += blockDim.x) {
+    const int jb = ja + half2;
+    const bool hasb = jb <= h;
+    double Aa = 0.0, Ba = 0.0, Ab = 0.0, Bb = 0.0;
+    double cda, sda, cdb, sdb;
+    trig_at(ctab, n, ja % n, cda, sda);
+    trig_at(ctab, n, hasb ? jb % n : 0, cdb, sdb);
+    double ca = 1.0, sa = 0.0, cb = 1.0, sb = 0.0;
+    int ta = 0, tb = 0;
+    for (int q0 = 1; q0 < h; q0 += kRotAnchor) {
+      ta += ja * (q0 == 1 ? 1 : kRotAnchor);
+      ta %= n;
+      tb += (hasb ? jb : 0) * (q0 == 1 ? 1 : kRotAnchor);
+      tb %= n;
+      trig_at(ctab, n, ta, ca, sa);
+      trig_at(ctab, n, tb, cb, sb);
+      const int qend = min(h, q0 + kRotAnchor);
+      for (int q = q0; q < qend; ++q) {
+        const double2 y = ys[q];
+        Aa = fma(y.x, ca, Aa);
+        Ba = fma(y.y, sa, Ba);
+        Ab = fma(y.x, cb, Ab);
+        Bb = fma(y.y, sb, Bb);
+        const double ca2 = fma(ca, cda, -sa * sda), sa2 = fma(sa, cda, ca * sda);
+        const double cb2 = fma(cb, cdb, -sb * sdb), sb2 = fma(sb, cdb, cb * sdb);
+        ca = ca2;
+        sa = sa2;
+        cb = cb2;
+        sb = sb2;
+      }
     }
-    f[j] = base + A - B;
-    if (j > 0 && j < h) f[n - j] = base + A + B;
+    const double ya = 0.5 * (ys[0].x + ((ja & 1) ? -ys[h].x : ys[h].x));
+    f[ja] = ya + Aa - Ba;
+    if (ja > 0 && ja < h) f[n - ja] = ya + Aa + Ba;
+    if (hasb) {
+      const double yb = 0.5 * (ys[0].x + ((jb & 1) ? -ys[h].x : ys[h].x));
+      f[jb] = yb + Ab - Bb;
+      if (jb > 0 && jb < h) f[n - jb] = yb + Ab + Bb;
+    }
   }
 }
 
