@@ -55,31 +55,8 @@ void check_block(int lmin, int lmax, int64_t LW, int64_t ld) {
               "leading dimension smaller than lmax-lmin+1");
 }
 
-// Scoped device buffer owned by the context.
-struct DevBuf {
-  Ctx* c;
-  void* p = nullptr;
-  DevBuf(Ctx* cc, size_t bytes) : c(cc) { p = ctx_alloc(c, bytes); }
-  ~DevBuf() {
-    if (p) {
-      (void)hipStreamSynchronize(c->stream);
-      try {
-        ctx_free(c, p);
-      } catch (...) {
-      }
-    }
-  }
-  double* d() const { return static_cast<double*>(p); }
-};
-
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
   PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
-}
-
-// copy an n x n device block (ld n) into a host block with leading dim ld
-// (staged through pinned memory; returns when the host block is complete)
-void d2h_block(Ctx* c, double* host, int64_t ld, const double* dev, int n) {
-  d2h_matrix(c, host, ld, dev, n, n, n);
 }
 
 }  // namespace
@@ -124,11 +101,14 @@ int psx_mcm(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t L
     PSX_REQUIRE(W && M, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
     const int n = lmax - lmin + 1;
-    DevBuf dW(c, sizeof(double) * LW), dM(c, sizeof(double) * n * (size_t)n);
-    h2d(c, dW.p, W, sizeof(double) * LW);
-    int st = psx_mcm_d(h, kind, lmin, lmax, dW.d(), LW, dM.d(), n);
+    const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
+    XferBuf x(c, xfer_bytes({wb, mb}));
+    double* dW = x.take(wb);
+    double* dM = x.take(mb);
+    h2d(c, dW, W, wb);
+    int st = psx_mcm_d(h, kind, lmin, lmax, dW, LW, dM, n);
     if (st != PSX_OK) throw Error{st, c->last_error};
-    d2h_block(c, M, ldM, dM.d(), n);
+    d2h_matrix(c, M, ldM, dM, n, n, n);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -164,13 +144,16 @@ int psx_mcm_pm(psx_ctx* h, int lmin, int lmax, const double* W, int64_t LW,
     PSX_REQUIRE(W && Mpp && Mmm, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
     const int n = lmax - lmin + 1;
-    DevBuf dW(c, sizeof(double) * LW), dP(c, sizeof(double) * n * (size_t)n),
-        dQ(c, sizeof(double) * n * (size_t)n);
-    h2d(c, dW.p, W, sizeof(double) * LW);
-    int st = psx_mcm_pm_d(h, lmin, lmax, dW.d(), LW, dP.d(), dQ.d(), n);
+    const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
+    XferBuf x(c, xfer_bytes({wb, mb, mb}));
+    double* dW = x.take(wb);
+    double* dP = x.take(mb);
+    double* dQ = x.take(mb);
+    h2d(c, dW, W, wb);
+    int st = psx_mcm_pm_d(h, lmin, lmax, dW, LW, dP, dQ, n);
     if (st != PSX_OK) throw Error{st, c->last_error};
-    d2h_block(c, Mpp, ldM, dP.d(), n);
-    d2h_block(c, Mmm, ldM, dQ.d(), n);
+    const D2HMatrix out[2] = {{Mpp, ldM, dP, n, n, n}, {Mmm, ldM, dQ, n, n, n}};
+    d2h_matrices(c, out, 2);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -229,18 +212,18 @@ int psx_mcm_set(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
     const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
     const double* Wh[4] = {W_TT, W_TE, W_ET, W_PP};
     double* Mh[5] = {M_TT, M_TE, M_ET, M_PP, M_MM};
-    DevBuf dw(c, 4 * wb), dm(c, 5 * mb);
+    XferBuf x(c, xfer_bytes({wb, wb, wb, wb, mb, mb, mb, mb, mb}));
     const double* dWp[4];
     double* dMp[5];
     for (int k = 0; k < 4; ++k) {
       dWp[k] = nullptr;
       if (Wh[k]) {
-        double* p = dw.d() + k * LW;
+        double* p = x.take(wb);
         h2d(c, p, Wh[k], wb);
         dWp[k] = p;
       }
     }
-    for (int k = 0; k < 5; ++k) dMp[k] = dm.d() + (size_t)k * n * n;
+    for (int k = 0; k < 5; ++k) dMp[k] = x.take(mb);
     const bool want[5] = {W_TT != nullptr, W_TE != nullptr, W_ET != nullptr,
                           W_PP != nullptr, W_PP != nullptr};
     for (int k = 0; k < 5; ++k)
@@ -248,8 +231,11 @@ int psx_mcm_set(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
     int st = psx_mcm_set_d(h, lmin, lmax, LW, dWp[0], dWp[1], dWp[2], dWp[3], dMp[0],
                            dMp[1], dMp[2], dMp[3], dMp[4], n);
     if (st != PSX_OK) throw Error{st, c->last_error};
+    D2HMatrix out[5];
+    int nout = 0;
     for (int k = 0; k < 5; ++k)
-      if (want[k]) d2h_block(c, Mh[k], ldM, dMp[k], n);
+      if (want[k]) out[nout++] = {Mh[k], ldM, dMp[k], n, n, n};
+    d2h_matrices(c, out, nout);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -304,11 +290,13 @@ int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spect
     PSX_HIP(hipSetDevice(c->device));
     const int n = lmax - lmin + 1;
     const size_t sb = sizeof(double) * (lmax + 1), wb = sizeof(double) * LW;
-    DevBuf buf(c, (nS + nR) * sb + nW * wb), dC(c, sizeof(double) * n * (size_t)n);
+    const size_t cb = sizeof(double) * n * (size_t)n;
+    XferBuf x(c, xfer_bytes({(nS + nR) * sb + nW * wb, cb}));
+    char* p = (char*)x.take((nS + nR) * sb + nW * wb);
+    double* dC = x.take(cb);
     const double* ds[4] = {nullptr, nullptr, nullptr, nullptr};
     const double* dr[4] = {nullptr, nullptr, nullptr, nullptr};
     const double* dw[8] = {nullptr};
-    char* p = (char*)buf.p;
     for (int i = 0; i < nS; ++i) {
       PSX_REQUIRE(spectra[i], PSX_ERR_ARGUMENT, "null spectrum");
       h2d(c, p, spectra[i], sb);
@@ -327,9 +315,9 @@ int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spect
       dw[i] = (const double*)p;
       p += wb;
     }
-    int st = psx_cov_d(h, kind, lmin, lmax, ds, dr, dw, LW, precision, dC.d(), n);
+    int st = psx_cov_d(h, kind, lmin, lmax, ds, dr, dw, LW, precision, dC, n);
     if (st != PSX_OK) throw Error{st, c->last_error};
-    d2h_block(c, Cm, ldC, dC.d(), n);
+    d2h_matrix(c, Cm, ldC, dC, n, n, n);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }

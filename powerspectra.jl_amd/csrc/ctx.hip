@@ -5,11 +5,20 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <vector>
 
 namespace psx {
 
 void* ctx_alloc(Ctx* c, size_t bytes) {
   if (bytes == 0) bytes = 8;
+  if (c->in_use + bytes > c->budget && c->xfer && !c->xfer_busy) {
+    // the idle transfer buffer is a cache: give it back before failing
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    void* p = c->xfer;
+    c->xfer = nullptr;
+    c->xfer_bytes = 0;
+    ctx_free(c, p);
+  }
   PSX_REQUIRE(c->in_use + bytes <= c->budget, PSX_ERR_BUDGET,
               "allocation of " + std::to_string(bytes) +
                   " bytes exceeds the context HBM budget");
@@ -42,6 +51,32 @@ void* ctx_scratch(Ctx* c, size_t bytes) {
   c->scratch = ctx_alloc(c, want);
   c->scratch_bytes = want;
   return c->scratch;
+}
+
+XferBuf::XferBuf(Ctx* cc, size_t bytes) : c(cc) {
+  PSX_REQUIRE(!c->xfer_busy, PSX_ERR_ARGUMENT, "transfer buffer already in use");
+  if (bytes > c->xfer_bytes) {
+    if (c->xfer) {
+      PSX_HIP(hipStreamSynchronize(c->stream));
+      void* p = c->xfer;
+      c->xfer = nullptr;
+      c->xfer_bytes = 0;
+      ctx_free(c, p);
+    }
+    c->xfer = ctx_alloc(c, bytes);
+    c->xfer_bytes = bytes;
+  }
+  c->xfer_busy = true;
+  cap = bytes;
+}
+
+XferBuf::~XferBuf() { c->xfer_busy = false; }
+
+double* XferBuf::take(size_t bytes) {
+  const size_t at = (used + 255) & ~size_t(255);
+  PSX_REQUIRE(at + bytes <= cap, PSX_ERR_ARGUMENT, "transfer buffer carve overflow");
+  used = at + bytes;
+  return reinterpret_cast<double*>(static_cast<char*>(c->xfer) + at);
 }
 
 TimedLaunch::TimedLaunch(Ctx* ctx, const char* kname) : c(ctx), name(kname) {
@@ -305,40 +340,79 @@ void release_stage(Ctx* c) {
   }
 }
 
-void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
-                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols) {
-  const size_t colb = sizeof(double) * (size_t)rows;
-  if (rows <= 0 || cols <= 0) return;
-  if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
-    PSX_HIP(hipMemcpy2DAsync(host, ldh * sizeof(double), dev, ldd * sizeof(double), colb,
-                             (size_t)cols, hipMemcpyDeviceToHost, s));
-    PSX_HIP(hipStreamSynchronize(s));
-    return;
+namespace {
+// one column block of a staged copy: nc columns of `rows` doubles
+struct Chunk {
+  double* host;
+  int64_t ldh;
+  const double* dev;
+  int64_t ldd;
+  int64_t rows, nc;
+};
+
+void chunk_to_stage(hipStream_t s, void* stage, const Chunk& k) {
+  const size_t colb = sizeof(double) * (size_t)k.rows;
+  if (k.ldd == k.rows)
+    PSX_HIP(hipMemcpyAsync(stage, k.dev, colb * (size_t)k.nc, hipMemcpyDeviceToHost, s));
+  else
+    PSX_HIP(hipMemcpy2DAsync(stage, colb, k.dev, k.ldd * sizeof(double), colb, (size_t)k.nc,
+                             hipMemcpyDeviceToHost, s));
+}
+}  // namespace
+
+void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2],
+                     const D2HMatrix* m, int nm) {
+  // Every matrix is cut into stage-sized column blocks and all blocks go
+  // through one two-buffer pipeline, so the DMA stays busy across matrix
+  // boundaries.  Small matrices, and columns longer than a stage buffer, are
+  // copied directly.
+  std::vector<Chunk> ch;
+  for (int i = 0; i < nm; ++i) {
+    const D2HMatrix& x = m[i];
+    if (x.rows <= 0 || x.cols <= 0) continue;
+    const size_t colb = sizeof(double) * (size_t)x.rows;
+    if (colb * (size_t)x.cols <= kDirectBytes || colb > kStageBytes) {
+      PSX_HIP(hipMemcpy2DAsync(x.host, x.ldh * sizeof(double), x.dev, x.ldd * sizeof(double),
+                               colb, (size_t)x.cols, hipMemcpyDeviceToHost, s));
+      continue;
+    }
+    const int64_t cpc = (int64_t)(kStageBytes / colb);  // columns per block
+    for (int64_t j0 = 0; j0 < x.cols; j0 += cpc)
+      ch.push_back({x.host + j0 * x.ldh, x.ldh, x.dev + j0 * x.ldd, x.ldd, x.rows,
+                    std::min(cpc, x.cols - j0)});
   }
-  const int64_t cpc = (int64_t)(kStageBytes / colb);  // columns per chunk
-  const int64_t nch = (cols + cpc - 1) / cpc;
-  auto issue = [&](int64_t k) {
-    const int b = (int)(k & 1);
-    const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
-    PSX_HIP(hipMemcpy2DAsync(stage[b], colb, dev + j0 * ldd, ldd * sizeof(double), colb,
-                             (size_t)nc, hipMemcpyDeviceToHost, s));
-    PSX_HIP(hipEventRecord(ev[b], s));
+  const size_t nch = ch.size();
+  auto issue = [&](size_t k) {
+    chunk_to_stage(s, stage[k & 1], ch[k]);
+    PSX_HIP(hipEventRecord(ev[k & 1], s));
   };
-  issue(0);
+  if (nch > 0) issue(0);
   if (nch > 1) issue(1);
-  for (int64_t k = 0; k < nch; ++k) {
-    const int b = (int)(k & 1);
-    const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
-    PSX_HIP(hipEventSynchronize(ev[b]));
-    copy_cols(host + j0 * ldh, ldh, (const double*)stage[b], rows, rows, nc);
+  for (size_t k = 0; k < nch; ++k) {
+    const Chunk& x = ch[k];
+    PSX_HIP(hipEventSynchronize(ev[k & 1]));
+    copy_cols(x.host, x.ldh, (const double*)stage[k & 1], x.rows, x.rows, x.nc);
     if (k + 2 < nch) issue(k + 2);
   }
+  PSX_HIP(hipStreamSynchronize(s));
+}
+
+void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
+                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols) {
+  const D2HMatrix m{host, ldh, dev, ldd, rows, cols};
+  d2h_matrices_on(s, stage, ev, &m, 1);
+}
+
+void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm) {
+  for (int i = 0; i < nm; ++i)
+    if ((size_t)m[i].rows * (size_t)m[i].cols * sizeof(double) > kDirectBytes) ensure_stage(c);
+  d2h_matrices_on(c->stream, c->stage, c->stage_ev, m, nm);
 }
 
 void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
                 int64_t rows, int64_t cols) {
-  if ((size_t)rows * (size_t)cols * sizeof(double) > kDirectBytes) ensure_stage(c);
-  d2h_matrix_on(c->stream, c->stage, c->stage_ev, host, ldh, dev, ldd, rows, cols);
+  const D2HMatrix m{host, ldh, dev, ldd, rows, cols};
+  d2h_matrices(c, &m, 1);
 }
 
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
@@ -359,8 +433,12 @@ void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ld
     const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
     if (k >= 2) PSX_HIP(hipEventSynchronize(c->stage_ev[b]));  // buffer b free again
     copy_cols((double*)c->stage[b], rows, host + j0 * ldh, ldh, rows, nc);
-    PSX_HIP(hipMemcpy2DAsync(dev + j0 * ldd, ldd * sizeof(double), c->stage[b], colb, colb,
-                             (size_t)nc, hipMemcpyHostToDevice, c->stream));
+    if (ldd == rows)
+      PSX_HIP(hipMemcpyAsync(dev + j0 * ldd, c->stage[b], colb * (size_t)nc,
+                             hipMemcpyHostToDevice, c->stream));
+    else
+      PSX_HIP(hipMemcpy2DAsync(dev + j0 * ldd, ldd * sizeof(double), c->stage[b], colb, colb,
+                               (size_t)nc, hipMemcpyHostToDevice, c->stream));
     PSX_HIP(hipEventRecord(c->stage_ev[b], c->stream));
   }
   PSX_HIP(hipStreamSynchronize(c->stream));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -76,6 +77,12 @@ struct Ctx {
   size_t scratch_bytes = 0;
   // rocSOLVER/rocBLAS handle (created lazily)
   void* blas = nullptr;
+  // device copies of a host-pointer call's inputs and outputs, kept between
+  // calls (grown on demand, given back when an allocation would otherwise
+  // exceed the budget); busy while a call holds it
+  void* xfer = nullptr;
+  size_t xfer_bytes = 0;
+  bool xfer_busy = false;
   // two pinned host staging buffers for large host <-> device copies
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -85,6 +92,23 @@ void resolve_timers(Ctx* c);
 void* ctx_alloc(Ctx* c, size_t bytes);
 void ctx_free(Ctx* c, void* p);
 void* ctx_scratch(Ctx* c, size_t bytes);
+// The context's transfer buffer held for one host-pointer call; take() carves
+// 256-byte aligned pieces, and the sizes passed to the constructor must cover
+// them (xfer_bytes(...) sums them with their padding).
+struct XferBuf {
+  Ctx* c;
+  size_t cap = 0, used = 0;
+  XferBuf(Ctx* cc, size_t bytes);
+  ~XferBuf();
+  XferBuf(const XferBuf&) = delete;
+  XferBuf& operator=(const XferBuf&) = delete;
+  double* take(size_t bytes);
+};
+inline size_t xfer_bytes(std::initializer_list<size_t> pieces) {
+  size_t t = 0;
+  for (size_t b : pieces) t += (b + 255) & ~size_t(255);
+  return t;
+}
 void ensure_tables(Ctx* c, int nmax_index);
 void release_blas(Ctx* c);
 void release_stage(Ctx* c);
@@ -109,6 +133,17 @@ void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ld
 constexpr size_t kStageBytes = size_t(32) << 20;
 void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
                    int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols);
+// several matrices through one pipeline (the DMA does not drain between them)
+struct D2HMatrix {
+  double* host;
+  int64_t ldh;
+  const double* dev;
+  int64_t ldd;
+  int64_t rows, cols;
+};
+void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm);
+void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2],
+                     const D2HMatrix* m, int nm);
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
                 int64_t rows, int64_t cols);
 

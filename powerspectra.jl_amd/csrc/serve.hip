@@ -177,9 +177,11 @@ void execute(Server* S, Slot* sl, Job& j) {
   if (err) PSX_HIP(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, sl->stream));
   PSX_HIP(hipStreamSynchronize(sl->stream));
   PSX_REQUIRE(h == 0, PSX_ERR_DOMAIN, "sqrt of a negative spectrum product (Julia DomainError)");
-  if (j.host_out)
-    for (int k = 0; k < j.nout; ++k)
-      d2h_matrix_on(sl->stream, sl->stage, sl->ev, j.rq.out[k], j.rq.ld, outs[k], n, n, n);
+  if (j.host_out) {
+    D2HMatrix m[5];
+    for (int k = 0; k < j.nout; ++k) m[k] = {j.rq.out[k], j.rq.ld, outs[k], n, n, n};
+    d2h_matrices_on(sl->stream, sl->stage, sl->ev, m, j.nout);
+  }
 }
 
 void worker(Server* S, Slot* sl) {
@@ -598,8 +600,10 @@ int psx_fetch(psx_server* hs, uint64_t ticket, double* const* out, int64_t ld) {
     std::lock_guard<std::recursive_mutex> lc(S->ctx->mu);
     PSX_HIP(hipSetDevice(S->ctx->device));
     const size_t blk = (size_t)j->n * j->n;
+    D2HMatrix m[5];
     for (int k = 0; k < j->nout; ++k)
-      d2h_matrix(S->ctx, out[k], ld, (const double*)j->out_mem + k * blk, j->n, j->n, j->n);
+      m[k] = {out[k], ld, (const double*)j->out_mem + k * blk, j->n, j->n, j->n};
+    d2h_matrices(S->ctx, m, j->nout);
   });
 }
 
