@@ -59,6 +59,80 @@ void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
   PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
 }
 
+ContractRequest block_request(int lmin, int lmax, int64_t LW, int64_t ld) {
+  ContractRequest rq;
+  rq.lmin = lmin;
+  rq.lmax = lmax;
+  rq.LW = LW;
+  rq.ldo = ld;
+  rq.epi = EPI_MCM;
+  return rq;
+}
+
+ContractRequest mcm_request(int kind, int lmin, int lmax, const double* dW, int64_t LW,
+                            double* dM, int64_t ldM) {
+  ContractRequest rq = block_request(lmin, lmax, LW, ldM);
+  rq.nfam[mcm_family(kind)] = 1;
+  rq.dW[0] = dW;
+  rq.dout[0] = dM;
+  return rq;
+}
+
+ContractRequest pm_request(int lmin, int lmax, const double* dW, int64_t LW, double* dMpp,
+                           double* dMmm, int64_t ldM) {
+  ContractRequest rq = block_request(lmin, lmax, LW, ldM);
+  rq.nfam[FAM_PE] = 1;
+  rq.nfam[FAM_PO] = 1;
+  rq.dW[0] = rq.dW[1] = dW;
+  rq.dout[0] = dMpp;
+  rq.dout[1] = dMmm;
+  return rq;
+}
+
+// TT, TE, ET, ++ and -- from the four W spectra (TT, TE, ET, PP) in one launch
+ContractRequest set_request(int lmin, int lmax, int64_t LW, const double* const dW[4],
+                            double* const dM[5], int64_t ldM) {
+  ContractRequest rq = block_request(lmin, lmax, LW, ldM);
+  rq.nfam[FAM_00] = 1;
+  rq.nfam[FAM_02] = 2;
+  rq.nfam[FAM_PE] = 1;
+  rq.nfam[FAM_PO] = 1;
+  const double* Ws[5] = {dW[0], dW[1], dW[2], dW[3], dW[3]};
+  for (int k = 0; k < 5; ++k) {
+    rq.dW[k] = Ws[k];
+    rq.dout[k] = dM[k];
+  }
+  return rq;
+}
+
+ContractRequest cov_request(int kind, int lmin, int lmax, const double* const* dspectra,
+                            const double* const* dratios, const double* const* dW, int64_t LW,
+                            int precision, double* dC, int64_t ldC) {
+  int nW = 0, nS = 0, nR = 0;
+  PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
+              "unknown covariance kind");
+  PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED, PSX_ERR_ARGUMENT,
+              "precision must be PSX_FP64 or PSX_MIXED");
+  PSX_REQUIRE(precision == PSX_FP64, PSX_ERR_ARGUMENT,
+              "PSX_MIXED covariance contraction is not available in this build");
+  PSX_REQUIRE(dW && dC && dspectra, PSX_ERR_ARGUMENT, "null pointer");
+  for (int i = 0; i < nW; ++i) PSX_REQUIRE(dW[i], PSX_ERR_ARGUMENT, "null W spectrum");
+  for (int i = 0; i < nS; ++i) PSX_REQUIRE(dspectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+  for (int i = 0; i < nR; ++i)
+    PSX_REQUIRE(dratios && dratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+  ContractRequest rq = block_request(lmin, lmax, LW, ldC);
+  cov_layout(kind, rq, dW);
+  rq.epi = EPI_COV;
+  rq.cov_kind = kind;
+  rq.dout[0] = dC;
+  for (int i = 0; i < 4; ++i) {
+    rq.dspec[i] = i < nS ? dspectra[i] : nullptr;
+    rq.dratio[i] = i < nR ? dratios[i] : nullptr;
+  }
+  rq.precision = precision;
+  return rq;
+}
+
 }  // namespace
 
 extern "C" {
@@ -80,16 +154,7 @@ int psx_mcm_d(psx_ctx* h, int kind, int lmin, int lmax, const double* dW,
     check_block(lmin, lmax, LW, ldM);
     PSX_REQUIRE(dW && dM, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
-    ContractRequest rq;
-    rq.lmin = lmin;
-    rq.lmax = lmax;
-    rq.LW = LW;
-    rq.nfam[mcm_family(kind)] = 1;
-    rq.dW[0] = dW;
-    rq.epi = EPI_MCM;
-    rq.dout[0] = dM;
-    rq.ldo = ldM;
-    run_contraction(c, rq);
+    run_contraction(c, mcm_request(kind, lmin, lmax, dW, LW, dM, ldM));
   });
 }
 
@@ -106,10 +171,7 @@ int psx_mcm(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t L
     double* dW = x.take(wb);
     double* dM = x.take(mb);
     h2d(c, dW, W, wb);
-    int st = psx_mcm_d(h, kind, lmin, lmax, dW, LW, dM, n);
-    if (st != PSX_OK) throw Error{st, c->last_error};
-    d2h_matrix(c, M, ldM, dM, n, n, n);
-    PSX_HIP(hipStreamSynchronize(c->stream));
+    run_contraction_to_host(c, mcm_request(kind, lmin, lmax, dW, LW, dM, n), &M, ldM);
   });
 }
 
@@ -120,19 +182,7 @@ int psx_mcm_pm_d(psx_ctx* h, int lmin, int lmax, const double* dW, int64_t LW,
     check_block(lmin, lmax, LW, ldM);
     PSX_REQUIRE(dW && dMpp && dMmm, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
-    ContractRequest rq;
-    rq.lmin = lmin;
-    rq.lmax = lmax;
-    rq.LW = LW;
-    rq.nfam[FAM_PE] = 1;
-    rq.nfam[FAM_PO] = 1;
-    rq.dW[0] = dW;
-    rq.dW[1] = dW;
-    rq.epi = EPI_MCM;
-    rq.dout[0] = dMpp;
-    rq.dout[1] = dMmm;
-    rq.ldo = ldM;
-    run_contraction(c, rq);
+    run_contraction(c, pm_request(lmin, lmax, dW, LW, dMpp, dMmm, ldM));
   });
 }
 
@@ -150,11 +200,8 @@ int psx_mcm_pm(psx_ctx* h, int lmin, int lmax, const double* W, int64_t LW,
     double* dP = x.take(mb);
     double* dQ = x.take(mb);
     h2d(c, dW, W, wb);
-    int st = psx_mcm_pm_d(h, lmin, lmax, dW, LW, dP, dQ, n);
-    if (st != PSX_OK) throw Error{st, c->last_error};
-    const D2HMatrix out[2] = {{Mpp, ldM, dP, n, n, n}, {Mmm, ldM, dQ, n, n, n}};
-    d2h_matrices(c, out, 2);
-    PSX_HIP(hipStreamSynchronize(c->stream));
+    double* const host[2] = {Mpp, Mmm};
+    run_contraction_to_host(c, pm_request(lmin, lmax, dW, LW, dP, dQ, n), host, ldM);
   });
 }
 
@@ -170,23 +217,9 @@ int psx_mcm_set_d(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* dW_T
     if (all) {
       PSX_REQUIRE(dM_TT && dM_TE && dM_ET && dM_PP && dM_MM, PSX_ERR_ARGUMENT,
                   "null output pointer");
-      ContractRequest rq;
-      rq.lmin = lmin;
-      rq.lmax = lmax;
-      rq.LW = LW;
-      rq.nfam[FAM_00] = 1;
-      rq.nfam[FAM_02] = 2;
-      rq.nfam[FAM_PE] = 1;
-      rq.nfam[FAM_PO] = 1;
-      const double* Ws[5] = {dW_TT, dW_TE, dW_ET, dW_PP, dW_PP};
-      double* Ms[5] = {dM_TT, dM_TE, dM_ET, dM_PP, dM_MM};
-      for (int k = 0; k < 5; ++k) {
-        rq.dW[k] = Ws[k];
-        rq.dout[k] = Ms[k];
-      }
-      rq.epi = EPI_MCM;
-      rq.ldo = ldM;
-      run_contraction(c, rq);
+      const double* const dW[4] = {dW_TT, dW_TE, dW_ET, dW_PP};
+      double* const dM[5] = {dM_TT, dM_TE, dM_ET, dM_PP, dM_MM};
+      run_contraction(c, set_request(lmin, lmax, LW, dW, dM, ldM));
       return;
     }
     // partial set: run the blocks that were asked for
@@ -212,31 +245,32 @@ int psx_mcm_set(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
     const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
     const double* Wh[4] = {W_TT, W_TE, W_ET, W_PP};
     double* Mh[5] = {M_TT, M_TE, M_ET, M_PP, M_MM};
-    XferBuf x(c, xfer_bytes({wb, wb, wb, wb, mb, mb, mb, mb, mb}));
-    const double* dWp[4];
-    double* dMp[5];
-    for (int k = 0; k < 4; ++k) {
-      dWp[k] = nullptr;
-      if (Wh[k]) {
-        double* p = x.take(wb);
-        h2d(c, p, Wh[k], wb);
-        dWp[k] = p;
-      }
-    }
-    for (int k = 0; k < 5; ++k) dMp[k] = x.take(mb);
     const bool want[5] = {W_TT != nullptr, W_TE != nullptr, W_ET != nullptr,
                           W_PP != nullptr, W_PP != nullptr};
     for (int k = 0; k < 5; ++k)
       if (want[k]) PSX_REQUIRE(Mh[k] != nullptr, PSX_ERR_ARGUMENT, "null output pointer");
-    int st = psx_mcm_set_d(h, lmin, lmax, LW, dWp[0], dWp[1], dWp[2], dWp[3], dMp[0],
-                           dMp[1], dMp[2], dMp[3], dMp[4], n);
-    if (st != PSX_OK) throw Error{st, c->last_error};
-    D2HMatrix out[5];
-    int nout = 0;
-    for (int k = 0; k < 5; ++k)
-      if (want[k]) out[nout++] = {Mh[k], ldM, dMp[k], n, n, n};
-    d2h_matrices(c, out, nout);
-    PSX_HIP(hipStreamSynchronize(c->stream));
+    XferBuf x(c, xfer_bytes({wb, wb, wb, wb, mb, mb, mb, mb, mb}));
+    double* dWp[4];
+    double* dMp[5];
+    for (int k = 0; k < 4; ++k) {
+      dWp[k] = x.take(wb);
+      if (Wh[k]) h2d(c, dWp[k], Wh[k], wb);
+    }
+    for (int k = 0; k < 5; ++k) dMp[k] = x.take(mb);
+    if (W_TT && W_TE && W_ET && W_PP) {
+      run_contraction_to_host(c, set_request(lmin, lmax, LW, dWp, dMp, n), Mh, ldM);
+      return;
+    }
+    // partial set: the requested blocks one after another
+    for (int k = 0; k < 3; ++k)
+      if (Wh[k]) {
+        const int kind = k == 0 ? PSX_MCM_00 : PSX_MCM_02;
+        run_contraction_to_host(c, mcm_request(kind, lmin, lmax, dWp[k], LW, dMp[k], n),
+                                &Mh[k], ldM);
+      }
+    if (W_PP)
+      run_contraction_to_host(c, pm_request(lmin, lmax, dWp[3], LW, dMp[3], dMp[4], n),
+                              &Mh[3], ldM);
   });
 }
 
@@ -246,33 +280,9 @@ int psx_cov_d(psx_ctx* h, int kind, int lmin, int lmax, const double* const* dsp
   Ctx* c = C(h);
   return guarded(c, [&] {
     check_block(lmin, lmax, LW, ldC);
-    int nW = 0, nS = 0, nR = 0;
-    PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
-                "unknown covariance kind");
-    PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED, PSX_ERR_ARGUMENT,
-                "precision must be PSX_FP64 or PSX_MIXED");
-    PSX_REQUIRE(precision == PSX_FP64, PSX_ERR_ARGUMENT,
-                "PSX_MIXED covariance contraction is not available in this build");
-    PSX_REQUIRE(dW && dC && dspectra, PSX_ERR_ARGUMENT, "null pointer");
-    for (int i = 0; i < nW; ++i) PSX_REQUIRE(dW[i], PSX_ERR_ARGUMENT, "null W spectrum");
-    for (int i = 0; i < nS; ++i) PSX_REQUIRE(dspectra[i], PSX_ERR_ARGUMENT, "null spectrum");
-    for (int i = 0; i < nR; ++i)
-      PSX_REQUIRE(dratios && dratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+    ContractRequest rq =
+        cov_request(kind, lmin, lmax, dspectra, dratios, dW, LW, precision, dC, ldC);
     PSX_HIP(hipSetDevice(c->device));
-    ContractRequest rq;
-    rq.lmin = lmin;
-    rq.lmax = lmax;
-    rq.LW = LW;
-    cov_layout(kind, rq, dW);
-    rq.epi = EPI_COV;
-    rq.cov_kind = kind;
-    rq.dout[0] = dC;
-    rq.ldo = ldC;
-    for (int i = 0; i < 4; ++i) {
-      rq.dspec[i] = i < nS ? dspectra[i] : nullptr;
-      rq.dratio[i] = i < nR ? dratios[i] : nullptr;
-    }
-    rq.precision = precision;
     run_contraction(c, rq);
   });
 }
@@ -287,6 +297,10 @@ int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spect
     PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
                 "unknown covariance kind");
     PSX_REQUIRE(W && Cm && spectra, PSX_ERR_ARGUMENT, "null pointer");
+    for (int i = 0; i < nS; ++i) PSX_REQUIRE(spectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+    for (int i = 0; i < nR; ++i)
+      PSX_REQUIRE(ratios && ratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+    for (int i = 0; i < nW; ++i) PSX_REQUIRE(W[i], PSX_ERR_ARGUMENT, "null W spectrum");
     PSX_HIP(hipSetDevice(c->device));
     const int n = lmax - lmin + 1;
     const size_t sb = sizeof(double) * (lmax + 1), wb = sizeof(double) * LW;
@@ -297,28 +311,20 @@ int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spect
     const double* ds[4] = {nullptr, nullptr, nullptr, nullptr};
     const double* dr[4] = {nullptr, nullptr, nullptr, nullptr};
     const double* dw[8] = {nullptr};
-    for (int i = 0; i < nS; ++i) {
-      PSX_REQUIRE(spectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+    for (int i = 0; i < nS; ++i, p += sb) {
       h2d(c, p, spectra[i], sb);
       ds[i] = (const double*)p;
-      p += sb;
     }
-    for (int i = 0; i < nR; ++i) {
-      PSX_REQUIRE(ratios && ratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+    for (int i = 0; i < nR; ++i, p += sb) {
       h2d(c, p, ratios[i], sb);
       dr[i] = (const double*)p;
-      p += sb;
     }
-    for (int i = 0; i < nW; ++i) {
-      PSX_REQUIRE(W[i], PSX_ERR_ARGUMENT, "null W spectrum");
+    for (int i = 0; i < nW; ++i, p += wb) {
       h2d(c, p, W[i], wb);
       dw[i] = (const double*)p;
-      p += wb;
     }
-    int st = psx_cov_d(h, kind, lmin, lmax, ds, dr, dw, LW, precision, dC, n);
-    if (st != PSX_OK) throw Error{st, c->last_error};
-    d2h_matrix(c, Cm, ldC, dC, n, n, n);
-    PSX_HIP(hipStreamSynchronize(c->stream));
+    ContractRequest rq = cov_request(kind, lmin, lmax, ds, dr, dw, LW, precision, dC, n);
+    run_contraction_to_host(c, rq, &Cm, ldC);
   });
 }
 

@@ -378,9 +378,10 @@ __global__ void __launch_bounds__(256) record_kernel(RecArgs a) {
 
 // ---- tile lists ----------------------------------------------------------
 struct TileKey {
-  int lmin, lmax, LW, R, waves;
+  int lmin, lmax, LW, R, waves, nbands;
   bool operator<(const TileKey& o) const {
-    return std::tie(lmin, lmax, LW, R, waves) < std::tie(o.lmin, o.lmax, o.LW, o.R, o.waves);
+    return std::tie(lmin, lmax, LW, R, waves, nbands) <
+           std::tie(o.lmin, o.lmax, o.LW, o.R, o.waves, o.nbands);
   }
 };
 
@@ -475,6 +476,59 @@ std::vector<Tile> make_tiles(int lmin, int lmax, int LW, int R, int waves) {
   return out;
 }
 
+// One l1 band of a tile list: workgroups [wg0, wg0 + nwg).  Once this band
+// and every earlier one have run, every pair with l1 < done_l is written, and
+// with it every output column l < done_l: column l holds pairs (l1, l) with
+// l1 <= l and, mirrored, pairs (l, l2) with l2 > l, so it needs exactly the
+// pairs with l1 <= l.
+struct TileBand {
+  int wg0, nwg, done_l;
+};
+
+// Reorder the workgroups of `tiles` (blocks of `waves` tiles) into nbands l1
+// bands by the top lane of their highest tile, keeping the launch order
+// inside a band.  Band edges sit at 1/8, 1/4, 1/2 and 3/4 of the l range: the
+// work per l1 row grows like l1 (lmax - l1), so the first band is cheap and
+// the columns it completes can be copied out while the rest runs.
+std::vector<TileBand> band_tiles(std::vector<Tile>& tiles, int waves, int lmin, int lmax,
+                                 int nbands) {
+  const int nwg = (int)(tiles.size() / waves);
+  if (nbands <= 1) return {TileBand{0, nwg, lmax + 1}};
+  static const double kEdge[4] = {0.125, 0.25, 0.5, 0.75};
+  const int n = lmax - lmin + 1;
+  std::vector<int> edge;
+  for (int k = 0; k < nbands - 1 && k < 4; ++k) edge.push_back(lmin + (int)(kEdge[k] * n));
+  const int nb = (int)edge.size() + 1;
+  auto band_of = [&](int w) {
+    int top = -1;
+    for (int j = 0; j < waves; ++j) top = std::max(top, tiles[(size_t)w * waves + j].hi);
+    int b = 0;
+    while (b < nb - 1 && top >= edge[b]) ++b;
+    return b;
+  };
+  std::vector<std::vector<int>> members(nb);
+  for (int w = 0; w < nwg; ++w) members[band_of(w)].push_back(w);
+  std::vector<Tile> out;
+  out.reserve(tiles.size());
+  std::vector<TileBand> bands;
+  for (int b = 0; b < nb; ++b) {
+    TileBand tb{(int)(out.size() / waves), (int)members[b].size(), lmax + 1};
+    for (int w : members[b])
+      for (int j = 0; j < waves; ++j) out.push_back(tiles[(size_t)w * waves + j]);
+    bands.push_back(tb);
+  }
+  // done_l of band b: the lowest l1 any later band still owns
+  int low = lmax + 1;
+  for (int b = nb - 1; b >= 0; --b) {
+    bands[b].done_l = low;
+    const TileBand& tb = bands[b];
+    for (size_t i = (size_t)tb.wg0 * waves; i < (size_t)(tb.wg0 + tb.nwg) * waves; ++i)
+      if (out[i].hi >= out[i].lo) low = std::min(low, out[i].lo);
+  }
+  tiles.swap(out);
+  return bands;
+}
+
 struct Geometry {
   int R;
   int wh_len;     // records (l3 = -kWOff .. wh_len - kWOff - 1)
@@ -528,6 +582,7 @@ struct CachedTiles {
   Tile* d = nullptr;
   int n = 0;
   std::vector<Tile> host;
+  std::vector<TileBand> bands;  // in launch order, covering all workgroups
   Geometry geo;  // checked once per tile list (the table size is checked per call)
 };
 
@@ -551,13 +606,15 @@ void release_tile_cache(Ctx* c) {
 
 namespace {
 
-const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int waves) {
+const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int waves,
+                             int nbands) {
   std::lock_guard<std::mutex> lk(g_tile_mu);
-  auto key = std::make_pair((void*)c, TileKey{lmin, lmax, LW, R, waves});
+  auto key = std::make_pair((void*)c, TileKey{lmin, lmax, LW, R, waves, nbands});
   auto it = g_tile_cache.find(key);
   if (it != g_tile_cache.end()) return it->second;
   CachedTiles ct;
   ct.host = make_tiles(lmin, lmax, LW, R, waves);
+  ct.bands = band_tiles(ct.host, waves, lmin, lmax, nbands);
   ct.geo = check_geometry(ct.host, lmax, LW, R, 0);
   ct.n = (int)ct.host.size();
   ct.d = (Tile*)ctx_alloc(c, sizeof(Tile) * std::max(1, ct.n));
@@ -664,7 +721,9 @@ int nv_of(const ContractRequest& rq) {
 }  // namespace
 
 static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
-                                    void* scratch, size_t scratch_bytes);
+                                    void* scratch, size_t scratch_bytes, int nbands = 1,
+                                    std::vector<std::pair<int, hipEvent_t>>* band_done =
+                                        nullptr);
 
 struct RecLayout {
   int nv, ne, rs, ft, ft1;
@@ -702,13 +761,18 @@ int* run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
   return run_contraction_checked(c, rqe, s, scratch, scratch_bytes);
 }
 
+// With band_done, the tiles run as nbands launches in l1 bands and after each
+// one an event is recorded on s and listed with the band's done_l; the
+// covariance flag is then left to the caller.
 static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream_t s,
-                                    void* scratch, size_t scratch_bytes) {
+                                    void* scratch, size_t scratch_bytes, int nbands,
+                                    std::vector<std::pair<int, hipEvent_t>>* band_done) {
   const int nv = nv_of(rq);
   const int n = rq.lmax - rq.lmin + 1;
   PSX_REQUIRE(rq.ldo >= n, PSX_ERR_AXES, "leading dimension smaller than lmax-lmin+1");
   const Config cf = pick_config(rq);
-  const CachedTiles& tl = get_tiles(c, rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves);
+  const CachedTiles& tl =
+      get_tiles(c, rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves, band_done ? nbands : 1);
   ensure_tables(c, (int)rq.LW + 2 * rq.lmax + 512);
   const Geometry& g = tl.geo;
   PSX_REQUIRE(g.tab_need + kTabOff < c->tables.n, PSX_ERR_ARGUMENT,
@@ -766,6 +830,26 @@ static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   }
   ka.err = err;
   const int threads = 64 * cf.waves;
+  if (band_done) {
+    // every band on s, an event after each (one timed region for all)
+    PSX_REQUIRE(s == c->stream && tl.bands.size() <= kMaxBands, PSX_ERR_ARGUMENT,
+                "banded contraction needs the context stream");
+    ensure_band_events(c);
+    TimedLaunch tlch(c, cf.name);
+    for (size_t b = 0; b < tl.bands.size(); ++b) {
+      const TileBand& tb = tl.bands[b];
+      if (tb.nwg > 0) {
+        KArgs kb = ka;
+        kb.tiles = tl.d + (size_t)tb.wg0 * cf.waves;
+        kb.ntiles = tb.nwg * cf.waves;
+        hipLaunchKernelGGL(cf.fn, dim3((unsigned)tb.nwg), dim3(threads), 0, s, kb);
+        PSX_HIP(hipGetLastError());
+      }
+      PSX_HIP(hipEventRecord(c->band_ev[b], s));
+      band_done->emplace_back(tb.done_l, c->band_ev[b]);
+    }
+    return rq.epi == EPI_COV ? err : nullptr;
+  }
   const unsigned blocks = (unsigned)((tl.n + cf.waves - 1) / cf.waves);
   {
     if (s == c->stream) {
@@ -789,6 +873,57 @@ static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
 void run_contraction(Ctx* c, const ContractRequest& rq) {
   void* scratch = ctx_scratch(c, contraction_scratch_bytes(rq));
   run_contraction_on(c, rq, c->stream, scratch, c->scratch_bytes);
+}
+
+void run_contraction_to_host(Ctx* c, const ContractRequest& rq, double* const* host,
+                             int64_t ldh) {
+  const int n = rq.lmax - rq.lmin + 1;
+  const int nout = rq.epi == EPI_MCM ? nv_of(rq) : 1;
+  PSX_REQUIRE(rq.ldo == n, PSX_ERR_ARGUMENT, "device outputs must be packed (ld = n)");
+  for (int k = 0; k < nout; ++k) PSX_REQUIRE(host[k], PSX_ERR_ARGUMENT, "null output pointer");
+  // bands only pay when the outputs outgrow one staging buffer
+  const bool banded = (size_t)nout * n * (size_t)n * sizeof(double) > kStageBytes;
+  void* scratch = ctx_scratch(c, contraction_scratch_bytes(rq));
+  std::vector<std::pair<int, hipEvent_t>> done;
+  int* err = nullptr;
+  if (banded) {
+    const int nv = nv_of(rq);
+    PSX_REQUIRE(nv >= 1 && nv <= kMaxV, PSX_ERR_ARGUMENT, "bad number of W spectra");
+    PSX_REQUIRE(rq.LW >= 1 && rq.LW < (1 << 24), PSX_ERR_ARGUMENT, "bad W length");
+    PSX_REQUIRE(rq.lmax <= 16383, PSX_ERR_ARGUMENT, "lmax above 16383 is not supported");
+    ContractRequest rqe = rq;
+    rqe.LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
+    err = run_contraction_checked(c, rqe, c->stream, scratch, c->scratch_bytes, kMaxBands,
+                                  &done);
+  } else {
+    // one launch; it checks the covariance flag itself, and the copy follows
+    // it on the context stream
+    run_contraction_on(c, rq, c->stream, scratch, c->scratch_bytes);
+    std::vector<D2HMatrix> whole;
+    for (int k = 0; k < nout; ++k) whole.push_back({host[k], ldh, rq.dout[k], n, n, n});
+    d2h_matrices(c, whole.data(), nout);
+    return;
+  }
+  // columns [j0, j1) of every output once the band completing them is done
+  std::vector<D2HMatrix> pieces;
+  int64_t j0 = 0;
+  for (const auto& bd : done) {
+    const int64_t j1 = std::min<int64_t>(n, bd.first - rq.lmin);
+    if (j1 <= j0) continue;
+    for (int k = 0; k < nout; ++k)
+      pieces.push_back({host[k] + j0 * ldh, ldh, rq.dout[k] + j0 * n, n, n, j1 - j0,
+                        bd.second});
+    j0 = j1;
+  }
+  PSX_REQUIRE(j0 == n, PSX_ERR_ARGUMENT, "band bookkeeping left columns uncovered");
+  d2h_matrices_overlapped(c, pieces.data(), (int)pieces.size());
+  PSX_HIP(hipStreamSynchronize(c->stream));
+  if (err) {
+    int h = 0;
+    PSX_HIP(hipMemcpy(&h, err, sizeof(int), hipMemcpyDeviceToHost));
+    PSX_REQUIRE(h == 0, PSX_ERR_DOMAIN,
+                "sqrt of a negative spectrum product (Julia DomainError)");
+  }
 }
 
 }  // namespace psx

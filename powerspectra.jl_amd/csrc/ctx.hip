@@ -302,6 +302,7 @@ void ensure_stage(Ctx* c) {
     PSX_HIP(hipHostMalloc(&c->stage[i], kStageBytes, hipHostMallocDefault));
     PSX_HIP(hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
   }
+  PSX_HIP(hipStreamCreateWithFlags(&c->xstream, hipStreamNonBlocking));
 }
 
 // dst(:, j) = src(:, j) for j < cols, column-major, split over host threads
@@ -332,12 +333,24 @@ void copy_cols(double* dst, int64_t ldd, const double* src, int64_t lds, int64_t
 }  // namespace
 
 void release_stage(Ctx* c) {
+  if (c->xstream) (void)hipStreamSynchronize(c->xstream);
   for (int i = 0; i < 2; ++i) {
     if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
     if (c->stage[i]) (void)hipHostFree(c->stage[i]);
     c->stage_ev[i] = nullptr;
     c->stage[i] = nullptr;
   }
+  if (c->xstream) (void)hipStreamDestroy(c->xstream);
+  c->xstream = nullptr;
+  for (hipEvent_t& e : c->band_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+}
+
+void ensure_band_events(Ctx* c) {
+  for (hipEvent_t& e : c->band_ev)
+    if (!e) PSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 namespace {
@@ -348,6 +361,7 @@ struct Chunk {
   const double* dev;
   int64_t ldd;
   int64_t rows, nc;
+  hipEvent_t after;
 };
 
 void chunk_to_stage(hipStream_t s, void* stage, const Chunk& k) {
@@ -365,13 +379,16 @@ void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2]
   // Every matrix is cut into stage-sized column blocks and all blocks go
   // through one two-buffer pipeline, so the DMA stays busy across matrix
   // boundaries.  Small matrices, and columns longer than a stage buffer, are
-  // copied directly.
+  // copied directly, ahead of the pipeline.
   std::vector<Chunk> ch;
   for (int i = 0; i < nm; ++i) {
     const D2HMatrix& x = m[i];
     if (x.rows <= 0 || x.cols <= 0) continue;
     const size_t colb = sizeof(double) * (size_t)x.rows;
-    if (colb * (size_t)x.cols <= kDirectBytes || colb > kStageBytes) {
+    // (a piece that waits for an event is always staged: a direct copy to
+    // pageable memory would hold this thread until the event fires)
+    if ((!x.after && colb * (size_t)x.cols <= kDirectBytes) || colb > kStageBytes) {
+      if (x.after) PSX_HIP(hipStreamWaitEvent(s, x.after, 0));
       PSX_HIP(hipMemcpy2DAsync(x.host, x.ldh * sizeof(double), x.dev, x.ldd * sizeof(double),
                                colb, (size_t)x.cols, hipMemcpyDeviceToHost, s));
       continue;
@@ -379,10 +396,11 @@ void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2]
     const int64_t cpc = (int64_t)(kStageBytes / colb);  // columns per block
     for (int64_t j0 = 0; j0 < x.cols; j0 += cpc)
       ch.push_back({x.host + j0 * x.ldh, x.ldh, x.dev + j0 * x.ldd, x.ldd, x.rows,
-                    std::min(cpc, x.cols - j0)});
+                    std::min(cpc, x.cols - j0), x.after});
   }
   const size_t nch = ch.size();
   auto issue = [&](size_t k) {
+    if (ch[k].after) PSX_HIP(hipStreamWaitEvent(s, ch[k].after, 0));
     chunk_to_stage(s, stage[k & 1], ch[k]);
     PSX_HIP(hipEventRecord(ev[k & 1], s));
   };
@@ -405,7 +423,8 @@ void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], 
 
 void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm) {
   for (int i = 0; i < nm; ++i)
-    if ((size_t)m[i].rows * (size_t)m[i].cols * sizeof(double) > kDirectBytes) ensure_stage(c);
+    if ((size_t)m[i].rows * (size_t)m[i].cols * sizeof(double) > kDirectBytes || m[i].after)
+      ensure_stage(c);
   d2h_matrices_on(c->stream, c->stage, c->stage_ev, m, nm);
 }
 
@@ -413,6 +432,11 @@ void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ld
                 int64_t rows, int64_t cols) {
   const D2HMatrix m{host, ldh, dev, ldd, rows, cols};
   d2h_matrices(c, &m, 1);
+}
+
+void d2h_matrices_overlapped(Ctx* c, const D2HMatrix* m, int nm) {
+  ensure_stage(c);
+  d2h_matrices_on(c->xstream, c->stage, c->stage_ev, m, nm);
 }
 
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,

@@ -83,10 +83,16 @@ struct Ctx {
   void* xfer = nullptr;
   size_t xfer_bytes = 0;
   bool xfer_busy = false;
-  // two pinned host staging buffers for large host <-> device copies
+  // two pinned host staging buffers for large host <-> device copies, and the
+  // stream that copies device results out while later kernels still run
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  hipStream_t xstream = nullptr;
+  // one event per l1 band of a banded contraction
+  hipEvent_t band_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
+constexpr size_t kMaxBands = 5;
+void ensure_band_events(Ctx* c);
 
 void resolve_timers(Ctx* c);
 void* ctx_alloc(Ctx* c, size_t bytes);
@@ -140,8 +146,12 @@ struct D2HMatrix {
   const double* dev;
   int64_t ldd;
   int64_t rows, cols;
+  hipEvent_t after = nullptr;  // the copy waits for this event when set
 };
 void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm);
+// the same on the context's copy stream, so that the pieces waiting for
+// events of the context stream copy out while later work there still runs
+void d2h_matrices_overlapped(Ctx* c, const D2HMatrix* m, int nm);
 void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2],
                      const D2HMatrix* m, int nm);
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
@@ -197,6 +207,13 @@ void run_contraction(Ctx* c, const ContractRequest& rq);
 int* run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
                         void* scratch, size_t scratch_bytes);
 size_t contraction_scratch_bytes(const ContractRequest& rq);
+// A contraction into packed device outputs (rq.ldo = n) followed by their
+// copy to the host blocks host[k] (leading dimension ldh).  Large outputs run
+// in l1 bands: the columns each band completes are copied out while the
+// later bands compute.  Returns with the host blocks complete; raises the
+// covariance domain error after the copy.
+void run_contraction_to_host(Ctx* c, const ContractRequest& rq, double* const* host,
+                             int64_t ldh);
 // request building shared by the entry points and the server
 int mcm_family(int kind);
 void cov_layout(int kind, ContractRequest& rq, const double* const* W);
