@@ -481,6 +481,13 @@ def serve_requests(n_req: int, seed: int, lmin_: int = 511, lmax_: int = 4095):
     return out
 
 
+# Poisson load points of the serving workload: (fraction of saturation,
+# admission policy, deadline slack); the second is the headline.
+SERVE_RUNS = [(0.8, "SCHED_FIFO", 2.0), (0.8, "SCHED_DEADLINE", 2.0),
+              (0.8, "SCHED_DEADLINE", 4.0), (0.5, "SCHED_DEADLINE", 2.0),
+              (0.4, "SCHED_DEADLINE", 2.0), (0.3, "SCHED_DEADLINE", 2.0)]
+
+
 def run_serve(args, pkg) -> dict:
     """Continuous batching through psx_server: saturation throughput, then a
     Poisson stream at 80 % of it; latency of every request in the mix against
@@ -584,9 +591,7 @@ def run_serve(args, pkg) -> dict:
                 "slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99),
                 "p99_over_alone_p99": pct(lat, 99) / pct(alone, 99)}
 
-    runs = [poisson(0.8 * sat, abi.SCHED_FIFO), poisson(0.8 * sat, abi.SCHED_DEADLINE, 2.0),
-            poisson(0.8 * sat, abi.SCHED_DEADLINE, 4.0), poisson(0.5 * sat, abi.SCHED_DEADLINE, 2.0),
-            poisson(0.4 * sat, abi.SCHED_DEADLINE, 2.0), poisson(0.3 * sat, abi.SCHED_DEADLINE, 2.0)]
+    runs = [poisson(load * sat, getattr(abi, policy), slack) for load, policy, slack in SERVE_RUNS]
     srv.close()
     ctx.close()
     head = runs[1]
