@@ -342,6 +342,159 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
   }
 }
 
+// ---- triangular solves with the LU factors, one right-hand side ------------
+// A blocked substitution in kTrsRows-row blocks, one launch per block step.
+// Forward (unit lower L): launch k updates every row block from k on with the
+// solution of block k-1, then workgroup 0, which holds block k, solves block
+// k's diagonal triangle.  Back (U, diagonal included): launch k, from the last
+// block down, updates row blocks k, k-1, .., 0 with block k+1's solution, and
+// workgroup 0 solves block k.  Each launch reads its columns once, so a
+// triangle costs n^2/2 reads.
+//   The update is a column-major GEMV with two threads per row, each taking
+// 64 columns with all its loads in flight.  Workgroup 0 stages the whole
+// diagonal block in LDS (132 KB) while it runs its own rows' update, then
+// solves the block from LDS in two 64-row sub-blocks, one wave each, the
+// solved value broadcast by v_readlane.
+constexpr int kTrsRows = 128;
+constexpr int kTrsLd = kTrsRows + 1;   // LDS column stride of the diagonal block
+constexpr int kTrsMaxRhs = 4;          // more right-hand sides go through rocSOLVER's getrs
+
+// lane j's value in every lane (j uniform: two v_readlane_b32)
+__device__ inline double trs_bcast(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <bool UPPER>
+__global__ void __launch_bounds__(256) trs_step(const double* __restrict__ A, int64_t lda, int n,
+                                                int k, double* __restrict__ x) {
+  __shared__ double D[kTrsRows * kTrsLd];  // diagonal block, column-major (workgroup 0)
+  __shared__ double xs[kTrsRows];          // source block's solution
+  __shared__ double part[2][kTrsRows];     // the two half-row partial sums
+  const int nb = (n + kTrsRows - 1) / kTrsRows;
+  const int t = threadIdx.x;
+  const int rl = t & (kTrsRows - 1), h = t >> 7;  // row in the block, column half
+  const int rb = UPPER ? k - (int)blockIdx.x : k + (int)blockIdx.x;
+  const int R0 = rb * kTrsRows;
+  const int r = R0 + rl;
+  const bool diag = blockIdx.x == 0;
+  const int cn = min(kTrsRows, n - R0);  // rows in this block (= columns of the diagonal)
+  if (diag) {
+    // thread t fills row t & 127 of columns 2m + (t >> 7), m = 0..63, 32 loads
+    // in flight at a time
+    const bool rin = rl < cn;
+    const double* a = A + (R0 + (rin ? rl : 0)) + (int64_t)R0 * lda;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      double v[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        const int ic = 2 * (half * 32 + q) + h;
+        v[q] = (rin && ic < cn) ? a[(int64_t)ic * lda] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 32; ++q) D[(2 * (half * 32 + q) + h) * kTrsLd + rl] = v[q];
+    }
+  }
+  double y = (h == 0 && r < n) ? x[r] : 0.0;
+  const int src = UPPER ? k + 1 : k - 1;
+  if (src >= 0 && src < nb) {
+    const int c0 = src * kTrsRows, sn = min(kTrsRows, n - c0);
+    if (t < sn) xs[t] = x[c0 + t];
+    __syncthreads();
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    if (r < n) {
+      const int j0 = h * 64;
+      const double* a = A + r + (int64_t)(c0 + j0) * lda;
+      if (sn == kTrsRows) {  // a full source block: all 64 loads in flight
+        double v[64];
+#pragma unroll
+        for (int q = 0; q < 64; ++q) v[q] = a[(int64_t)q * lda];
+#pragma unroll
+        for (int q = 0; q < 64; q += 4) {
+          p0 = fma(v[q], xs[j0 + q], p0);
+          p1 = fma(v[q + 1], xs[j0 + q + 1], p1);
+          p2 = fma(v[q + 2], xs[j0 + q + 2], p2);
+          p3 = fma(v[q + 3], xs[j0 + q + 3], p3);
+        }
+      } else {
+        const int m = min(sn, j0 + 64) - j0;
+        for (int q = 0; q < m; ++q) p0 = fma(a[(int64_t)q * lda], xs[j0 + q], p0);
+      }
+    }
+    part[h][rl] = (p0 + p1) + (p2 + p3);
+    __syncthreads();
+    if (h == 0) y -= part[0][rl] + part[1][rl];
+  }
+  if (!diag) {
+    if (h == 0 && r < n) x[r] = y;
+    return;
+  }
+  __syncthreads();  // D complete
+  // the diagonal block: rows 0..63 are wave 0, rows 64..127 wave 1
+  const int w = t >> 6, lane = t & 63;
+  const int nsub = (cn + 63) >> 6;
+  for (int q = 0; q < nsub; ++q) {
+    const int s = UPPER ? nsub - 1 - q : q;
+    const int s0 = s * 64, sn = min(64, cn - s0);
+    if (w == s) {
+      const double* d = D + s0 * kTrsLd + s0 + lane;  // column j at d[j * kTrsLd]
+      double T[64];
+#pragma unroll
+      for (int j = 0; j < 64; ++j) T[j] = d[j * kTrsLd];
+      if (!UPPER) {
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+          if (j < sn) {
+            const double xj = trs_bcast(y, j);
+            if (lane > j) y = fma(-T[j], xj, y);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 63; j >= 0; --j) {
+          if (j < sn) {
+            if (lane == j) y = y / T[j];
+            const double xj = trs_bcast(y, j);
+            if (lane < j) y = fma(-T[j], xj, y);
+          }
+        }
+      }
+      if (lane < sn) xs[s0 + lane] = y;
+    }
+    __syncthreads();
+    const bool later = t < cn && (UPPER ? (w < s) : (w > s));
+    if (later) {
+      const double* d = D + s0 * kTrsLd + t;
+      double p0 = 0.0, p1 = 0.0;
+      int j = 0;
+      for (; j + 2 <= sn; j += 2) {
+        p0 = fma(d[j * kTrsLd], xs[s0 + j], p0);
+        p1 = fma(d[(j + 1) * kTrsLd], xs[s0 + j + 1], p1);
+      }
+      if (j < sn) p0 = fma(d[j * kTrsLd], xs[s0 + j], p0);
+      y -= p0 + p1;
+    }
+  }
+  if (t < cn) x[r] = y;
+}
+
+// x = L^{-1} x, then x = U^{-1} x, with the packed LU factors of dA.
+void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
+  const int nb = (n + kTrsRows - 1) / kTrsRows;
+  hipStream_t s = c->stream;
+  // (the first step of each sweep has no source block: only the diagonal)
+  for (int k = 0; k < nb; ++k)
+    hipLaunchKernelGGL(trs_step<false>, dim3(k == 0 ? 1 : nb - k), dim3(256), 0, s, dA, lda, n,
+                       k, dx);
+  for (int k = nb - 1; k >= 0; --k)
+    hipLaunchKernelGGL(trs_step<true>, dim3(k == nb - 1 ? 1 : k + 1), dim3(256), 0, s, dA, lda,
+                       n, k, dx);
+  PSX_HIP(hipGetLastError());
+}
+
 // dA (n x n, lda) is overwritten by its LU factors; dB (n x nrhs) by X.
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                   int nrhs) {
@@ -356,9 +509,15 @@ void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ld
     PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
                 "LU: zero pivot at position " + std::to_string(hinfo) +
                     " (Julia SingularException)");
-    if (nrhs > 0)
+    if (nrhs > 0 && nrhs <= kTrsMaxRhs) {
+      // P b by the LAPACK interchanges, then the two substitutions per column
+      PSX_ROC(rocsolver_dlaswp(h, nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
+                               1));
+      for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
+    } else if (nrhs > 0) {
       PSX_ROC(rocsolver_dgetrs(h, rocblas_operation_none, n, nrhs, dA, (rocblas_int)lda,
                                (const rocblas_int*)ipiv, dB, (rocblas_int)ldb));
+    }
     PSX_HIP(hipStreamSynchronize(c->stream));
   } catch (...) {
     (void)hipStreamSynchronize(c->stream);

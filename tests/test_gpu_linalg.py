@@ -134,3 +134,55 @@ def test_lu_solve_device_buffers(ctx):
         ctx.free(dA)
         ctx.free(dB)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 255, 256, 257, 320, 511, 777, 2100])
+def test_lu_solve_substitution_sizes(ctx, n):
+    """Solves with 1-4 right-hand sides run the library's blocked forward and
+    back substitution (256-row steps, 64-row diagonal sub-blocks); 5 go
+    through rocSOLVER's getrs.  Both against LAPACK (scipy lu_solve)."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n)) + np.diag(rng.uniform(1.0, 2.0, n)) * np.sqrt(n)
+    ref_lu = sl.lu_factor(A)
+    for nrhs in (1, 2, 4, 5):
+        B = rng.standard_normal((n, nrhs))
+        X = ctx.lu_solve(A, B)
+        R = sl.lu_solve(ref_lu, B)
+        assert np.linalg.norm(X - R) / np.linalg.norm(R) < 1e-12, (n, nrhs)
+
+
+def test_lu_solve_substitution_padded_lda(ctx):
+    """psx_lu_solve_d with lda and ldb larger than n, and pivoting across
+    blocks.  The matrix has cond ~ 6e6 (numpy and scipy solves differ by
+    5e-12), so the check is the normwise backward error
+    |A x - b| / (|A| |x| + |b|), which a stable LU solve keeps near machine
+    epsilon whatever the conditioning, and the forward error against scipy
+    within cond x 1e-15."""
+    import scipy.linalg as sl
+    n, lda, ldb = 600, 613, 640
+    rng = np.random.default_rng(8)
+    A = rng.uniform(0.0, 1.0, (n, n))
+    A[n - 64:, :] += 10.0 * np.arange(1, 65)[:, None]
+    b = rng.standard_normal((n, 2))
+    Ap = np.zeros((lda, n), order="F")
+    Ap[:n] = A
+    bp = np.full((ldb, 2), 7.0, order="F")
+    bp[:n] = b
+    dA, dB = ctx.alloc(8 * lda * n), ctx.alloc(8 * ldb * 2)
+    try:
+        ctx.h2d(dA, Ap)
+        ctx.h2d(dB, bp)
+        ctx.check(ctx.lib.psx_lu_solve_d(ctx.handle, n, dA, lda, dB, ldb, 2), "lu_solve_d")
+        got = np.empty((ldb, 2), order="F")
+        ctx.d2h(got, dB)
+    finally:
+        ctx.free(dA)
+        ctx.free(dB)
+    x = got[:n]
+    eta = np.linalg.norm(A @ x - b) / (np.linalg.norm(A, 2) * np.linalg.norm(x)
+                                        + np.linalg.norm(b))
+    assert eta < 1e-15
+    R = sl.lu_solve(sl.lu_factor(A), b)
+    assert np.linalg.norm(x - R) / np.linalg.norm(R) < np.linalg.cond(A) * 1e-15
+    assert (got[n:] == 7.0).all()
