@@ -11,13 +11,16 @@ namespace psx {
 
 void* ctx_alloc(Ctx* c, size_t bytes) {
   if (bytes == 0) bytes = 8;
-  if (c->in_use + bytes > c->budget && c->xfer && !c->xfer_busy) {
-    // the idle transfer buffer is a cache: give it back before failing
-    PSX_HIP(hipStreamSynchronize(c->stream));
-    void* p = c->xfer;
-    c->xfer = nullptr;
-    c->xfer_bytes = 0;
-    ctx_free(c, p);
+  // the idle cached buffers are caches: give them back before failing
+  for (CachedSlot* sl : {&c->xfer, &c->work}) {
+    if (c->in_use + bytes <= c->budget) break;
+    if (sl->p && !sl->busy) {
+      PSX_HIP(hipStreamSynchronize(c->stream));
+      void* p = sl->p;
+      sl->p = nullptr;
+      sl->bytes = 0;
+      ctx_free(c, p);
+    }
   }
   PSX_REQUIRE(c->in_use + bytes <= c->budget, PSX_ERR_BUDGET,
               "allocation of " + std::to_string(bytes) +
@@ -53,30 +56,30 @@ void* ctx_scratch(Ctx* c, size_t bytes) {
   return c->scratch;
 }
 
-XferBuf::XferBuf(Ctx* cc, size_t bytes) : c(cc) {
-  PSX_REQUIRE(!c->xfer_busy, PSX_ERR_ARGUMENT, "transfer buffer already in use");
-  if (bytes > c->xfer_bytes) {
-    if (c->xfer) {
+HeldBuf::HeldBuf(Ctx* cc, CachedSlot& s, size_t bytes) : c(cc), slot(&s) {
+  PSX_REQUIRE(!s.busy, PSX_ERR_ARGUMENT, "cached device buffer already in use");
+  if (bytes > s.bytes) {
+    if (s.p) {
       PSX_HIP(hipStreamSynchronize(c->stream));
-      void* p = c->xfer;
-      c->xfer = nullptr;
-      c->xfer_bytes = 0;
+      void* p = s.p;
+      s.p = nullptr;
+      s.bytes = 0;
       ctx_free(c, p);
     }
-    c->xfer = ctx_alloc(c, bytes);
-    c->xfer_bytes = bytes;
+    s.p = ctx_alloc(c, bytes);
+    s.bytes = bytes;
   }
-  c->xfer_busy = true;
+  s.busy = true;
   cap = bytes;
 }
 
-XferBuf::~XferBuf() { c->xfer_busy = false; }
+HeldBuf::~HeldBuf() { slot->busy = false; }
 
-double* XferBuf::take(size_t bytes) {
+double* HeldBuf::take(size_t bytes) {
   const size_t at = (used + 255) & ~size_t(255);
-  PSX_REQUIRE(at + bytes <= cap, PSX_ERR_ARGUMENT, "transfer buffer carve overflow");
+  PSX_REQUIRE(at + bytes <= cap, PSX_ERR_ARGUMENT, "cached buffer carve overflow");
   used = at + bytes;
-  return reinterpret_cast<double*>(static_cast<char*>(c->xfer) + at);
+  return reinterpret_cast<double*>(static_cast<char*>(slot->p) + at);
 }
 
 TimedLaunch::TimedLaunch(Ctx* ctx, const char* kname) : c(ctx), name(kname) {

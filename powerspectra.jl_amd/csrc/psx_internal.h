@@ -60,6 +60,13 @@ struct KernelTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
 
+// A device buffer the context keeps between calls; busy while held.
+struct CachedSlot {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool busy = false;
+};
+
 struct Ctx {
   int device = 0;
   size_t budget = 0;
@@ -77,12 +84,10 @@ struct Ctx {
   size_t scratch_bytes = 0;
   // rocSOLVER/rocBLAS handle (created lazily)
   void* blas = nullptr;
-  // device copies of a host-pointer call's inputs and outputs, kept between
-  // calls (grown on demand, given back when an allocation would otherwise
-  // exceed the budget); busy while a call holds it
-  void* xfer = nullptr;
-  size_t xfer_bytes = 0;
-  bool xfer_busy = false;
+  // two device buffers kept between calls (grown on demand, given back when
+  // an allocation would otherwise exceed the budget): the device copies of a
+  // host-pointer call's inputs and outputs, and the work area of a transform
+  CachedSlot xfer, work;
   // two pinned host staging buffers for large host <-> device copies, and the
   // stream that copies device results out while later kernels still run
   void* stage[2] = {nullptr, nullptr};
@@ -98,17 +103,28 @@ void resolve_timers(Ctx* c);
 void* ctx_alloc(Ctx* c, size_t bytes);
 void ctx_free(Ctx* c, void* p);
 void* ctx_scratch(Ctx* c, size_t bytes);
-// The context's transfer buffer held for one host-pointer call; take() carves
-// 256-byte aligned pieces, and the sizes passed to the constructor must cover
-// them (xfer_bytes(...) sums them with their padding).
-struct XferBuf {
+// One of the context's cached buffers held for the duration of a call;
+// take() carves 256-byte aligned pieces, and the size passed to the
+// constructor must cover them (xfer_bytes(...) sums pieces with their
+// padding).  Holding a slot that is already held raises.
+struct HeldBuf {
   Ctx* c;
+  CachedSlot* slot;
   size_t cap = 0, used = 0;
-  XferBuf(Ctx* cc, size_t bytes);
-  ~XferBuf();
-  XferBuf(const XferBuf&) = delete;
-  XferBuf& operator=(const XferBuf&) = delete;
+  HeldBuf(Ctx* cc, CachedSlot& s, size_t bytes);
+  ~HeldBuf();
+  HeldBuf(const HeldBuf&) = delete;
+  HeldBuf& operator=(const HeldBuf&) = delete;
   double* take(size_t bytes);
+  void* base() const { return slot->p; }
+};
+// the transfer buffer of a host-pointer call
+struct XferBuf : HeldBuf {
+  XferBuf(Ctx* cc, size_t bytes) : HeldBuf(cc, cc->xfer, bytes) {}
+};
+// the work area of a transform (spherical-harmonic passes, cross-spectra)
+struct WorkBuf : HeldBuf {
+  WorkBuf(Ctx* cc, size_t bytes) : HeldBuf(cc, cc->work, bytes) {}
 };
 inline size_t xfer_bytes(std::initializer_list<size_t> pieces) {
   size_t t = 0;

@@ -1360,46 +1360,32 @@ void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double
                  double* dalms) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
-  void* mem = ctx_alloc(c, sht_work_bytes(P));
-  try {
-    const ShtBufs B = carve(P, mem);
-    const int64_t npix = P.R.npix;
-    for (int i = 0; i < nmaps; ++i) {
-      const double* map = dmaps + i * npix;
-      double2* alm = (double2*)dalms + i * P.nalm;
-      analysis(c, P, B, map, alm, false);
-      for (int it = 0; it < niter; ++it) {
-        synthesis(c, P, B, alm, B.syn);
-        hipLaunchKernelGGL(residual, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
-                           c->stream, map, B.syn, npix, B.res);
-        PSX_HIP(hipGetLastError());
-        analysis(c, P, B, B.res, alm, true);
-      }
+  WorkBuf wk(c, sht_work_bytes(P));
+  const ShtBufs B = carve(P, wk.base());
+  const int64_t npix = P.R.npix;
+  for (int i = 0; i < nmaps; ++i) {
+    const double* map = dmaps + i * npix;
+    double2* alm = (double2*)dalms + i * P.nalm;
+    analysis(c, P, B, map, alm, false);
+    for (int it = 0; it < niter; ++it) {
+      synthesis(c, P, B, alm, B.syn);
+      hipLaunchKernelGGL(residual, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+                         c->stream, map, B.syn, npix, B.res);
+      PSX_HIP(hipGetLastError());
+      analysis(c, P, B, B.res, alm, true);
     }
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, mem);
-    throw;
   }
-  ctx_free(c, mem);
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
-  void* mem = ctx_alloc(c, sht_work_bytes(P));
-  try {
-    const ShtBufs B = carve(P, mem);
-    for (int i = 0; i < nmaps; ++i)
-      synthesis(c, P, B, (const double2*)dalms + i * P.nalm, dmaps + i * P.R.npix);
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, mem);
-    throw;
-  }
-  ctx_free(c, mem);
+  WorkBuf wk(c, sht_work_bytes(P));
+  const ShtBufs B = carve(P, wk.base());
+  for (int i = 0; i < nmaps; ++i)
+    synthesis(c, P, B, (const double2*)dalms + i * P.nalm, dmaps + i * P.R.npix);
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 // Polarised map2alm: per map (I, Q, U) contiguous, npix each, into (T, E, B)
@@ -1412,31 +1398,24 @@ void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const do
   const int64_t npix = P.R.npix, nalm = P.nalm;
   for (int i = 0; i < nmaps; ++i)
     map2alm_dev(c, nside, lmax, niter, 1, diqu + 3 * i * npix, dteb + 6 * i * nalm);
-  void* mem = ctx_alloc(c, pol_work_bytes(P));
-  try {
-    const PolBufs B = carve_pol(P, mem);
-    for (int i = 0; i < nmaps; ++i) {
-      const double* Q = diqu + (3 * i + 1) * npix;
-      const double* U = Q + npix;
-      double2* aE = (double2*)dteb + (3 * i + 1) * nalm;
-      double2* aB = aE + nalm;
-      analysis_pol(c, P, B, Q, U, aE, aB, false);
-      for (int it = 0; it < niter; ++it) {
-        synthesis_pol(c, P, B, aE, aB, B.syn, B.syn + npix);
-        // Q and U are adjacent in the input, as are the two syntheses
-        hipLaunchKernelGGL(residual, dim3((unsigned)((2 * npix + 255) / 256)), dim3(256), 0,
-                           c->stream, Q, (const double*)B.syn, 2 * npix, B.res);
-        PSX_HIP(hipGetLastError());
-        analysis_pol(c, P, B, B.res, B.res + npix, aE, aB, true);
-      }
+  WorkBuf wk(c, pol_work_bytes(P));
+  const PolBufs B = carve_pol(P, wk.base());
+  for (int i = 0; i < nmaps; ++i) {
+    const double* Q = diqu + (3 * i + 1) * npix;
+    const double* U = Q + npix;
+    double2* aE = (double2*)dteb + (3 * i + 1) * nalm;
+    double2* aB = aE + nalm;
+    analysis_pol(c, P, B, Q, U, aE, aB, false);
+    for (int it = 0; it < niter; ++it) {
+      synthesis_pol(c, P, B, aE, aB, B.syn, B.syn + npix);
+      // Q and U are adjacent in the input, as are the two syntheses
+      hipLaunchKernelGGL(residual, dim3((unsigned)((2 * npix + 255) / 256)), dim3(256), 0,
+                         c->stream, Q, (const double*)B.syn, 2 * npix, B.res);
+      PSX_HIP(hipGetLastError());
+      analysis_pol(c, P, B, B.res, B.res + npix, aE, aB, true);
     }
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, mem);
-    throw;
   }
-  ctx_free(c, mem);
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb, double* diqu) {
@@ -1445,21 +1424,14 @@ void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb,
   const int64_t npix = P.R.npix, nalm = P.nalm;
   for (int i = 0; i < nmaps; ++i)
     alm2map_dev(c, nside, lmax, 1, dteb + 6 * i * nalm, diqu + 3 * i * npix);
-  void* mem = ctx_alloc(c, pol_work_bytes(P));
-  try {
-    const PolBufs B = carve_pol(P, mem);
-    for (int i = 0; i < nmaps; ++i) {
-      const double2* aE = (const double2*)dteb + (3 * i + 1) * nalm;
-      double* Q = diqu + (3 * i + 1) * npix;
-      synthesis_pol(c, P, B, aE, aE + nalm, Q, Q + npix);
-    }
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, mem);
-    throw;
+  WorkBuf wk(c, pol_work_bytes(P));
+  const PolBufs B = carve_pol(P, wk.base());
+  for (int i = 0; i < nmaps; ++i) {
+    const double2* aE = (const double2*)dteb + (3 * i + 1) * nalm;
+    double* Q = diqu + (3 * i + 1) * npix;
+    synthesis_pol(c, P, B, aE, aE + nalm, Q, Q + npix);
   }
-  ctx_free(c, mem);
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 constexpr int kMaxSpectraMaps = 8;
@@ -1473,8 +1445,9 @@ void cross_spectra_dev(Ctx* c, int lmax, int nmaps, const double* dalms, double*
   const int np = nmaps * (nmaps + 1) / 2;
   const int nchunk = std::min(64, lmax + 1);
   const int mchunk = (lmax + 1 + nchunk - 1) / nchunk;
-  double* part = (double*)ctx_alloc(c, sizeof(double) * (size_t)nchunk * np * (lmax + 1));
-  try {
+  WorkBuf wk(c, sizeof(double) * (size_t)nchunk * np * (lmax + 1));
+  double* part = (double*)wk.base();
+  {
     const dim3 grid((unsigned)((lmax + 1 + 255) / 256), (unsigned)nchunk);
     const double2* a = (const double2*)dalms;
     hipStream_t s = c->stream;
@@ -1492,12 +1465,7 @@ void cross_spectra_dev(Ctx* c, int lmax, int nmaps, const double* dalms, double*
                        dim3(256), 0, s, (const double*)part, nchunk, nmaps, lmax, dcl);
     PSX_HIP(hipGetLastError());
     PSX_HIP(hipStreamSynchronize(s));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, part);
-    throw;
   }
-  ctx_free(c, part);
 }
 
 }  // namespace psx
@@ -1515,20 +1483,13 @@ int psx_map2alm(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const dou
     PSX_HIP(hipSetDevice(c->device));
     const int64_t npix = 12LL * nside * nside;
     const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
-    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
-    double* da = nullptr;
-    try {
-      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
-      h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
-      map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
-      d2h_matrix(c, alms, 2 * nalm, da, 2 * nalm, 2 * nalm, nmaps);
-    } catch (...) {
-      ctx_free(c, dm);
-      if (da) ctx_free(c, da);
-      throw;
-    }
-    ctx_free(c, dm);
-    ctx_free(c, da);
+    const size_t mb = sizeof(double) * npix * nmaps, ab = 2 * sizeof(double) * nalm * nmaps;
+    XferBuf x(c, xfer_bytes({mb, ab}));
+    double* dm = x.take(mb);
+    double* da = x.take(ab);
+    h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
+    map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
+    d2h_matrix(c, alms, 2 * nalm, da, 2 * nalm, 2 * nalm, nmaps);
   });
 }
 
@@ -1540,20 +1501,13 @@ int psx_alm2map(psx_ctx* h, int nside, int lmax, int nmaps, const double* alms, 
     PSX_HIP(hipSetDevice(c->device));
     const int64_t npix = 12LL * nside * nside;
     const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
-    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
-    double* da = nullptr;
-    try {
-      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
-      h2d_matrix(c, da, 2 * nalm, alms, 2 * nalm, 2 * nalm, nmaps);
-      alm2map_dev(c, nside, lmax, nmaps, da, dm);
-      d2h_matrix(c, maps, npix, dm, npix, npix, nmaps);
-    } catch (...) {
-      ctx_free(c, dm);
-      if (da) ctx_free(c, da);
-      throw;
-    }
-    ctx_free(c, dm);
-    ctx_free(c, da);
+    const size_t mb = sizeof(double) * npix * nmaps, ab = 2 * sizeof(double) * nalm * nmaps;
+    XferBuf x(c, xfer_bytes({mb, ab}));
+    double* dm = x.take(mb);
+    double* da = x.take(ab);
+    h2d_matrix(c, da, 2 * nalm, alms, 2 * nalm, 2 * nalm, nmaps);
+    alm2map_dev(c, nside, lmax, nmaps, da, dm);
+    d2h_matrix(c, maps, npix, dm, npix, npix, nmaps);
   });
 }
 
@@ -1569,25 +1523,16 @@ int psx_mask_spectra(psx_ctx* h, int nside, int lmax, int niter, int nmaps, cons
     const int64_t npix = 12LL * nside * nside;
     const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
     const int64_t ncl = (int64_t)nmaps * nmaps * (lmax + 1);
-    double* dm = (double*)ctx_alloc(c, sizeof(double) * npix * nmaps);
-    double* da = nullptr;
-    double* dc = nullptr;
-    try {
-      da = (double*)ctx_alloc(c, 2 * sizeof(double) * nalm * nmaps);
-      dc = (double*)ctx_alloc(c, sizeof(double) * ncl);
-      h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
-      map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
-      cross_spectra_dev(c, lmax, nmaps, da, dc);
-      d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
-    } catch (...) {
-      ctx_free(c, dm);
-      if (da) ctx_free(c, da);
-      if (dc) ctx_free(c, dc);
-      throw;
-    }
-    ctx_free(c, dm);
-    ctx_free(c, da);
-    ctx_free(c, dc);
+    const size_t mb = sizeof(double) * npix * nmaps, ab = 2 * sizeof(double) * nalm * nmaps,
+                 cb = sizeof(double) * ncl;
+    XferBuf x(c, xfer_bytes({mb, ab, cb}));
+    double* dm = x.take(mb);
+    double* da = x.take(ab);
+    double* dc = x.take(cb);
+    h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
+    map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
+    cross_spectra_dev(c, lmax, nmaps, da, dc);
+    d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
   });
 }
 
@@ -1600,20 +1545,13 @@ int psx_map2alm_pol(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const
     PSX_HIP(hipSetDevice(c->device));
     const int64_t npix = 12LL * nside * nside;
     const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
-    double* dm = (double*)ctx_alloc(c, 3 * sizeof(double) * npix * nmaps);
-    double* da = nullptr;
-    try {
-      da = (double*)ctx_alloc(c, 6 * sizeof(double) * nalm * nmaps);
-      h2d_matrix(c, dm, npix, iqu, npix, npix, 3 * nmaps);
-      map2alm_pol_dev(c, nside, lmax, niter, nmaps, dm, da);
-      d2h_matrix(c, teb, 2 * nalm, da, 2 * nalm, 2 * nalm, 3 * nmaps);
-    } catch (...) {
-      ctx_free(c, dm);
-      if (da) ctx_free(c, da);
-      throw;
-    }
-    ctx_free(c, dm);
-    ctx_free(c, da);
+    const size_t mb = 3 * sizeof(double) * npix * nmaps, ab = 6 * sizeof(double) * nalm * nmaps;
+    XferBuf x(c, xfer_bytes({mb, ab}));
+    double* dm = x.take(mb);
+    double* da = x.take(ab);
+    h2d_matrix(c, dm, npix, iqu, npix, npix, 3 * nmaps);
+    map2alm_pol_dev(c, nside, lmax, niter, nmaps, dm, da);
+    d2h_matrix(c, teb, 2 * nalm, da, 2 * nalm, 2 * nalm, 3 * nmaps);
   });
 }
 
@@ -1625,20 +1563,13 @@ int psx_alm2map_pol(psx_ctx* h, int nside, int lmax, int nmaps, const double* te
     PSX_HIP(hipSetDevice(c->device));
     const int64_t npix = 12LL * nside * nside;
     const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
-    double* dm = (double*)ctx_alloc(c, 3 * sizeof(double) * npix * nmaps);
-    double* da = nullptr;
-    try {
-      da = (double*)ctx_alloc(c, 6 * sizeof(double) * nalm * nmaps);
-      h2d_matrix(c, da, 2 * nalm, teb, 2 * nalm, 2 * nalm, 3 * nmaps);
-      alm2map_pol_dev(c, nside, lmax, nmaps, da, dm);
-      d2h_matrix(c, iqu, npix, dm, npix, npix, 3 * nmaps);
-    } catch (...) {
-      ctx_free(c, dm);
-      if (da) ctx_free(c, da);
-      throw;
-    }
-    ctx_free(c, dm);
-    ctx_free(c, da);
+    const size_t mb = 3 * sizeof(double) * npix * nmaps, ab = 6 * sizeof(double) * nalm * nmaps;
+    XferBuf x(c, xfer_bytes({mb, ab}));
+    double* dm = x.take(mb);
+    double* da = x.take(ab);
+    h2d_matrix(c, da, 2 * nalm, teb, 2 * nalm, 2 * nalm, 3 * nmaps);
+    alm2map_pol_dev(c, nside, lmax, nmaps, da, dm);
+    d2h_matrix(c, iqu, npix, dm, npix, npix, 3 * nmaps);
   });
 }
 
