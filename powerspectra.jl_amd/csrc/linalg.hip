@@ -537,23 +537,15 @@ static void transpose(Ctx* c, int n, const double* dIn, double* dOut) {
 void decouple_dev(Ctx* c, int n, const double* dY, double* dB1, double* dB2,
                   double* dOut) {
   const size_t nn = (size_t)n * n;
-  double* tmp = (double*)ctx_alloc(c, sizeof(double) * nn);
-  try {
-    PSX_HIP(hipMemcpyAsync(tmp, dY, sizeof(double) * nn, hipMemcpyDeviceToDevice,
-                           c->stream));
-    lu_solve_dev(c, n, dB1, n, tmp, n, n);  // tmp = B1^{-1} Y
-    transpose(c, n, tmp, dOut);             // dOut = tmp^T
-    lu_solve_dev(c, n, dB2, n, dOut, n, n); // dOut = B2^{-1} tmp^T
-    transpose(c, n, dOut, tmp);             // tmp = (B2^{-1} tmp^T)^T
-    PSX_HIP(hipMemcpyAsync(dOut, tmp, sizeof(double) * nn, hipMemcpyDeviceToDevice,
-                           c->stream));
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, tmp);
-    throw;
-  }
-  ctx_free(c, tmp);
+  WorkBuf wk(c, sizeof(double) * nn);
+  double* tmp = (double*)wk.base();
+  PSX_HIP(hipMemcpyAsync(tmp, dY, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
+  lu_solve_dev(c, n, dB1, n, tmp, n, n);  // tmp = B1^{-1} Y
+  transpose(c, n, tmp, dOut);             // dOut = tmp^T
+  lu_solve_dev(c, n, dB2, n, dOut, n, n); // dOut = B2^{-1} tmp^T
+  transpose(c, n, dOut, tmp);             // tmp = (B2^{-1} tmp^T)^T
+  PSX_HIP(hipMemcpyAsync(dOut, tmp, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace psx
@@ -570,23 +562,16 @@ int psx_lu_solve(psx_ctx* hctx, int n, const double* A, int64_t lda, double* B,
                 "bad sizes for lu_solve");
     PSX_REQUIRE(A && (B || nrhs == 0), PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
-    const size_t nn = (size_t)n * n, nb = (size_t)n * std::max(nrhs, 1);
-    double* dA = (double*)ctx_alloc(c, sizeof(double) * nn);
-    double* dB = (double*)ctx_alloc(c, sizeof(double) * nb);
-    try {
-      h2d_matrix(c, dA, n, A, lda, n, n);
-      if (nrhs > 0) h2d_matrix(c, dB, n, B, ldb, n, nrhs);
-      lu_solve_dev(c, n, dA, n, dB, n, nrhs);
-      if (nrhs > 0) d2h_matrix(c, B, ldb, dB, n, n, nrhs);
-      PSX_HIP(hipStreamSynchronize(c->stream));
-    } catch (...) {
-      (void)hipStreamSynchronize(c->stream);
-      ctx_free(c, dA);
-      ctx_free(c, dB);
-      throw;
-    }
-    ctx_free(c, dA);
-    ctx_free(c, dB);
+    const size_t ab = sizeof(double) * n * (size_t)n,
+                 bb = sizeof(double) * n * (size_t)std::max(nrhs, 1);
+    XferBuf x(c, xfer_bytes({ab, bb}));
+    double* dA = x.take(ab);
+    double* dB = x.take(bb);
+    h2d_matrix(c, dA, n, A, lda, n, n);
+    if (nrhs > 0) h2d_matrix(c, dB, n, B, ldb, n, nrhs);
+    lu_solve_dev(c, n, dA, n, dB, n, nrhs);
+    if (nrhs > 0) d2h_matrix(c, B, ldb, dB, n, n, nrhs);
+    PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -609,29 +594,20 @@ int psx_lu_factor(psx_ctx* hctx, int n, const double* A, int64_t lda, double* LU
     PSX_REQUIRE(n >= 1 && lda >= n && ldlu >= n, PSX_ERR_ARGUMENT, "bad sizes for lu_factor");
     PSX_REQUIRE(A && LU && ipiv, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
-    const size_t nn = (size_t)n * n;
-    double* dA = (double*)ctx_alloc(c, sizeof(double) * nn);
-    int* dp = nullptr;
-    try {
-      dp = (int*)ctx_alloc(c, sizeof(int) * ((size_t)n + 64));
-      h2d_matrix(c, dA, n, A, lda, n, n);
-      lu_factor_dev(c, n, dA, n, dp, dp + n + 32);
-      int hinfo = 0;
-      PSX_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
-      PSX_HIP(hipMemcpyAsync(&hinfo, dp + n + 32, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      d2h_matrix(c, LU, ldlu, dA, n, n, n);
-      PSX_HIP(hipStreamSynchronize(c->stream));
-      PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
-                  "LU: zero pivot at position " + std::to_string(hinfo) +
-                      " (Julia SingularException)");
-    } catch (...) {
-      (void)hipStreamSynchronize(c->stream);
-      ctx_free(c, dA);
-      if (dp) ctx_free(c, dp);
-      throw;
-    }
-    ctx_free(c, dA);
-    ctx_free(c, dp);
+    const size_t ab = sizeof(double) * n * (size_t)n, pb = sizeof(int) * ((size_t)n + 64);
+    XferBuf x(c, xfer_bytes({ab, pb}));
+    double* dA = x.take(ab);
+    int* dp = reinterpret_cast<int*>(x.take(pb));
+    h2d_matrix(c, dA, n, A, lda, n, n);
+    lu_factor_dev(c, n, dA, n, dp, dp + n + 32);
+    int hinfo = 0;
+    PSX_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipMemcpyAsync(&hinfo, dp + n + 32, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    d2h_matrix(c, LU, ldlu, dA, n, n, n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
+                "LU: zero pivot at position " + std::to_string(hinfo) +
+                    " (Julia SingularException)");
   });
 }
 
@@ -643,20 +619,14 @@ int psx_decouple_covmat(psx_ctx* hctx, int n, const double* Y, const double* B1,
     PSX_REQUIRE(Y && B1 && B2 && out, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
     const size_t nn = (size_t)n * n, b = sizeof(double) * nn;
-    double* d = (double*)ctx_alloc(c, 4 * b);
-    try {
-      h2d_matrix(c, d, n, Y, n, n, n);
-      h2d_matrix(c, d + nn, n, B1, n, n, n);
-      h2d_matrix(c, d + 2 * nn, n, B2, n, n, n);
-      decouple_dev(c, n, d, d + nn, d + 2 * nn, d + 3 * nn);
-      d2h_matrix(c, out, n, d + 3 * nn, n, n, n);
-      PSX_HIP(hipStreamSynchronize(c->stream));
-    } catch (...) {
-      (void)hipStreamSynchronize(c->stream);
-      ctx_free(c, d);
-      throw;
-    }
-    ctx_free(c, d);
+    XferBuf x(c, xfer_bytes({4 * b}));
+    double* d = x.take(4 * b);
+    h2d_matrix(c, d, n, Y, n, n, n);
+    h2d_matrix(c, d + nn, n, B1, n, n, n);
+    h2d_matrix(c, d + 2 * nn, n, B2, n, n, n);
+    decouple_dev(c, n, d, d + nn, d + 2 * nn, d + 3 * nn);
+    d2h_matrix(c, out, n, d + 3 * nn, n, n, n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

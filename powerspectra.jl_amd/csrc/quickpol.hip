@@ -285,23 +285,16 @@ int psx_quickpol_xi(psx_ctx* h, int nu1, int nu2, int s1, int s2, int lmax, int 
     PSX_REQUIRE(band && (W || LW == 0), PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
     const int64_t ldb = (int64_t)bl + bu + 1;
-    double* dW = (double*)ctx_alloc(c, sizeof(double) * (LW > 0 ? LW : 1));
-    double* dB = nullptr;
-    try {
-      dB = (double*)ctx_alloc(c, sizeof(double) * ldb * (lmax + 1));
-      if (LW > 0) h2d_matrix(c, dW, LW, W, LW, LW, 1);
-      h2d_matrix(c, dB, ldb, band, ldb, ldb, lmax + 1);
-      quickpol_xi_dev(c, nu1, nu2, s1, s2, lmax, bl, bu, dW, LW, dB);
-      d2h_matrix(c, band, ldb, dB, ldb, ldb, lmax + 1);
-      PSX_HIP(hipStreamSynchronize(c->stream));
-    } catch (...) {
-      (void)hipStreamSynchronize(c->stream);
-      ctx_free(c, dW);
-      if (dB) ctx_free(c, dB);
-      throw;
-    }
-    ctx_free(c, dW);
-    ctx_free(c, dB);
+    const size_t wb = sizeof(double) * (LW > 0 ? LW : 1),
+                 bb = sizeof(double) * ldb * (size_t)(lmax + 1);
+    XferBuf x(c, xfer_bytes({wb, bb}));
+    double* dW = x.take(wb);
+    double* dB = x.take(bb);
+    if (LW > 0) h2d_matrix(c, dW, LW, W, LW, LW, 1);
+    h2d_matrix(c, dB, ldb, band, ldb, ldb, lmax + 1);
+    quickpol_xi_dev(c, nu1, nu2, s1, s2, lmax, bl, bu, dW, LW, dB);
+    d2h_matrix(c, band, ldb, dB, ldb, ldb, lmax + 1);
+    PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }
 
