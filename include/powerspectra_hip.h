@@ -14,12 +14,13 @@
  *   - Spectra (W, signal, noise ratios) are 0-based: W[l] for l = 0 .. LW-1
  *     (SpectralVector, reference src/spectralarray.jl:17).
  *   - Host-pointer entry points copy in and out; no pointer is retained.
- *     The mode-coupling and covariance ones keep their device copies in a
- *     per-context transfer buffer between calls (counted in
- *     psx_ctx_bytes_in_use, given back when a later allocation would
- *     otherwise exceed the budget), and when the outputs exceed 32 MB they
- *     run the contraction in l1 bands, copying the columns each band
- *     completes to the host while the later bands compute.
+ *     They keep their device copies in a per-context transfer buffer
+ *     between calls, and the transforms their work arrays in a per-context
+ *     work buffer (both counted in psx_ctx_bytes_in_use, given back when a
+ *     later allocation would otherwise exceed the budget).  Mode-coupling
+ *     and covariance outputs over 32 MB are computed in l1 bands, the
+ *     columns each band completes copied to the host while the later bands
+ *     compute.
  *     `_d` entry points take device pointers and enqueue on the context's
  *     stream; they return after the work is enqueued (call
  *     psx_ctx_synchronize to wait).
