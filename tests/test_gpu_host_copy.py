@@ -230,3 +230,30 @@ def test_transfer_buffer_counts_and_yields(pkg):
         assert np.array_equal(ctx.mcm(0, 0, lmax, W), M)
     finally:
         ctx.close()
+
+
+def test_work_buffer_yields_to_allocations(pkg):
+    """A transform leaves its device copies in the transfer buffer and its ring
+    arrays in the work buffer.  With a budget that holds exactly what the call
+    left behind, a device allocation larger than the transfer buffer succeeds
+    by taking both back, and the transform then gives the same bits again."""
+    from powerspectra_amd.masks import config_masks
+    nside, lmax, m = 128, 383, 2
+    maps = np.stack(config_masks(nside, 20, (0, 1)))
+    probe = pkg.Context(0)
+    try:
+        base = probe.bytes_in_use()
+        probe.mask_spectra(maps, nside, lmax)
+        held = probe.bytes_in_use() - base
+    finally:
+        probe.close()
+    npix, nalm = 12 * nside * nside, (lmax + 1) * (lmax + 2) // 2
+    xfer = 8 * npix * m + 16 * nalm * m + 8 * m * m * (lmax + 1) + 768
+    ctx = pkg.Context(0, hbm_budget=base + held)
+    try:
+        cl = ctx.mask_spectra(maps, nside, lmax)
+        p = ctx.alloc(xfer + 8 * npix)
+        ctx.free(p)
+        assert np.array_equal(ctx.mask_spectra(maps, nside, lmax), cl)
+    finally:
+        ctx.close()
