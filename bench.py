@@ -238,13 +238,39 @@ def run_cov(args, pkg) -> dict:
         dt, ks = _timed(ctx, f, [kname], args.steps, args.warmup)
         res[kname] = _kernel_roofline(kname, ks[kname], ev, od)
         total += dt / args.steps
+    # decouple_covmat (M^-1 C M^-T) of the TTTT block with the TT mode-coupling
+    # matrix of the same W, host buffers in and out, warm
+    ctx.check(L.psx_cov_d(ctx.handle, 0, 0, lmax, arr(sp), arr(rt), arr(dW[:8]), LW, 0, dC, n),
+              "psx_cov_tttt")
+    Y = np.empty((n, n), order="F")
+    ctx.d2h(Y, dC)
+    ctx.check(L.psx_mcm_d(ctx.handle, 0, 0, lmax, dW[0], LW, dC, n), "psx_mcm_00")
+    M = np.empty((n, n), order="F")
+    ctx.d2h(M, dC)
+    D = ctx.decouple_covmat(Y, M, M)
+    reps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        D = ctx.decouple_covmat(Y, M, M)
+    t_dec = (time.perf_counter() - t0) / reps
+    back = M @ D @ M.T
+    t0 = time.perf_counter()  # the same two LAPACK solves on the host, for scale
+    R = np.linalg.solve(M, np.linalg.solve(M, Y).T).T
+    t_np = time.perf_counter() - t0
+    dec = {"decouple_covmat_s": t_dec, "n": n, "numpy_lapack_s": t_np,
+           "reconstruction_rel_err": float(np.linalg.norm(back - Y) / np.linalg.norm(Y)),
+           "vs_numpy_rel_err": float(np.linalg.norm(D - R) / np.linalg.norm(R)),
+           "numpy_reconstruction_rel_err":
+               float(np.linalg.norm(M @ R @ M.T - Y) / np.linalg.norm(Y)),
+           "cond_M": float(np.linalg.cond(M))}
     for d in dW + sp + rt + [dC]:
         ctx.free(d)
     ctx.close()
     return _line("coupled covariance blocks at lmax %d" % lmax, 7 * n * n / total, "elements/s",
                  args, total * 1e3,
                  {"workload": f"cov_lmax{lmax}: TTTT, EEEE, TTTE, TETE, TEEE, TEEE-Planck, TTEE "
-                              "blocks one at a time", "lmax": lmax, "LW": LW}, kernels=res)
+                              "blocks one at a time", "lmax": lmax, "LW": LW}, kernels=res,
+                 decouple=dec)
 
 
 def run_solve(args, pkg) -> dict:

@@ -357,7 +357,7 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
 // solved value broadcast by v_readlane.
 constexpr int kTrsRows = 128;
 constexpr int kTrsLd = kTrsRows + 1;   // LDS column stride of the diagonal block
-constexpr int kTrsMaxRhs = 4;          // more right-hand sides go through rocSOLVER's getrs
+constexpr int kTrsMaxRhs = 4;          // more right-hand sides take the blocked trsm below
 
 // lane j's value in every lane (j uniform: two v_readlane_b32)
 __device__ inline double trs_bcast(double v, int j) {
@@ -495,6 +495,113 @@ void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
   PSX_HIP(hipGetLastError());
 }
 
+// ---- triangular solves with many right-hand sides ---------------------------
+// Blocked left substitution X = T^{-1} B for the LU factors (T = unit lower L
+// or upper U) and m right-hand sides, in kTrsRows-row blocks: the diagonal
+// block is solved by substitution (trsm_diag, one thread per right-hand side,
+// the block staged in LDS), and the rows still to come are updated by a
+// rocBLAS GEMM.  Substitution keeps LAPACK's backward error on ill-conditioned
+// U, which a trsm built on inverted diagonal blocks does not.
+template <bool UPPER>
+__global__ void __launch_bounds__(256) trsm_diag(const double* __restrict__ A, int64_t lda, int n,
+                                                 int k, double* __restrict__ B, int64_t ldb,
+                                                 int m) {
+  __shared__ double D[kTrsRows * kTrsLd];  // the diagonal block, column-major
+  const int t = threadIdx.x;
+  const int R0 = k * kTrsRows, cn = min(kTrsRows, n - R0);
+  {
+    const int rl = t & (kTrsRows - 1), h = t >> 7;
+    const bool rin = rl < cn;
+    const double* a = A + (R0 + (rin ? rl : 0)) + (int64_t)R0 * lda;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      double v[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        const int ic = 2 * (half * 32 + q) + h;
+        v[q] = (rin && ic < cn) ? a[(int64_t)ic * lda] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 32; ++q) D[(2 * (half * 32 + q) + h) * kTrsLd + rl] = v[q];
+    }
+  }
+  __syncthreads();
+  const int col = (int)blockIdx.x * 256 + t;
+  if (col >= m) return;  // no barrier below
+  double* b = B + (int64_t)col * ldb + R0;
+  // 32-row sub-blocks: the one being solved sits in registers, and each
+  // solved sub-block is applied to the rows of the block still to come
+  constexpr int S = 32;
+  const int nsub = (cn + S - 1) / S;
+  for (int q = 0; q < nsub; ++q) {
+    const int sb = UPPER ? nsub - 1 - q : q;
+    const int s0 = sb * S, sn = min(S, cn - s0);
+    double x[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = i < sn ? b[s0 + i] : 0.0;
+    if (!UPPER) {
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        if (j < sn) {
+#pragma unroll
+          for (int i = j + 1; i < S; ++i)
+            x[i] = fma(-D[(s0 + j) * kTrsLd + s0 + i], x[j], x[i]);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads with its step
+      }
+    } else {
+#pragma unroll
+      for (int j = S - 1; j >= 0; --j) {
+        if (j < sn) {
+          x[j] = x[j] / D[(s0 + j) * kTrsLd + s0 + j];
+#pragma unroll
+          for (int i = 0; i < j; ++i) x[i] = fma(-D[(s0 + j) * kTrsLd + s0 + i], x[j], x[i]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+      if (i < sn) b[s0 + i] = x[i];
+    // rows of this diagonal block not yet solved (below for L, above for U)
+    const int r0 = UPPER ? 0 : s0 + sn, r1 = UPPER ? s0 : cn;
+    for (int i = r0; i < r1; ++i) {
+      double acc = b[i];
+#pragma unroll
+      for (int j = 0; j < S; ++j) acc = fma(-D[(s0 + j) * kTrsLd + i], x[j], acc);
+      b[i] = acc;
+    }
+  }
+}
+
+// B (n x m, ldb) = T^{-1} B with T the unit lower (UPPER false) or upper
+// factor packed in dA.
+static void trsm_left_dev(Ctx* c, bool upper, int n, const double* dA, int64_t lda, double* dB,
+                          int64_t ldb, int m) {
+  rocblas_handle h = blas(c);
+  hipStream_t s = c->stream;
+  const int nb = (n + kTrsRows - 1) / kTrsRows;
+  const unsigned g = (unsigned)((m + 255) / 256);
+  const double one = 1.0, mone = -1.0;
+  for (int q = 0; q < nb; ++q) {
+    const int k = upper ? nb - 1 - q : q;
+    const int R0 = k * kTrsRows, cn = std::min(kTrsRows, n - R0);
+    if (upper)
+      hipLaunchKernelGGL(trsm_diag<true>, dim3(g), dim3(256), 0, s, dA, lda, n, k, dB, ldb, m);
+    else
+      hipLaunchKernelGGL(trsm_diag<false>, dim3(g), dim3(256), 0, s, dA, lda, n, k, dB, ldb, m);
+    PSX_HIP(hipGetLastError());
+    if (!upper && R0 + cn < n)  // rows below: B2 -= L21 X1
+      PSX_ROC(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n - R0 - cn, m, cn,
+                            &mone, dA + (R0 + cn) + (int64_t)R0 * lda, (rocblas_int)lda,
+                            dB + R0, (rocblas_int)ldb, &one, dB + R0 + cn, (rocblas_int)ldb));
+    if (upper && R0 > 0)        // rows above: B0 -= U01 X1
+      PSX_ROC(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R0, m, cn, &mone,
+                            dA + (int64_t)R0 * lda, (rocblas_int)lda, dB + R0, (rocblas_int)ldb,
+                            &one, dB, (rocblas_int)ldb));
+  }
+}
+
 // dA (n x n, lda) is overwritten by its LU factors; dB (n x nrhs) by X.
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                   int nrhs) {
@@ -515,8 +622,10 @@ void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ld
                                1));
       for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
     } else if (nrhs > 0) {
-      PSX_ROC(rocsolver_dgetrs(h, rocblas_operation_none, n, nrhs, dA, (rocblas_int)lda,
-                               (const rocblas_int*)ipiv, dB, (rocblas_int)ldb));
+      PSX_ROC(rocsolver_dlaswp(h, nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
+                               1));
+      trsm_left_dev(c, false, n, dA, lda, dB, ldb, nrhs);
+      trsm_left_dev(c, true, n, dA, lda, dB, ldb, nrhs);
     }
     PSX_HIP(hipStreamSynchronize(c->stream));
   } catch (...) {
