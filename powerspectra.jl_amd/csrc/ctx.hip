@@ -418,12 +418,6 @@ void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2]
   PSX_HIP(hipStreamSynchronize(s));
 }
 
-void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
-                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols) {
-  const D2HMatrix m{host, ldh, dev, ldd, rows, cols};
-  d2h_matrices_on(s, stage, ev, &m, 1);
-}
-
 void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm) {
   for (int i = 0; i < nm; ++i)
     if ((size_t)m[i].rows * (size_t)m[i].cols * sizeof(double) > kDirectBytes || m[i].after)

@@ -150,11 +150,9 @@ void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb,
 // complete.  Large copies are staged through the context's pinned buffers.
 void d2h_matrix(Ctx* c, double* host, int64_t ldh, const double* dev, int64_t ldd,
                 int64_t rows, int64_t cols);
-// the same on stream s through two caller-owned pinned buffers of
-// kStageBytes each and their events
+// size of each of the two pinned staging buffers (the context's, and each
+// server slot's)
 constexpr size_t kStageBytes = size_t(32) << 20;
-void d2h_matrix_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2], double* host,
-                   int64_t ldh, const double* dev, int64_t ldd, int64_t rows, int64_t cols);
 // several matrices through one pipeline (the DMA does not drain between them)
 struct D2HMatrix {
   double* host;
@@ -168,6 +166,8 @@ void d2h_matrices(Ctx* c, const D2HMatrix* m, int nm);
 // the same on the context's copy stream, so that the pieces waiting for
 // events of the context stream copy out while later work there still runs
 void d2h_matrices_overlapped(Ctx* c, const D2HMatrix* m, int nm);
+// the same on stream s through two caller-owned pinned buffers of
+// kStageBytes each and their events
 void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2],
                      const D2HMatrix* m, int nm);
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
