@@ -157,9 +157,10 @@ int psx_cov_d(psx_ctx* ctx, int kind, int lmin, int lmax,
               const double* const* dW, int64_t LW, int precision,
               double* dC, int64_t ldC);
 
-/* ---- FP64 dense linear algebra (rocSOLVER) ----------------------------- */
+/* ---- FP64 dense linear algebra ----------------------------------------- */
 /* decouple_covmat (reference src/covariance.jl:8-14): out = B1^{-1} Y B2^{-T},
- * all n x n column-major with leading dimension n.  Y is not modified. */
+ * all n x n column-major with leading dimension n.  Y is not modified.  When
+ * B1 and B2 are the same pointer the matrix is factored once. */
 int psx_decouple_covmat(psx_ctx* ctx, int n, const double* Y, const double* B1,
                         const double* B2, double* out);
 /* `A \ B` (reference src/blockspectralmatrix.jl:101-129): LU with partial

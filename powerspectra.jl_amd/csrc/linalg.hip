@@ -603,37 +603,55 @@ static void trsm_left_dev(Ctx* c, bool upper, int n, const double* dA, int64_t l
 }
 
 // dA (n x n, lda) is overwritten by its LU factors; dB (n x nrhs) by X.
+// Factor dA in place; ipiv (device, n + 64 ints) receives the LAPACK
+// interchanges.  Raises SingularException on a zero pivot.
+static void lu_factor_checked(Ctx* c, int n, double* dA, int64_t lda, int* ipiv) {
+  int* info = ipiv + n + 32;
+  lu_factor_dev(c, n, dA, lda, ipiv, info);
+  int hinfo = 0;
+  PSX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c->stream));
+  PSX_HIP(hipStreamSynchronize(c->stream));
+  PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
+              "LU: zero pivot at position " + std::to_string(hinfo) +
+                  " (Julia SingularException)");
+}
+
+// B = A^{-1} B from the factors of A in dA and its interchanges ipiv.
+static void lu_getrs_dev(Ctx* c, int n, const double* dA, int64_t lda, const int* ipiv,
+                         double* dB, int64_t ldb, int nrhs) {
+  if (nrhs <= 0) return;
+  // P b by the LAPACK interchanges, then the two substitutions
+  PSX_ROC(rocsolver_dlaswp(blas(c), nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
+                           1));
+  if (nrhs <= kTrsMaxRhs) {
+    for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
+  } else {
+    trsm_left_dev(c, false, n, dA, lda, dB, ldb, nrhs);
+    trsm_left_dev(c, true, n, dA, lda, dB, ldb, nrhs);
+  }
+}
+
+// Device LAPACK interchanges for one factorization, freed on scope exit
+// after the stream has drained.
+struct Pivots {
+  Ctx* c;
+  int* p;
+  Pivots(Ctx* cc, int n) : c(cc), p((int*)ctx_alloc(cc, sizeof(int) * ((size_t)n + 64))) {}
+  ~Pivots() {
+    (void)hipStreamSynchronize(c->stream);
+    try {
+      ctx_free(c, p);
+    } catch (...) {
+    }
+  }
+};
+
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                   int nrhs) {
-  rocblas_handle h = blas(c);
-  int* ipiv = (int*)ctx_alloc(c, sizeof(int) * ((size_t)n + 64));
-  try {
-    int* info = ipiv + n + 32;
-    lu_factor_dev(c, n, dA, lda, ipiv, info);
-    int hinfo = 0;
-    PSX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c->stream));
-    PSX_HIP(hipStreamSynchronize(c->stream));
-    PSX_REQUIRE(hinfo == 0, PSX_ERR_SINGULAR,
-                "LU: zero pivot at position " + std::to_string(hinfo) +
-                    " (Julia SingularException)");
-    if (nrhs > 0 && nrhs <= kTrsMaxRhs) {
-      // P b by the LAPACK interchanges, then the two substitutions per column
-      PSX_ROC(rocsolver_dlaswp(h, nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
-                               1));
-      for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
-    } else if (nrhs > 0) {
-      PSX_ROC(rocsolver_dlaswp(h, nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
-                               1));
-      trsm_left_dev(c, false, n, dA, lda, dB, ldb, nrhs);
-      trsm_left_dev(c, true, n, dA, lda, dB, ldb, nrhs);
-    }
-    PSX_HIP(hipStreamSynchronize(c->stream));
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    ctx_free(c, ipiv);
-    throw;
-  }
-  ctx_free(c, ipiv);
+  Pivots pv(c, n);
+  lu_factor_checked(c, n, dA, lda, pv.p);
+  lu_getrs_dev(c, n, dA, lda, pv.p, dB, ldb, nrhs);
+  PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 static void transpose(Ctx* c, int n, const double* dIn, double* dOut) {
@@ -643,16 +661,20 @@ static void transpose(Ctx* c, int n, const double* dIn, double* dOut) {
 }
 
 // dOut = B1^{-1} Y B2^{-T}; dB1, dB2 are overwritten by their LU factors.
+// dB2 == dB1 (one matrix on both sides) is factored once.
 void decouple_dev(Ctx* c, int n, const double* dY, double* dB1, double* dB2,
                   double* dOut) {
   const size_t nn = (size_t)n * n;
   WorkBuf wk(c, sizeof(double) * nn);
   double* tmp = (double*)wk.base();
   PSX_HIP(hipMemcpyAsync(tmp, dY, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
-  lu_solve_dev(c, n, dB1, n, tmp, n, n);  // tmp = B1^{-1} Y
-  transpose(c, n, tmp, dOut);             // dOut = tmp^T
-  lu_solve_dev(c, n, dB2, n, dOut, n, n); // dOut = B2^{-1} tmp^T
-  transpose(c, n, dOut, tmp);             // tmp = (B2^{-1} tmp^T)^T
+  Pivots pv(c, n);
+  lu_factor_checked(c, n, dB1, n, pv.p);
+  lu_getrs_dev(c, n, dB1, n, pv.p, tmp, n, n);   // tmp = B1^{-1} Y
+  transpose(c, n, tmp, dOut);                     // dOut = tmp^T
+  if (dB2 != dB1) lu_factor_checked(c, n, dB2, n, pv.p);
+  lu_getrs_dev(c, n, dB2, n, pv.p, dOut, n, n);  // dOut = B2^{-1} tmp^T
+  transpose(c, n, dOut, tmp);                     // tmp = (B2^{-1} tmp^T)^T
   PSX_HIP(hipMemcpyAsync(dOut, tmp, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
   PSX_HIP(hipStreamSynchronize(c->stream));
 }
@@ -727,13 +749,16 @@ int psx_decouple_covmat(psx_ctx* hctx, int n, const double* Y, const double* B1,
     PSX_REQUIRE(n >= 1, PSX_ERR_ARGUMENT, "n must be positive");
     PSX_REQUIRE(Y && B1 && B2 && out, PSX_ERR_ARGUMENT, "null pointer");
     PSX_HIP(hipSetDevice(c->device));
+    // one matrix passed for both B1 and B2 is copied and factored once
+    const bool same = B1 == B2;
     const size_t nn = (size_t)n * n, b = sizeof(double) * nn;
     XferBuf x(c, xfer_bytes({4 * b}));
     double* d = x.take(4 * b);
+    double* dB2 = same ? d + nn : d + 2 * nn;
     h2d_matrix(c, d, n, Y, n, n, n);
     h2d_matrix(c, d + nn, n, B1, n, n, n);
-    h2d_matrix(c, d + 2 * nn, n, B2, n, n, n);
-    decouple_dev(c, n, d, d + nn, d + 2 * nn, d + 3 * nn);
+    if (!same) h2d_matrix(c, dB2, n, B2, n, n, n);
+    decouple_dev(c, n, d, d + nn, dB2, d + 3 * nn);
     d2h_matrix(c, out, n, d + 3 * nn, n, n, n);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });

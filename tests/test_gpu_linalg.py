@@ -226,3 +226,16 @@ def test_decouple_ill_conditioned(ctx):
     rec = lambda Z: np.linalg.norm(Bm @ Z @ Bm.T - Y) / np.linalg.norm(Y)
     assert rec(D) < 10 * rec(R) + 1e-15, (rec(D), rec(R))
     assert np.linalg.norm(D - R) / np.linalg.norm(R) < 1e8 * 1e-14
+
+
+def test_decouple_shared_matrix_factored_once(ctx):
+    """decouple_covmat(Y, M, M) with one array on both sides factors it once;
+    the result equals the call with an equal copy as B2 bit for bit."""
+    lmax = 400
+    M = ctx.mcm(0, 0, lmax, mask_like_W(lmax + 1, seed=12))
+    X = np.random.default_rng(6).standard_normal(M.shape)
+    Y = np.asfortranarray(X @ X.T)
+    same = ctx.decouple_covmat(Y, M, M)
+    apart = ctx.decouple_covmat(Y, M, np.array(M, order="F", copy=True))
+    assert np.array_equal(same, apart)
+    assert np.linalg.norm(M @ same @ M.T - Y) / np.linalg.norm(Y) < 1e-11
