@@ -399,9 +399,12 @@ class Context:
         return LU, ipiv
 
     def decouple_covmat(self, Y, B1, B2):
+        """B1^{-1} Y B2^{-T}; one array passed as both B1 and B2 reaches the
+        library as one pointer, so it is factored once."""
         Y = np.asfortranarray(Y, dtype=np.float64)
+        shared = B2 is B1
         B1 = np.asfortranarray(B1, dtype=np.float64)
-        B2 = np.asfortranarray(B2, dtype=np.float64)
+        B2 = B1 if shared else np.asfortranarray(B2, dtype=np.float64)
         n = Y.shape[0]
         if Y.shape != (n, n) or B1.shape != (n, n) or B2.shape != (n, n):
             raise AxisError("decouple_covmat needs three n x n matrices")
