@@ -616,23 +616,9 @@ static void lu_factor_checked(Ctx* c, int n, double* dA, int64_t lda, int* ipiv)
                   " (Julia SingularException)");
 }
 
-// B = A^{-1} B from the factors of A in dA and its interchanges ipiv.
-static void lu_getrs_dev(Ctx* c, int n, const double* dA, int64_t lda, const int* ipiv,
-                         double* dB, int64_t ldb, int nrhs) {
-  if (nrhs <= 0) return;
-  // P b by the LAPACK interchanges, then the two substitutions
-  PSX_ROC(rocsolver_dlaswp(blas(c), nrhs, dB, (rocblas_int)ldb, 1, n, (const rocblas_int*)ipiv,
-                           1));
-  if (nrhs <= kTrsMaxRhs) {
-    for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
-  } else {
-    trsm_left_dev(c, false, n, dA, lda, dB, ldb, nrhs);
-    trsm_left_dev(c, true, n, dA, lda, dB, ldb, nrhs);
-  }
-}
-
-// Device LAPACK interchanges for one factorization, freed on scope exit
-// after the stream has drained.
+// A device int array of n + 64 entries (LAPACK interchanges and their info
+// slot, or a row permutation), freed on scope exit after the stream has
+// drained.
 struct Pivots {
   Ctx* c;
   int* p;
@@ -645,6 +631,69 @@ struct Pivots {
     }
   }
 };
+
+// The row order LAPACK's interchanges produce, perm[i] = the original row
+// that ends at position i, built by one thread in LDS (the interchanges are
+// sequential by definition) and written out by the workgroup.
+constexpr int kPermMax = 16384;
+__global__ void __launch_bounds__(256) lu_perm(const int* __restrict__ ipiv, int n,
+                                               int* __restrict__ perm) {
+  __shared__ int pr[kPermMax], iv[kPermMax];
+  for (int i = threadIdx.x; i < n; i += 256) {
+    pr[i] = i;
+    iv[i] = ipiv[i] - 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < n; ++i) {
+      const int q = iv[i];
+      if (q != i && q > i && q < n) {
+        const int t = pr[i];
+        pr[i] = pr[q];
+        pr[q] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) perm[i] = pr[i];
+}
+
+// Column blockIdx.x of B (n rows, ldb) reordered in place: b[i] = b[perm[i]],
+// through LDS.
+__global__ void __launch_bounds__(256) lu_permute_cols(const int* __restrict__ perm, int n,
+                                                       double* __restrict__ B, int64_t ldb) {
+  __shared__ double col[kPermMax];
+  double* b = B + (int64_t)blockIdx.x * ldb;
+  for (int i = threadIdx.x; i < n; i += 256) col[i] = b[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) b[i] = col[perm[i]];
+}
+
+// B = A^{-1} B from the factors of A in dA and its interchanges ipiv.
+static void lu_getrs_dev(Ctx* c, int n, const double* dA, int64_t lda, const int* ipiv,
+                         double* dB, int64_t ldb, int nrhs) {
+  if (nrhs <= 0) return;
+  // P b by the LAPACK interchanges, then the two substitutions.  Many
+  // columns share one permutation, built once and applied as a gather;
+  // a few columns take rocSOLVER's laswp (the sequential chain is the same).
+  if (nrhs > kTrsMaxRhs && n <= kPermMax) {
+    Pivots perm(c, n);
+    hipLaunchKernelGGL(lu_perm, dim3(1), dim3(256), 0, c->stream, ipiv, n, perm.p);
+    hipLaunchKernelGGL(lu_permute_cols, dim3((unsigned)nrhs), dim3(256), 0, c->stream,
+                       (const int*)perm.p, n, dB, ldb);
+    PSX_HIP(hipGetLastError());
+  } else {
+    PSX_ROC(rocsolver_dlaswp(blas(c), nrhs, dB, (rocblas_int)ldb, 1, n,
+                             (const rocblas_int*)ipiv, 1));
+  }
+  if (nrhs <= kTrsMaxRhs) {
+    for (int j = 0; j < nrhs; ++j) lu_trs_dev(c, n, dA, lda, dB + (int64_t)j * ldb);
+  } else {
+    trsm_left_dev(c, false, n, dA, lda, dB, ldb, nrhs);
+    trsm_left_dev(c, true, n, dA, lda, dB, ldb, nrhs);
+  }
+}
+
 
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                   int nrhs) {
