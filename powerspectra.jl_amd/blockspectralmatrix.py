@@ -2,7 +2,7 @@
 
 Mirrors src/blockspectralmatrix.jl: block assembly (hvcat/vcat/hcat, :17-45),
 `\\` with LU and partial pivoting (:89-129, here the library's blocked getrf
-and rocSOLVER getrs through psx_lu_solve), getblock and the @spectra unpack
+and triangular substitution through psx_lu_solve), getblock and the @spectra unpack
 (:144-184).
 """
 from __future__ import annotations
