@@ -72,6 +72,11 @@ struct LuWork {
   int* piv;           // the leaf's pivot rows (original positions)
   int* ipiv;          // LAPACK interchanges, 1-based, n entries
   int* info;          // first zero pivot column (1-based) or 0
+  // the leaf's net row movement: position mkey[k] receives the row at
+  // position mval[k], k < *nmove (at most 2 kLuLeaf)
+  int* mkey;
+  int* mval;
+  int* nmove;
 };
 
 // |x| for the pivot search; NaN ranks above everything, so a NaN column
@@ -265,22 +270,42 @@ __global__ void __launch_bounds__(64) lu_leaf_ipiv(int c, int nc, LuWork w) {
       }
     }
   }
+  // publish the positions whose row changed, compacted
+  const bool moved = lane < m && key != val;
+  const unsigned long long hm = __ballot(moved);
+  const int slot = __popcll(hm & ((1ull << lane) - 1ull));
+  if (moved) {
+    w.mkey[slot] = key;
+    w.mval[slot] = val;
+  }
+  if (lane == 0) *w.nmove = __popcll(hm);
 }
 
-// Apply the leaf's interchanges (positions c .. c+nc-1) to every column.
-__global__ void __launch_bounds__(256) lu_laswp(double* A, int64_t lda, int ncols, int c, int nc,
-                                                const int* ipiv) {
-  const int col = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (col >= ncols) return;
-  double* a = A + (int64_t)col * lda;
-  for (int j = 0; j < nc; ++j) {
-    const int q = ipiv[c + j] - 1;
-    if (q != c + j) {
-      const double t = a[c + j];
-      a[c + j] = a[q];
-      a[q] = t;
-    }
+// Apply the leaf's row movement to every column, one thread per column: the
+// moved rows' values are all loaded, then all stored (the same result as the
+// leaf's interchanges applied one after another).
+__global__ void __launch_bounds__(256) lu_leaf_move(double* A, int64_t lda, int ncols, LuWork w) {
+  __shared__ int sk[2 * kLuLeaf], sv[2 * kLuLeaf];
+  __shared__ int sn;
+  const int t = threadIdx.x;
+  if (t == 0) sn = *w.nmove;
+  __syncthreads();
+  const int nm = sn;
+  if (t < nm) {
+    sk[t] = w.mkey[t];
+    sv[t] = w.mval[t];
   }
+  __syncthreads();
+  const int col = (int)blockIdx.x * 256 + t;
+  if (col >= ncols || nm == 0) return;
+  double* a = A + (int64_t)col * lda;
+  double v[2 * kLuLeaf];
+#pragma unroll
+  for (int k = 0; k < 2 * kLuLeaf; ++k)
+    if (k < nm) v[k] = a[sv[k]];
+#pragma unroll
+  for (int k = 0; k < 2 * kLuLeaf; ++k)
+    if (k < nm) a[sk[k]] = v[k];
 }
 
 void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinfo) {
@@ -288,8 +313,9 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
   hipStream_t s = c->stream;
   const int Gmax = (n + kLuRows - 1) / kLuRows;
   const size_t rows_bytes = 2 * (size_t)Gmax * kLuLeaf * sizeof(double);
-  char* base = (char*)ctx_scratch(c, rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf + (size_t)n +
-                                         256);
+  const size_t move_off = (rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf + (size_t)n + 255) &
+                          ~size_t(255);
+  char* base = (char*)ctx_scratch(c, move_off + sizeof(int) * (4 * kLuLeaf + 1) + 256);
   LuWork w;
   w.cstride = Gmax;
   w.candrow = (double*)base;                                  // 2 Gmax kLuLeaf doubles
@@ -297,6 +323,9 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
   w.candi = (int*)(base + rows_bytes + 16 * (size_t)Gmax);    // 2 Gmax ints
   w.piv = (int*)(base + rows_bytes + 24 * (size_t)Gmax);      // kLuLeaf ints
   w.done = (unsigned char*)(base + rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf);  // n flags
+  w.mkey = (int*)(base + move_off);                           // 2 kLuLeaf ints
+  w.mval = w.mkey + 2 * kLuLeaf;                              // 2 kLuLeaf ints
+  w.nmove = w.mval + 2 * kLuLeaf;                             // 1 int
   w.ipiv = dipiv;
   w.info = dinfo;
   PSX_HIP(hipMemsetAsync(dinfo, 0, sizeof(int), s));
@@ -312,8 +341,7 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
         hipLaunchKernelGGL(lu_leaf_col, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, j, cend, G,
                            w);
       hipLaunchKernelGGL(lu_leaf_ipiv, dim3(1), dim3(64), 0, s, cl, nc, w);
-      hipLaunchKernelGGL(lu_laswp, dim3((n + 255) / 256), dim3(256), 0, s, dA, lda, n, cl, nc,
-                         (const int*)dipiv);
+      hipLaunchKernelGGL(lu_leaf_move, dim3((n + 255) / 256), dim3(256), 0, s, dA, lda, n, w);
       PSX_HIP(hipGetLastError());
       if (cend < kend) {   // rest of the block: U = L11^-1 A, then the update
         double* L11 = dA + cl + (int64_t)cl * lda;
