@@ -56,7 +56,7 @@ void release_blas(Ctx* c) {
 // column measured slower: the cross-XCD barrier costs more than a launch.)
 constexpr int kLuLeaf = 32;
 constexpr int kLuPanel = 256;
-constexpr int kLuRows = 256;     // rows per workgroup in the leaf kernels (one per thread)
+constexpr int kLuRows = 128;     // rows per workgroup in the leaf kernels (one per thread)
 
 struct LuWork {
   // per-workgroup pivot candidates, two buffers of cstride entries: the
