@@ -530,74 +530,107 @@ void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
 // the block staged in LDS), and the rows still to come are updated by a
 // rocBLAS GEMM.  Substitution keeps LAPACK's backward error on ill-conditioned
 // U, which a trsm built on inverted diagonal blocks does not.
+constexpr int kTsmRows = 64;               // diagonal block of the many-column solve
+constexpr int kTsmLd = kTsmRows + 1;        // its LDS column stride
+constexpr int kTsmCols = 192;               // right-hand sides per workgroup (one per thread)
+
 template <bool UPPER>
-__global__ void __launch_bounds__(256) trsm_diag(const double* __restrict__ A, int64_t lda, int n,
-                                                 int k, double* __restrict__ B, int64_t ldb,
-                                                 int m) {
-  __shared__ double D[kTrsRows * kTrsLd];  // the diagonal block, column-major
+__global__ void __launch_bounds__(kTsmCols) trsm_diag(const double* __restrict__ A, int64_t lda,
+                                                      int n, int k, double* __restrict__ B,
+                                                      int64_t ldb, int m) {
+  __shared__ double D[kTsmRows * kTsmLd];         // the diagonal block, column-major
+  __shared__ double Bs[kTsmRows][kTsmCols];       // this workgroup's rows of B
   const int t = threadIdx.x;
-  const int R0 = k * kTrsRows, cn = min(kTrsRows, n - R0);
+  const int R0 = k * kTsmRows, cn = min(kTsmRows, n - R0);
+  const int c0 = (int)blockIdx.x * kTsmCols, cm = min(kTsmCols, m - c0);
+  // stage D and B: 64 consecutive rows per wave, one column per wave and pass
+  const int lane = t & 63, wv = t >> 6;
   {
-    const int rl = t & (kTrsRows - 1), h = t >> 7;
-    const bool rin = rl < cn;
-    const double* a = A + (R0 + (rin ? rl : 0)) + (int64_t)R0 * lda;
+    const bool rin = lane < cn;
+    const double* a = A + (R0 + (rin ? lane : 0)) + (int64_t)R0 * lda;
+    constexpr int kW = kTsmCols / 64;                       // waves
+    constexpr int kPer = (kTsmRows + kW - 1) / kW;          // columns per wave, rounded up
+    double v[kPer];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      double v[32];
+    for (int q = 0; q < kPer; ++q) {
+      const int ic = q * kW + wv;
+      v[q] = (rin && ic < cn) ? a[(int64_t)ic * lda] : 0.0;
+    }
 #pragma unroll
-      for (int q = 0; q < 32; ++q) {
-        const int ic = 2 * (half * 32 + q) + h;
-        v[q] = (rin && ic < cn) ? a[(int64_t)ic * lda] : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 32; ++q) D[(2 * (half * 32 + q) + h) * kTrsLd + rl] = v[q];
+    for (int q = 0; q < kPer; ++q) {
+      const int ic = q * kW + wv;
+      if (ic < kTsmRows) D[ic * kTsmLd + lane] = v[q];
+    }
+  }
+  {
+    const bool rin = lane < cn;
+    const double* b = B + (R0 + (rin ? lane : 0)) + (int64_t)c0 * ldb;
+    constexpr int kPer = kTsmCols / (kTsmCols / 64);  // 64 columns per wave
+#pragma unroll 16
+    for (int q = 0; q < kPer; ++q) {
+      const int jc = q * (kTsmCols / 64) + wv;
+      Bs[lane][jc] = (rin && jc < cm) ? b[(int64_t)jc * ldb] : 0.0;
     }
   }
   __syncthreads();
-  const int col = (int)blockIdx.x * 256 + t;
-  if (col >= m) return;  // no barrier below
-  double* b = B + (int64_t)col * ldb + R0;
-  // 32-row sub-blocks: the one being solved sits in registers, and each
-  // solved sub-block is applied to the rows of the block still to come
-  constexpr int S = 32;
-  const int nsub = (cn + S - 1) / S;
-  for (int q = 0; q < nsub; ++q) {
-    const int sb = UPPER ? nsub - 1 - q : q;
-    const int s0 = sb * S, sn = min(S, cn - s0);
-    double x[S];
+  if (t < cm) {
+    // two 32-row sub-blocks; the one being solved sits in registers
+    constexpr int S = 32;
+    const int nsub = (cn + S - 1) / S;
+#pragma unroll 1
+    for (int q = 0; q < nsub; ++q) {
+      const int sb = UPPER ? nsub - 1 - q : q;
+      const int s0 = sb * S, sn = min(S, cn - s0);
+      double x[S];
 #pragma unroll
-    for (int i = 0; i < S; ++i) x[i] = i < sn ? b[s0 + i] : 0.0;
-    if (!UPPER) {
+      for (int i = 0; i < S; ++i) x[i] = Bs[(s0 + i) & (kTsmRows - 1)][t];
+      if (!UPPER) {
 #pragma unroll
-      for (int j = 0; j < S; ++j) {
-        if (j < sn) {
+        for (int j = 0; j < S; ++j) {
+          if (j < sn) {
 #pragma unroll
-          for (int i = j + 1; i < S; ++i)
-            x[i] = fma(-D[(s0 + j) * kTrsLd + s0 + i], x[j], x[i]);
+            for (int i = j + 1; i < S; ++i)
+              x[i] = fma(-D[(s0 + j) * kTsmLd + s0 + i], x[j], x[i]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
-        __builtin_amdgcn_sched_barrier(0);  // keep each column's LDS reads with its step
+      } else {
+#pragma unroll
+        for (int j = S - 1; j >= 0; --j) {
+          if (j < sn) {
+            x[j] = x[j] / D[(s0 + j) * kTsmLd + s0 + j];
+#pragma unroll
+            for (int i = 0; i < j; ++i) x[i] = fma(-D[(s0 + j) * kTsmLd + s0 + i], x[j], x[i]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
-    } else {
 #pragma unroll
-      for (int j = S - 1; j >= 0; --j) {
-        if (j < sn) {
-          x[j] = x[j] / D[(s0 + j) * kTrsLd + s0 + j];
+      for (int i = 0; i < S; ++i)
+        if (i < sn) Bs[s0 + i][t] = x[i];
+      // the other sub-block's rows take this one's contribution
+      const int r0 = UPPER ? 0 : s0 + sn, r1 = UPPER ? s0 : cn;
+#pragma unroll 4
+      for (int i = r0; i < r1; ++i) {
+        double p0 = 0.0, p1 = 0.0;
 #pragma unroll
-          for (int i = 0; i < j; ++i) x[i] = fma(-D[(s0 + j) * kTrsLd + s0 + i], x[j], x[i]);
+        for (int j = 0; j < S; j += 2) {
+          p0 = fma(D[(s0 + j) * kTsmLd + i], x[j], p0);
+          p1 = fma(D[(s0 + j + 1) * kTsmLd + i], x[j + 1], p1);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        Bs[i][t] -= p0 + p1;
       }
     }
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-      if (i < sn) b[s0 + i] = x[i];
-    // rows of this diagonal block not yet solved (below for L, above for U)
-    const int r0 = UPPER ? 0 : s0 + sn, r1 = UPPER ? s0 : cn;
-    for (int i = r0; i < r1; ++i) {
-      double acc = b[i];
-#pragma unroll
-      for (int j = 0; j < S; ++j) acc = fma(-D[(s0 + j) * kTrsLd + i], x[j], acc);
-      b[i] = acc;
+  }
+  __syncthreads();
+  {
+    const bool rin = lane < cn;
+    double* b = B + (R0 + (rin ? lane : 0)) + (int64_t)c0 * ldb;
+    constexpr int kPer = kTsmCols / (kTsmCols / 64);
+#pragma unroll 16
+    for (int q = 0; q < kPer; ++q) {
+      const int jc = q * (kTsmCols / 64) + wv;
+      if (rin && jc < cm) b[(int64_t)jc * ldb] = Bs[lane][jc];
     }
   }
 }
@@ -608,16 +641,18 @@ static void trsm_left_dev(Ctx* c, bool upper, int n, const double* dA, int64_t l
                           int64_t ldb, int m) {
   rocblas_handle h = blas(c);
   hipStream_t s = c->stream;
-  const int nb = (n + kTrsRows - 1) / kTrsRows;
-  const unsigned g = (unsigned)((m + 255) / 256);
+  const int nb = (n + kTsmRows - 1) / kTsmRows;
+  const unsigned g = (unsigned)((m + kTsmCols - 1) / kTsmCols);
   const double one = 1.0, mone = -1.0;
   for (int q = 0; q < nb; ++q) {
     const int k = upper ? nb - 1 - q : q;
-    const int R0 = k * kTrsRows, cn = std::min(kTrsRows, n - R0);
+    const int R0 = k * kTsmRows, cn = std::min(kTsmRows, n - R0);
     if (upper)
-      hipLaunchKernelGGL(trsm_diag<true>, dim3(g), dim3(256), 0, s, dA, lda, n, k, dB, ldb, m);
+      hipLaunchKernelGGL(trsm_diag<true>, dim3(g), dim3(kTsmCols), 0, s, dA, lda, n, k, dB, ldb,
+                         m);
     else
-      hipLaunchKernelGGL(trsm_diag<false>, dim3(g), dim3(256), 0, s, dA, lda, n, k, dB, ldb, m);
+      hipLaunchKernelGGL(trsm_diag<false>, dim3(g), dim3(kTsmCols), 0, s, dA, lda, n, k, dB, ldb,
+                         m);
     PSX_HIP(hipGetLastError());
     if (!upper && R0 + cn < n)  // rows below: B2 -= L21 X1
       PSX_ROC(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n - R0 - cn, m, cn,
