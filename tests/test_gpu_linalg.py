@@ -216,7 +216,7 @@ def test_decouple_ill_conditioned(ctx):
     10x of numpy's two LAPACK solves.  (On the TT mode-coupling matrix of the
     lmax-2508 covariance workload, cond 2.4e8, rocSOLVER's getrs, whose trsm
     inverts diagonal blocks, reproduced Y to 1.4e-7 against numpy's 2.6e-9;
-    the substitution gives 2.0e-9.)"""
+    the substitution gives 2-4e-9.)"""
     n = 600
     Bm = _ill_conditioned(n, 1e8, 3)
     X = np.random.default_rng(2).standard_normal((n, n))
