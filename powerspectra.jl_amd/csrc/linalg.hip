@@ -231,6 +231,102 @@ __global__ void __launch_bounds__(kLuRows) lu_leaf_col(double* A, int64_t lda, i
   }
 }
 
+// The leaf factored by one workgroup when its rows fit in registers: 1024
+// threads, thread t holding rows c + t + 1024 k (k < NR) of the leaf's
+// kLuWgLeaf columns.  Per column: a workgroup argmax (ties to the lower row,
+// NaN first, as lu_better), the pivot row through LDS, and the elimination
+// of every other unpivoted row -- the arithmetic of lu_leaf_col, with two
+// barriers in place of a launch.  Rows stay in place; the pivots go to
+// w.piv for lu_leaf_ipiv and lu_leaf_move, as on the multi-workgroup path.
+constexpr int kLuWgLeaf = 16;
+constexpr int kLuWgThreads = 1024;
+constexpr int kLuWgRowsMax = 3 * kLuWgThreads;   // leaf rows one workgroup takes
+
+template <int NR>
+__global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t lda, int n, int c,
+                                                           int cend, LuWork w) {
+  __shared__ double sv[kLuWgThreads / 64];
+  __shared__ int sr[kLuWgThreads / 64];
+  __shared__ double u[kLuWgLeaf];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nc = cend - c;
+  double vals[NR][kLuWgLeaf];
+  bool live[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int r = c + t + kLuWgThreads * k;
+    live[k] = r < n;
+#pragma unroll
+    for (int jj = 0; jj < kLuWgLeaf; ++jj)
+      vals[k][jj] = (live[k] && jj < nc) ? A[r + (int64_t)(c + jj) * lda] : 0.0;
+  }
+  for (int j = 0; j < nc; ++j) {
+    double v = -1.0;
+    int r = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double x = vals[k][0];
+#pragma unroll
+      for (int jj = 1; jj < kLuWgLeaf; ++jj)
+        if (jj == j) x = vals[k][jj];
+      if (live[k]) lu_better(v, r, lu_mag(x), c + t + kLuWgThreads * k);
+    }
+    for (int o = 32; o >= 1; o >>= 1) lu_better(v, r, __shfl_xor(v, o), __shfl_xor(r, o));
+    if (lane == 0) {
+      sv[wv] = v;
+      sr[wv] = r;
+    }
+    __syncthreads();
+    v = sv[0];
+    r = sr[0];
+    for (int q = 1; q < kLuWgThreads / 64; ++q) lu_better(v, r, sv[q], sr[q]);
+    const int p = r < n ? r : c + j;  // a candidate always exists; the clamp keeps accesses in
+                                      // bounds regardless
+    // the pivot row's owner publishes its entries and drops out
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      if (live[k] && c + t + kLuWgThreads * k == p) {
+#pragma unroll
+        for (int jj = 0; jj < kLuWgLeaf; ++jj)
+          if (jj >= j && jj < nc) u[jj] = vals[k][jj];
+        live[k] = false;
+      }
+    }
+    __syncthreads();
+    const double piv = u[j];
+    if (t == 0) {
+      w.piv[j] = p;
+      if (piv == 0.0 && *w.info == 0) *w.info = c + j + 1;
+    }
+    if (piv != 0.0) {
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        if (!live[k]) continue;
+        double x = vals[k][0];
+#pragma unroll
+        for (int jj = 1; jj < kLuWgLeaf; ++jj)
+          if (jj == j) x = vals[k][jj];
+        const double l = x / piv;
+#pragma unroll
+        for (int jj = 0; jj < kLuWgLeaf; ++jj) {
+          if (jj == j) vals[k][jj] = l;
+          if (jj > j && jj < nc) vals[k][jj] = fma(-l, u[jj], vals[k][jj]);
+        }
+      }
+    }
+  }
+  // rows whose pivot step set live false keep their (pivot-row) values
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int r = c + t + kLuWgThreads * k;
+    if (r < n) {
+#pragma unroll
+      for (int jj = 0; jj < kLuWgLeaf; ++jj)
+        if (jj < nc) A[r + (int64_t)(c + jj) * lda] = vals[k][jj];
+    }
+  }
+}
+
 // LAPACK interchanges of the leaf: at step j the row chosen (original
 // position piv[j]) is swapped with whatever row is at position c + j.  Only
 // positions touched by this leaf's swaps differ from the identity; one
@@ -332,14 +428,30 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
   const double one = 1.0, mone = -1.0;
   for (int k = 0; k < n; k += kLuPanel) {
     const int kend = std::min(n, k + kLuPanel);
-    for (int cl = k; cl < kend; cl += kLuLeaf) {
-      const int cend = std::min(kend, cl + kLuLeaf);
+    for (int cl = k; cl < kend;) {
+      // the trailing leaves, once their rows fit one workgroup, in one launch each
+      const int rows = n - cl;
+      const bool one_wg = rows <= kLuWgRowsMax;
+      const int cend = std::min(kend, cl + (one_wg ? kLuWgLeaf : kLuLeaf));
       const int nc = cend - cl;
-      const int G = (n - cl + kLuRows - 1) / kLuRows;
-      hipLaunchKernelGGL(lu_leaf_init, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, cend, w);
-      for (int j = 0; j < nc; ++j)
-        hipLaunchKernelGGL(lu_leaf_col, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, j, cend, G,
-                           w);
+      if (one_wg) {
+        const int nr = (rows + kLuWgThreads - 1) / kLuWgThreads;
+        if (nr == 1)
+          hipLaunchKernelGGL(lu_leaf_wg<1>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
+                             w);
+        else if (nr == 2)
+          hipLaunchKernelGGL(lu_leaf_wg<2>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
+                             w);
+        else
+          hipLaunchKernelGGL(lu_leaf_wg<3>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
+                             w);
+      } else {
+        const int G = (n - cl + kLuRows - 1) / kLuRows;
+        hipLaunchKernelGGL(lu_leaf_init, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, cend, w);
+        for (int j = 0; j < nc; ++j)
+          hipLaunchKernelGGL(lu_leaf_col, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, j, cend,
+                             G, w);
+      }
       hipLaunchKernelGGL(lu_leaf_ipiv, dim3(1), dim3(64), 0, s, cl, nc, w);
       hipLaunchKernelGGL(lu_leaf_move, dim3((n + 255) / 256), dim3(256), 0, s, dA, lda, n, w);
       PSX_HIP(hipGetLastError());
@@ -355,6 +467,7 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
                                 (rocblas_int)lda, A12, (rocblas_int)lda, &one,
                                 dA + cend + (int64_t)cend * lda, (rocblas_int)lda));
       }
+      cl = cend;
     }
     if (kend < n) {        // trailing matrix
       double* L11 = dA + k + (int64_t)k * lda;

@@ -68,10 +68,14 @@ def test_singular(ctx, pkg):
         ctx.decouple_covmat(np.eye(4), np.zeros((4, 4)), np.eye(4))
 
 
-@pytest.mark.parametrize("n", [1, 5, 32, 33, 256, 257, 289, 300, 513, 1000, 2100])
+@pytest.mark.parametrize("n", [1, 5, 16, 17, 32, 33, 256, 257, 289, 300, 513, 1000, 1025, 2100,
+                               3072, 3100, 4100])
 def test_lu_factor_vs_lapack(ctx, n):
     """The blocked getrf (leaf kernels + rocBLAS updates) against LAPACK's
-    getrf (scipy): the same row interchanges and factors to rounding."""
+    getrf (scipy): the same row interchanges and factors to rounding.  Up to
+    3072 leaf rows one workgroup factors a 16-column leaf in registers; above
+    that each column is a launch over many workgroups (n = 3100 and 4100 run
+    both, switching when the remaining rows drop to 3072)."""
     import scipy.linalg as sl
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
