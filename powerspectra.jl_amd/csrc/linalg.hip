@@ -7,6 +7,8 @@
 // output.
 #include "psx_internal.h"
 
+#include <utility>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -242,13 +244,80 @@ constexpr int kLuWgLeaf = 16;
 constexpr int kLuWgThreads = 1024;
 constexpr int kLuWgRowsMax = 3 * kLuWgThreads;   // leaf rows one workgroup takes
 
+// One column step of lu_leaf_wg (J static, so every register index is).
+template <int NR, int J>
+__device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
+                                           int t, int c, int n, int nc, double* sv, int* sr,
+                                           double* u, const LuWork& w) {
+  if (J >= nc) return;  // uniform
+  const int lane = t & 63, wv = t >> 6;
+  double v = -1.0;
+  int r = 0x7fffffff;
+#pragma unroll
+  for (int k = 0; k < NR; ++k)
+    if (live[k]) lu_better(v, r, lu_mag(vals[k][J]), c + t + kLuWgThreads * k);
+  for (int o = 32; o >= 1; o >>= 1) lu_better(v, r, __shfl_xor(v, o), __shfl_xor(r, o));
+  if (lane == 0) {
+    sv[wv] = v;
+    sr[wv] = r;
+  }
+  __syncthreads();
+  // every wave combines the 16 wave winners: one LDS read per lane, then a
+  // 16-lane butterfly, and lane 0's result to all lanes
+  {
+    constexpr int kW = kLuWgThreads / 64;
+    v = lane < kW ? sv[lane] : -1.0;
+    r = lane < kW ? sr[lane] : 0x7fffffff;
+    for (int o = kW / 2; o >= 1; o >>= 1) lu_better(v, r, __shfl_xor(v, o), __shfl_xor(r, o));
+    r = __shfl(r, 0);
+  }
+  // a candidate always exists; the clamp keeps accesses in bounds regardless
+  const int p = r < n ? r : c + J;
+  // the pivot row's owner publishes its entries and drops out
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    if (live[k] && c + t + kLuWgThreads * k == p) {
+#pragma unroll
+      for (int jj = J; jj < kLuWgLeaf; ++jj)
+        if (jj < nc) u[jj] = vals[k][jj];
+      live[k] = false;
+    }
+  }
+  __syncthreads();
+  const double piv = u[J];
+  if (t == 0) {
+    w.piv[J] = p;
+    if (piv == 0.0 && *w.info == 0) *w.info = c + J + 1;
+  }
+  if (piv != 0.0) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      if (live[k]) {  // the pivot rows are no longer live
+        const double l = vals[k][J] / piv;
+        vals[k][J] = l;
+#pragma unroll
+        for (int jj = J + 1; jj < kLuWgLeaf; ++jj)
+          if (jj < nc) vals[k][jj] = fma(-l, u[jj], vals[k][jj]);
+      }
+    }
+  }
+}
+
+template <int NR, int... J>
+__device__ __forceinline__ void lu_wg_steps(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
+                                            int t, int c, int n, int nc, double* sv, int* sr,
+                                            double* u, const LuWork& w,
+                                            std::integer_sequence<int, J...>) {
+  (lu_wg_step<NR, J>(vals, live, t, c, n, nc, sv, sr, u, w), ...);
+}
+
 template <int NR>
 __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t lda, int n, int c,
                                                            int cend, LuWork w) {
   __shared__ double sv[kLuWgThreads / 64];
   __shared__ int sr[kLuWgThreads / 64];
   __shared__ double u[kLuWgLeaf];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x;
   const int nc = cend - c;
   double vals[NR][kLuWgLeaf];
   bool live[NR];
@@ -260,62 +329,9 @@ __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t ld
     for (int jj = 0; jj < kLuWgLeaf; ++jj)
       vals[k][jj] = (live[k] && jj < nc) ? A[r + (int64_t)(c + jj) * lda] : 0.0;
   }
-  for (int j = 0; j < nc; ++j) {
-    double v = -1.0;
-    int r = 0x7fffffff;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      double x = vals[k][0];
-#pragma unroll
-      for (int jj = 1; jj < kLuWgLeaf; ++jj)
-        if (jj == j) x = vals[k][jj];
-      if (live[k]) lu_better(v, r, lu_mag(x), c + t + kLuWgThreads * k);
-    }
-    for (int o = 32; o >= 1; o >>= 1) lu_better(v, r, __shfl_xor(v, o), __shfl_xor(r, o));
-    if (lane == 0) {
-      sv[wv] = v;
-      sr[wv] = r;
-    }
-    __syncthreads();
-    v = sv[0];
-    r = sr[0];
-    for (int q = 1; q < kLuWgThreads / 64; ++q) lu_better(v, r, sv[q], sr[q]);
-    const int p = r < n ? r : c + j;  // a candidate always exists; the clamp keeps accesses in
-                                      // bounds regardless
-    // the pivot row's owner publishes its entries and drops out
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      if (live[k] && c + t + kLuWgThreads * k == p) {
-#pragma unroll
-        for (int jj = 0; jj < kLuWgLeaf; ++jj)
-          if (jj >= j && jj < nc) u[jj] = vals[k][jj];
-        live[k] = false;
-      }
-    }
-    __syncthreads();
-    const double piv = u[j];
-    if (t == 0) {
-      w.piv[j] = p;
-      if (piv == 0.0 && *w.info == 0) *w.info = c + j + 1;
-    }
-    if (piv != 0.0) {
-#pragma unroll
-      for (int k = 0; k < NR; ++k) {
-        if (!live[k]) continue;
-        double x = vals[k][0];
-#pragma unroll
-        for (int jj = 1; jj < kLuWgLeaf; ++jj)
-          if (jj == j) x = vals[k][jj];
-        const double l = x / piv;
-#pragma unroll
-        for (int jj = 0; jj < kLuWgLeaf; ++jj) {
-          if (jj == j) vals[k][jj] = l;
-          if (jj > j && jj < nc) vals[k][jj] = fma(-l, u[jj], vals[k][jj]);
-        }
-      }
-    }
-  }
-  // rows whose pivot step set live false keep their (pivot-row) values
+  lu_wg_steps<NR>(vals, live, t, c, n, nc, sv, sr, u, w,
+                  std::make_integer_sequence<int, kLuWgLeaf>{});
+  // pivot rows keep the values they had when chosen
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     const int r = c + t + kLuWgThreads * k;
