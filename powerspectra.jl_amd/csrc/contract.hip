@@ -684,8 +684,13 @@ Config cfg(const char* name) {
 Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
   if (rq.epi == EPI_MCM) {
-    if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
+    if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0) {
+      // TT: 8 anti-diagonals per tile up to 4096 rows (5 % faster at lmax 3071),
+      // 16 above (4.5 % faster at lmax 6143)
+      if (rq.lmax - rq.lmin < 4096)
+        return cfg<8, 1, 0, 0, 0, EPI_MCM, 0, PSX_WPE_00>("psx_mcm_00");
       return cfg<PSX_R_00, 1, 0, 0, 0, EPI_MCM, 0, PSX_WPE_00>("psx_mcm_00");
+    }
     if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
       return cfg<PSX_R_02, 0, 1, 0, 0, EPI_MCM, 0, PSX_WPE_02>("psx_mcm_02");
     if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
