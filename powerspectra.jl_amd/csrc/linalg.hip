@@ -244,11 +244,15 @@ constexpr int kLuWgLeaf = 16;
 constexpr int kLuWgThreads = 1024;
 constexpr int kLuWgRowsMax = 3 * kLuWgThreads;   // leaf rows one workgroup takes
 
+// LAPACK interchanges and net row movement of a leaf from its pivot rows, by
+// one wavefront (defined below).
+__device__ __forceinline__ void lu_leaf_ipiv_wave(int c, int nc, const int* piv, const LuWork& w);
+
 // One column step of lu_leaf_wg (J static, so every register index is).
 template <int NR, int J>
 __device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
                                            int t, int c, int n, int nc, double* sv, int* sr,
-                                           double* u, const LuWork& w) {
+                                           double* u, int* spiv, const LuWork& w) {
   if (J >= nc) return;  // uniform
   const int lane = t & 63, wv = t >> 6;
   double v = -1.0;
@@ -287,6 +291,7 @@ __device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (
   const double piv = u[J];
   if (t == 0) {
     w.piv[J] = p;
+    spiv[J] = p;
     if (piv == 0.0 && *w.info == 0) *w.info = c + J + 1;
   }
   if (piv != 0.0) {
@@ -306,9 +311,9 @@ __device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (
 template <int NR, int... J>
 __device__ __forceinline__ void lu_wg_steps(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
                                             int t, int c, int n, int nc, double* sv, int* sr,
-                                            double* u, const LuWork& w,
+                                            double* u, int* spiv, const LuWork& w,
                                             std::integer_sequence<int, J...>) {
-  (lu_wg_step<NR, J>(vals, live, t, c, n, nc, sv, sr, u, w), ...);
+  (lu_wg_step<NR, J>(vals, live, t, c, n, nc, sv, sr, u, spiv, w), ...);
 }
 
 template <int NR>
@@ -317,6 +322,7 @@ __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t ld
   __shared__ double sv[kLuWgThreads / 64];
   __shared__ int sr[kLuWgThreads / 64];
   __shared__ double u[kLuWgLeaf];
+  __shared__ int spiv[kLuWgLeaf];
   const int t = threadIdx.x;
   const int nc = cend - c;
   double vals[NR][kLuWgLeaf];
@@ -329,8 +335,12 @@ __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t ld
     for (int jj = 0; jj < kLuWgLeaf; ++jj)
       vals[k][jj] = (live[k] && jj < nc) ? A[r + (int64_t)(c + jj) * lda] : 0.0;
   }
-  lu_wg_steps<NR>(vals, live, t, c, n, nc, sv, sr, u, w,
+  lu_wg_steps<NR>(vals, live, t, c, n, nc, sv, sr, u, spiv, w,
                   std::make_integer_sequence<int, kLuWgLeaf>{});
+  // the leaf's interchanges and row movement (what lu_leaf_ipiv does on the
+  // other path), by wave 0 from the pivots in LDS
+  __syncthreads();
+  if (t < 64) lu_leaf_ipiv_wave(c, nc, spiv, w);
   // pivot rows keep the values they had when chosen
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
@@ -348,11 +358,11 @@ __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t ld
 // positions touched by this leaf's swaps differ from the identity; one
 // wavefront keeps them as (position, row) pairs, one pair per lane, and finds
 // entries by ballot.
-__global__ void __launch_bounds__(64) lu_leaf_ipiv(int c, int nc, LuWork w) {
-  const int lane = (int)threadIdx.x;
+__device__ __forceinline__ void lu_leaf_ipiv_wave(int c, int nc, const int* piv, const LuWork& w) {
+  const int lane = (int)(threadIdx.x & 63);
   int key = -1, val = -1, m = 0;
   for (int j = 0; j < nc; ++j) {
-    const int want = w.piv[j];
+    const int want = piv[j];
     const int pj = c + j;
     const unsigned long long hv = __ballot(key >= 0 && val == want);
     const int q = hv ? __shfl(key, __ffsll((long long)hv) - 1) : want;
@@ -391,6 +401,10 @@ __global__ void __launch_bounds__(64) lu_leaf_ipiv(int c, int nc, LuWork w) {
     w.mval[slot] = val;
   }
   if (lane == 0) *w.nmove = __popcll(hm);
+}
+
+__global__ void __launch_bounds__(64) lu_leaf_ipiv(int c, int nc, LuWork w) {
+  lu_leaf_ipiv_wave(c, nc, w.piv, w);
 }
 
 // Apply the leaf's row movement to every column, one thread per column: the
@@ -461,6 +475,7 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
         else
           hipLaunchKernelGGL(lu_leaf_wg<3>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
                              w);
+        // (the kernel converts its pivots itself)
       } else {
         const int G = (n - cl + kLuRows - 1) / kLuRows;
         hipLaunchKernelGGL(lu_leaf_init, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, cend, w);
@@ -468,7 +483,7 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
           hipLaunchKernelGGL(lu_leaf_col, dim3(G), dim3(kLuRows), 0, s, dA, lda, n, cl, j, cend,
                              G, w);
       }
-      hipLaunchKernelGGL(lu_leaf_ipiv, dim3(1), dim3(64), 0, s, cl, nc, w);
+      if (!one_wg) hipLaunchKernelGGL(lu_leaf_ipiv, dim3(1), dim3(64), 0, s, cl, nc, w);
       hipLaunchKernelGGL(lu_leaf_move, dim3((n + 255) / 256), dim3(256), 0, s, dA, lda, n, w);
       PSX_HIP(hipGetLastError());
       if (cend < kend) {   // rest of the block: U = L11^-1 A, then the update
