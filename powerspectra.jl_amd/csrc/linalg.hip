@@ -234,8 +234,8 @@ __global__ void __launch_bounds__(kLuRows) lu_leaf_col(double* A, int64_t lda, i
 }
 
 // The leaf factored by one workgroup when its rows fit in registers: 1024
-// threads, thread t holding rows c + t + 1024 k (k < NR) of the leaf's
-// kLuWgLeaf columns.  Per column: a workgroup argmax (ties to the lower row,
+// threads, thread t holding rows c + t + 1024 k (k < NR) of the leaf's W
+// columns (kLuLeaf with one row per thread, kLuWgLeaf with more).  Per column: a workgroup argmax (ties to the lower row,
 // NaN first, as lu_better), the pivot row through LDS, and the elimination
 // of every other unpivoted row -- the arithmetic of lu_leaf_col, with two
 // barriers in place of a launch.  Rows stay in place; the pivots go to
@@ -249,8 +249,8 @@ constexpr int kLuWgRowsMax = 3 * kLuWgThreads;   // leaf rows one workgroup take
 __device__ __forceinline__ void lu_leaf_ipiv_wave(int c, int nc, const int* piv, const LuWork& w);
 
 // One column step of lu_leaf_wg (J static, so every register index is).
-template <int NR, int J>
-__device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
+template <int NR, int W, int J>
+__device__ __forceinline__ void lu_wg_step(double (&vals)[NR][W], bool (&live)[NR],
                                            int t, int c, int n, int nc, double* sv, int* sr,
                                            double* u, int* spiv, const LuWork& w) {
   if (J >= nc) return;  // uniform
@@ -282,7 +282,7 @@ __device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (
   for (int k = 0; k < NR; ++k) {
     if (live[k] && c + t + kLuWgThreads * k == p) {
 #pragma unroll
-      for (int jj = J; jj < kLuWgLeaf; ++jj)
+      for (int jj = J; jj < W; ++jj)
         if (jj < nc) u[jj] = vals[k][jj];
       live[k] = false;
     }
@@ -301,42 +301,42 @@ __device__ __forceinline__ void lu_wg_step(double (&vals)[NR][kLuWgLeaf], bool (
         const double l = vals[k][J] / piv;
         vals[k][J] = l;
 #pragma unroll
-        for (int jj = J + 1; jj < kLuWgLeaf; ++jj)
+        for (int jj = J + 1; jj < W; ++jj)
           if (jj < nc) vals[k][jj] = fma(-l, u[jj], vals[k][jj]);
       }
     }
   }
 }
 
-template <int NR, int... J>
-__device__ __forceinline__ void lu_wg_steps(double (&vals)[NR][kLuWgLeaf], bool (&live)[NR],
+template <int NR, int W, int... J>
+__device__ __forceinline__ void lu_wg_steps(double (&vals)[NR][W], bool (&live)[NR],
                                             int t, int c, int n, int nc, double* sv, int* sr,
                                             double* u, int* spiv, const LuWork& w,
                                             std::integer_sequence<int, J...>) {
-  (lu_wg_step<NR, J>(vals, live, t, c, n, nc, sv, sr, u, spiv, w), ...);
+  (lu_wg_step<NR, W, J>(vals, live, t, c, n, nc, sv, sr, u, spiv, w), ...);
 }
 
-template <int NR>
+template <int NR, int W>
 __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t lda, int n, int c,
                                                            int cend, LuWork w) {
   __shared__ double sv[kLuWgThreads / 64];
   __shared__ int sr[kLuWgThreads / 64];
-  __shared__ double u[kLuWgLeaf];
-  __shared__ int spiv[kLuWgLeaf];
+  __shared__ double u[W];
+  __shared__ int spiv[W];
   const int t = threadIdx.x;
   const int nc = cend - c;
-  double vals[NR][kLuWgLeaf];
+  double vals[NR][W];
   bool live[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     const int r = c + t + kLuWgThreads * k;
     live[k] = r < n;
 #pragma unroll
-    for (int jj = 0; jj < kLuWgLeaf; ++jj)
+    for (int jj = 0; jj < W; ++jj)
       vals[k][jj] = (live[k] && jj < nc) ? A[r + (int64_t)(c + jj) * lda] : 0.0;
   }
-  lu_wg_steps<NR>(vals, live, t, c, n, nc, sv, sr, u, spiv, w,
-                  std::make_integer_sequence<int, kLuWgLeaf>{});
+  lu_wg_steps<NR, W>(vals, live, t, c, n, nc, sv, sr, u, spiv, w,
+                     std::make_integer_sequence<int, W>{});
   // the leaf's interchanges and row movement (what lu_leaf_ipiv does on the
   // other path), by wave 0 from the pivots in LDS
   __syncthreads();
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(kLuWgThreads) lu_leaf_wg(double* A, int64_t ld
     const int r = c + t + kLuWgThreads * k;
     if (r < n) {
 #pragma unroll
-      for (int jj = 0; jj < kLuWgLeaf; ++jj)
+      for (int jj = 0; jj < W; ++jj)
         if (jj < nc) A[r + (int64_t)(c + jj) * lda] = vals[k][jj];
     }
   }
@@ -462,19 +462,21 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
       // the trailing leaves, once their rows fit one workgroup, in one launch each
       const int rows = n - cl;
       const bool one_wg = rows <= kLuWgRowsMax;
-      const int cend = std::min(kend, cl + (one_wg ? kLuWgLeaf : kLuLeaf));
+      const int nr = (rows + kLuWgThreads - 1) / kLuWgThreads;
+      // one row per thread leaves room for kLuLeaf columns, more rows for kLuWgLeaf
+      const int width = !one_wg || nr == 1 ? kLuLeaf : kLuWgLeaf;
+      const int cend = std::min(kend, cl + width);
       const int nc = cend - cl;
       if (one_wg) {
-        const int nr = (rows + kLuWgThreads - 1) / kLuWgThreads;
         if (nr == 1)
-          hipLaunchKernelGGL(lu_leaf_wg<1>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
-                             w);
+          hipLaunchKernelGGL((lu_leaf_wg<1, kLuLeaf>), dim3(1), dim3(kLuWgThreads), 0, s, dA, lda,
+                             n, cl, cend, w);
         else if (nr == 2)
-          hipLaunchKernelGGL(lu_leaf_wg<2>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
-                             w);
+          hipLaunchKernelGGL((lu_leaf_wg<2, kLuWgLeaf>), dim3(1), dim3(kLuWgThreads), 0, s, dA,
+                             lda, n, cl, cend, w);
         else
-          hipLaunchKernelGGL(lu_leaf_wg<3>, dim3(1), dim3(kLuWgThreads), 0, s, dA, lda, n, cl, cend,
-                             w);
+          hipLaunchKernelGGL((lu_leaf_wg<3, kLuWgLeaf>), dim3(1), dim3(kLuWgThreads), 0, s, dA,
+                             lda, n, cl, cend, w);
         // (the kernel converts its pivots itself)
       } else {
         const int G = (n - cl + kLuRows - 1) / kLuRows;
