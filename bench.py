@@ -488,6 +488,58 @@ def run_e2e(args, pkg) -> dict:
                  kernel_ms={"psx_mcm_set": ks["psx_mcm_set"]})
 
 
+def run_master(args, pkg) -> dict:
+    """master (src/modecoupling.jl:317-341) end to end at the headline size:
+    two (I, Q, U) nside-1024 maps and their T and P masks to the nine decoupled
+    spectra at lmax 3071.  The pieces are:
+    - the GPU transforms (spin 2 for the maps, spin 0 for the masks);
+    - the five-block mode-coupling pass;
+    - the LU solves (five n = 3072 and two 2n = 6144 systems);
+    - alm2cl and the block assembly on the host, as in the reference."""
+    from powerspectra_amd.alm import alm_index
+    from powerspectra_amd.masks import config_masks
+    from powerspectra_amd.master import master
+    ctx = pkg.default_context()
+    nside = 1024
+    lmax = 3 * nside - 1
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    # two skies: Gaussian (T, E, B) alms with C_l = (l + 10)^-2.5 (E x 0.1, B x 0.01);
+    # map 2 = map 1 + a 10 % independent part
+    ls = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
+    assert ls.size == nalm and alm_index(lmax, lmax, lmax) == nalm - 1
+    cl = (ls + 10.0) ** -2.5
+    rng = np.random.default_rng(9)
+
+    def sky():
+        a = (rng.standard_normal((3, nalm)) + 1j * rng.standard_normal((3, nalm))) / np.sqrt(2)
+        a[:, : lmax + 1] = a[:, : lmax + 1].real * np.sqrt(2)   # m = 0 real
+        return a * np.sqrt(cl) * np.array([1.0, 0.1, 0.01])[:, None] ** 0.5
+
+    s1 = sky()
+    s2 = s1 + 0.1 * sky()
+    m1, m2 = ctx.alm2map_pol(np.stack([s1, s2]), nside, lmax)
+    maskT1, maskP1, maskT2, maskP2 = config_masks(nside, 200, (0, 1, 2, 3))
+    call = lambda: master(tuple(m1), maskT1, maskP1, tuple(m2), maskT2, maskP2, lmin=2, ctx=ctx)
+    out = call()  # warm: plans, tables, tile lists
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = call()
+    dt = (time.perf_counter() - t0) / steps
+    tt = np.asarray(out["TT"].parent)
+    band = slice(100 - 2, 2000 - 2)   # l = 100 .. 1999 (the spectra start at lmin = 2)
+    ratio = float(np.median(tt[band] / cl[:lmax + 1][100:2000]))
+    return _line("master: nine decoupled spectra of two (I, Q, U) nside-1024 maps at lmax 3071",
+                 1.0 / dt, "calls/s", args, dt * 1e3,
+                 {"workload": "master_nside1024_lmax3071: map2alm_pol of two maps, map2alm of "
+                              "four masks, alm2cl, the five-block mode-coupling pass, five "
+                              "n = 3072 and two n = 6144 LU solves", "nside": nside,
+                  "lmax": lmax},
+                 data="synthetic skies (Gaussian alms, C_l = (l + 10)^-2.5, E x 0.1, B x 0.01) "
+                      "and seeded nside-1024 cut-sky masks (seeds 0-3)",
+                 check={"tt_over_input_median_l100_2000": ratio})
+
+
 def serve_requests(n_req: int, seed: int, lmin_: int = 511, lmax_: int = 4095):
     """The serving stream: each request is an MCM (spin pair TT, TE, EE or BB;
     EE and BB need the ++ and -- blocks) or a covariance block (TTTT, EEEE,
@@ -648,7 +700,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
                     choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
-                             "sht_pol", "quickpol"],
+                             "sht_pol", "quickpol", "master"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -664,7 +716,7 @@ def main() -> int:
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
               "e2e": run_e2e, "serve": run_serve, "sht": run_sht, "sht_pol": run_sht_pol,
-              "quickpol": run_quickpol}[args.workload]
+              "quickpol": run_quickpol, "master": run_master}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]

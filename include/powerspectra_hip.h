@@ -206,6 +206,13 @@ int psx_alm2map(psx_ctx* ctx, int nside, int lmax, int nmaps, const double* alms
  * psx_map2alm. */
 int psx_mask_spectra(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
                      const double* maps, double* cls);
+/* Healpix.jl alm2cl of given coefficient sets, as maskedalm2spectra applies it
+ * to the masked maps' and the masks' alms (reference
+ * src/modecoupling.jl:355,363-364): alms[i] points at nalm complex a_lm of set i
+ * (1..8 sets, layout as psx_map2alm, each set anywhere in host memory), and
+ * cls[(i nsets + j)(lmax+1) + l] = alm2cl(a_i, a_j)[l] for every i, j, summed
+ * as in psx_mask_spectra.  lmax 0..8191. */
+int psx_alm2cl(psx_ctx* ctx, int lmax, int nsets, const double* const* alms, double* cls);
 /* Polarised analysis (Healpix.jl map2alm of a PolarizedHealpixMap, as master
  * calls it, reference src/modecoupling.jl:328-329): per map the I, Q, U rings
  * (3 x 12 nside^2 doubles, consecutive) into T, E, B a_lm (3 x nalm complex,

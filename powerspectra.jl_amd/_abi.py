@@ -97,6 +97,7 @@ SIGNATURES = {
     "psx_quickpol_xi": (_int, [_vp] + [_int] * 7 + [_dp, _i64, _dp]),
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
+    "psx_alm2cl": (_int, [_vp, _int, _int, ctypes.POINTER(_vp), _dp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map_pol": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
@@ -339,6 +340,21 @@ class Context:
         out = np.empty((n, n, lmax + 1))
         self.check(self.lib.psx_mask_spectra(self._h, nside, lmax, niter, n, dptr(M),
                                              dptr(out)), "psx_mask_spectra")
+        return out
+
+    def alm2cl(self, alms, lmax: int) -> np.ndarray:
+        """cl[i, j, l] = alm2cl(a_i, a_j)[l] of 1..8 coefficient sets (complex,
+        m-major, nalm each), summed on the device; the sets are not copied on
+        the host when they are already contiguous complex128."""
+        nalm = (lmax + 1) * (lmax + 2) // 2
+        sets = [np.ascontiguousarray(a, dtype=np.complex128) for a in alms]
+        for a in sets:
+            if a.ndim != 1 or a.shape[0] != nalm:
+                raise ArgumentError("alm length does not match lmax")
+        n = len(sets)
+        ptrs = (_vp * max(n, 1))(*[a.ctypes.data for a in sets])
+        out = np.empty((n, n, lmax + 1))
+        self.check(self.lib.psx_alm2cl(self._h, lmax, n, ptrs, dptr(out)), "psx_alm2cl")
         return out
 
     def map2alm_pol(self, iqu, nside: int, lmax: int, niter: int = 3) -> np.ndarray:

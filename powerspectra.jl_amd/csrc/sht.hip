@@ -1536,6 +1536,28 @@ int psx_mask_spectra(psx_ctx* h, int nside, int lmax, int niter, int nmaps, cons
   });
 }
 
+int psx_alm2cl(psx_ctx* h, int lmax, int nsets, const double* const* alms, double* cls) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(alms && cls, PSX_ERR_ARGUMENT, "bad arguments");
+    PSX_REQUIRE(nsets >= 1 && nsets <= kMaxSpectraMaps, PSX_ERR_ARGUMENT,
+                "cross-spectra of 1..8 coefficient sets per call");
+    PSX_REQUIRE(lmax >= 0 && lmax <= 8191, PSX_ERR_ARGUMENT, "lmax must be 0..8191");
+    for (int i = 0; i < nsets; ++i) PSX_REQUIRE(alms[i], PSX_ERR_ARGUMENT, "null alm pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    const int64_t ncl = (int64_t)nsets * nsets * (lmax + 1);
+    const size_t ab = 2 * sizeof(double) * nalm * nsets, cb = sizeof(double) * ncl;
+    XferBuf x(c, xfer_bytes({ab, cb}));
+    double* da = x.take(ab);
+    double* dc = x.take(cb);
+    for (int i = 0; i < nsets; ++i)
+      h2d_matrix(c, da + 2 * nalm * i, 2 * nalm, alms[i], 2 * nalm, 2 * nalm, 1);
+    cross_spectra_dev(c, lmax, nsets, da, dc);
+    d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
+  });
+}
+
 int psx_map2alm_pol(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const double* iqu,
                     double* teb) {
   Ctx* c = reinterpret_cast<Ctx*>(h);

@@ -16,10 +16,47 @@ from .modecoupling import mcm_set
 from .spectralarray import SpectralArray, SpectralVector
 
 
-def _pcl(a: Alm, b: Alm, lmin: int, lmax: int) -> SpectralArray:
-    """SpectralVector(alm2cl(a, b))[IdentityRange(lmin:end)] (src/modecoupling.jl:360)."""
-    cl = alm2cl(a, b)[: lmax + 1]
-    return SpectralVector(cl[lmin:], lmin - 1)
+def _cross_spectra(sets, ctx: Context):
+    """Function (i, j) -> alm2cl(sets[i], sets[j]) over a list of Alm.  Sets
+    that are the same object are transformed once; sets of one lmax with
+    mmax = lmax go through one device call (psx_alm2cl)."""
+    uniq, slot = [], []
+    for a in sets:
+        k = next((u for u, b in enumerate(uniq) if b is a), None)
+        if k is None:
+            uniq.append(a)
+            k = len(uniq) - 1
+        slot.append(k)
+    if len({(a.lmax, a.mmax) for a in uniq}) == 1 and uniq[0].mmax == uniq[0].lmax:
+        cl = ctx.alm2cl([a.alm for a in uniq], uniq[0].lmax)
+        return lambda i, j: cl[slot[i], slot[j]]
+    return lambda i, j: alm2cl(sets[i], sets[j])
+
+
+def _decoupled(W, P, lmin: int, lmax: int, ctx: Context) -> dict:
+    """The nine spectra of maskedalm2spectra (src/modecoupling.jl:343-380) from
+    the mask cross-spectra W(a, b) (a in "T1", "P1"; b in "T2", "P2") and the
+    pseudo-spectra P(X, Y) of map 1's X with map 2's Y."""
+    MTT, MTE, MET, Mpp, Mmm = mcm_set(W("T1", "T2")[: lmax + 1], W("T1", "P2")[: lmax + 1],
+                                      W("P1", "T2")[: lmax + 1], W("P1", "P2")[: lmax + 1],
+                                      lmin=lmin, lmax=lmax, ctx=ctx)
+    # SpectralVector(alm2cl(a, b))[IdentityRange(lmin:end)] (src/modecoupling.jl:355)
+    pcl = lambda XY: SpectralVector(np.ascontiguousarray(P(XY[0], XY[1])[lmin: lmax + 1]),
+                                    lmin - 1)
+    out = {"TT": solve(MTT, pcl("TT"), ctx=ctx)}
+    # TE and TB share mcm(TE) = mcm(TB), ET and BT share mcm(ET): one
+    # factorisation each, the two pseudo-spectra as two right-hand sides
+    for M, pair in ((MTE, ("TE", "TB")), (MET, ("ET", "BT"))):
+        b = [pcl(XY) for XY in pair]
+        X = solve(M, SpectralArray(np.column_stack([v.parent for v in b]), firsts=(lmin, 1)),
+                  ctx=ctx)
+        for k, XY in enumerate(pair):
+            out[XY] = SpectralArray(np.ascontiguousarray(X.parent[:, k]), firsts=(lmin,))
+    M_EE_BB = hvcat([[Mpp, Mmm], [Mmm, Mpp]])
+    M_EB_BE = hvcat([[Mpp, -Mmm], [-Mmm, Mpp]])
+    out["EE"], out["BB"] = spectra(solve(M_EE_BB, block_vector(pcl("EE"), pcl("BB")), ctx=ctx))
+    out["EB"], out["BE"] = spectra(solve(M_EB_BE, block_vector(pcl("EB"), pcl("BE")), ctx=ctx))
+    return out
 
 
 def maskedalm2spectra(maskedmap1, maskT1: Alm, maskP1: Alm, maskedmap2, maskT2: Alm,
@@ -34,41 +71,60 @@ def maskedalm2spectra(maskedmap1, maskT1: Alm, maskP1: Alm, maskedmap2, maskT2: 
     T1, E1, B1 = maskedmap1
     T2, E2, B2 = maskedmap2
     lmax = min(a.lmax for a in (T1, E1, B1, T2, E2, B2, maskT1, maskP1, maskT2, maskP2))
-    W = lambda a, b: alm2cl(a, b)[: lmax + 1]
-    MTT, MTE, MET, Mpp, Mmm = mcm_set(W(maskT1, maskT2), W(maskT1, maskP2), W(maskP1, maskT2),
-                                      W(maskP1, maskP2), lmin=lmin, lmax=lmax, ctx=ctx)
-    first = dict(T=(T1, T2), E=(E1, E2), B=(B1, B2))
-    blocks = {"TT": MTT, "TE": MTE, "ET": MET, "TB": MTE, "BT": MET}
-    out = {}
-    for XY, M in blocks.items():
-        pcl = _pcl(first[XY[0]][0], first[XY[1]][1], lmin, lmax)
-        out[XY] = solve(M, pcl, ctx=ctx)
-    M_EE_BB = hvcat([[Mpp, Mmm], [Mmm, Mpp]])
-    M_EB_BE = hvcat([[Mpp, -Mmm], [-Mmm, Mpp]])
-    out["EE"], out["BB"] = spectra(solve(M_EE_BB, block_vector(_pcl(E1, E2, lmin, lmax),
-                                                              _pcl(B1, B2, lmin, lmax)), ctx=ctx))
-    out["EB"], out["BE"] = spectra(solve(M_EB_BE, block_vector(_pcl(E1, B2, lmin, lmax),
-                                                              _pcl(B1, E2, lmin, lmax)), ctx=ctx))
-    return out
+    masks = _cross_spectra([maskT1, maskP1, maskT2, maskP2], ctx)
+    maps = _cross_spectra([T1, E1, B1, T2, E2, B2], ctx)
+    wi = {"T1": 0, "P1": 1, "T2": 2, "P2": 3}
+    pi = {"T": 0, "E": 1, "B": 2}
+    return _decoupled(lambda a, b: masks(wi[a], wi[b]), lambda X, Y: maps(pi[X], 3 + pi[Y]),
+                      lmin, lmax, ctx)
 
 
 def master(map1, maskT1, maskP1, map2, maskT2, maskP2, *, already_masked: bool = False,
            lmin: int = 0, transform=None, ctx: Context | None = None) -> dict:
     """Decoupled spectra of two polarised maps (I, Q, U) with their masks
     (src/modecoupling.jl:317-341).  `transform` = (pol_map2alm, map2alm):
-    the polarised and scalar analyses to use (defaults: the GPU transforms,
-    spin 0 and 2 for the maps, spin 0 for the masks)."""
-    from .sht import HealpixMap, map2alm, map2alm_pol
-    pol, scal = transform if transform is not None else (map2alm_pol, map2alm)
+    the polarised and scalar analyses to use (default: the GPU transforms at
+    lmax = 3 nside - 1 with three refinements, the masked maps analysed in one
+    psx_map2alm_pol call, their spectra summed on the device, and the masks
+    -- each distinct one once -- through psx_mask_spectra)."""
+    from .sht import HealpixMap, map2alm, map2alm_pol, nside2lmax
     px = lambda m: np.asarray(m.pixels if isinstance(m, HealpixMap) else m, dtype=np.float64)
-    I1, Q1, U1 = (px(m) for m in map1)
-    I2, Q2, U2 = (px(m) for m in map2)
-    if not already_masked:
-        I1, Q1, U1 = I1 * px(maskT1), Q1 * px(maskP1), U1 * px(maskP1)
-        I2, Q2, U2 = I2 * px(maskT2), Q2 * px(maskP2), U2 * px(maskP2)
-    return maskedalm2spectra(pol(I1, Q1, U1), scal(px(maskT1)), scal(px(maskP1)),
-                             pol(I2, Q2, U2), scal(px(maskT2)), scal(px(maskP2)),
-                             lmin=lmin, ctx=ctx)
+    mT1, mP1, mT2, mP2 = maskT1, maskP1, maskT2, maskP2
+    if transform is not None:
+        pol, scal = transform
+        I1, Q1, U1 = (px(m) for m in map1)
+        I2, Q2, U2 = (px(m) for m in map2)
+        if not already_masked:
+            I1, Q1, U1 = I1 * px(mT1), Q1 * px(mP1), U1 * px(mP1)
+            I2, Q2, U2 = I2 * px(mT2), Q2 * px(mP2), U2 * px(mP2)
+        return maskedalm2spectra(pol(I1, Q1, U1), scal(px(mT1)), scal(px(mP1)),
+                                 pol(I2, Q2, U2), scal(px(mT2)), scal(px(mP2)),
+                                 lmin=lmin, ctx=ctx)
+    ctx = ctx or default_context()
+    nside = HealpixMap(px(map1[0])).nside
+    npix = 12 * nside * nside
+    lmax = nside2lmax(nside)
+    iqu = np.empty((2, 3, npix))
+    for i, (m, mT, mP) in enumerate(((map1, mT1, mP1), (map2, mT2, mP2))):
+        for k, (x, w) in enumerate(zip(m, (mT, mP, mP))):
+            if already_masked:
+                iqu[i, k] = px(x)
+            else:
+                np.multiply(px(x), px(w), out=iqu[i, k])
+    teb = ctx.map2alm_pol(iqu, nside, lmax, 3)
+    del iqu
+    P = ctx.alm2cl([teb[0, 0], teb[0, 1], teb[0, 2], teb[1, 0], teb[1, 1], teb[1, 2]], lmax)
+    uniq, slot = [], {}
+    for name, w in (("T1", mT1), ("P1", mP1), ("T2", mT2), ("P2", mP2)):
+        k = next((u for u, v in enumerate(uniq) if v is w), None)
+        if k is None:
+            uniq.append(w)
+            k = len(uniq) - 1
+        slot[name] = k
+    W = ctx.mask_spectra(np.stack([px(w) for w in uniq]), nside, lmax, 3)
+    pi = {"T": 0, "E": 1, "B": 2}
+    return _decoupled(lambda a, b: W[slot[a], slot[b]], lambda X, Y: P[pi[X], 3 + pi[Y]],
+                      lmin, lmax, ctx)
 
 
 def binning_matrix(left_bins, right_bins, weight_function, lmax: int | None = None) -> np.ndarray:

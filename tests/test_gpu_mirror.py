@@ -213,3 +213,32 @@ def test_master_full_sky_is_identity(pkg):
                        ("EB", (E, B))):
         want = pkg.alm2cl(a, b)[2:]
         assert np.abs(out[XY].parent - want).max() <= 1e-9 * np.abs(want).max()
+
+
+def test_master_default_path_matches_alm_route(pkg, ctx, masks):
+    """master's default path (both masked maps in one polarised analysis, their
+    spectra summed on the device, each distinct mask transformed once) against
+    the same GPU transforms called one by one through maskedalm2spectra; maps
+    masked beforehand (already_masked) give the same spectra bit for bit."""
+    from powerspectra_amd.master import maskedalm2spectra, master
+    from powerspectra_amd.sht import map2alm, map2alm_pol
+    mT, mP = (np.asarray(m.pixels) for m in masks)
+    npix = 12 * NSIDE * NSIDE
+    rng = np.random.default_rng(21)
+    map1 = [rng.standard_normal(npix) for _ in range(3)]
+    map2 = [x + 0.3 * rng.standard_normal(npix) for x in map1]
+    # the P mask of map 2 is map 1's T mask: one object, transformed once
+    out = master(map1, mT, mP, map2, mP, mT, lmin=2, ctx=ctx)
+    a1 = map2alm_pol(map1[0] * mT, map1[1] * mP, map1[2] * mP, ctx=ctx)
+    a2 = map2alm_pol(map2[0] * mP, map2[1] * mT, map2[2] * mT, ctx=ctx)
+    ref = maskedalm2spectra(a1, map2alm(mT, ctx=ctx), map2alm(mP, ctx=ctx), a2,
+                            map2alm(mP, ctx=ctx), map2alm(mT, ctx=ctx), lmin=2, ctx=ctx)
+    pre = master([map1[0] * mT, map1[1] * mP, map1[2] * mP], mT, mP,
+                 [map2[0] * mP, map2[1] * mT, map2[2] * mT], mP, mT,
+                 already_masked=True, lmin=2, ctx=ctx)
+    assert sorted(out) == sorted(ref) == ["BB", "BE", "BT", "EB", "EE", "ET", "TB", "TE", "TT"]
+    for k in out:
+        assert out[k].firstindex() == 2 and out[k].lastindex() == 3 * NSIDE - 1
+        want = ref[k].parent
+        assert np.abs(out[k].parent - want).max() <= 1e-10 * np.abs(want).max(), k
+        assert np.array_equal(pre[k].parent, out[k].parent), k

@@ -251,3 +251,38 @@ def test_mask_spectra_config_masks_and_limits(pkg, ctx):
     assert np.array_equal(W[(0, 1)], W[(1, 0)])
     with pytest.raises(pkg.ArgumentError):
         ctx.mask_spectra(np.zeros((9, 12 * 4 * 4)), 4, 8, 0)
+
+
+@pytest.mark.parametrize("lmax,nsets", [(0, 1), (5, 2), (100, 6), (767, 8)])
+def test_alm2cl_device_vs_host(pkg, ctx, lmax, nsets):
+    """psx_alm2cl (all cross-spectra of coefficient sets anywhere in host
+    memory) against the host alm2cl of the same sets; a set passed twice gives
+    its auto-spectrum, bit for bit, in all four entries."""
+    rng = np.random.default_rng(lmax + nsets)
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    alms = [pkg.Alm(lmax, lmax, rng.standard_normal(nalm) + 1j * rng.standard_normal(nalm))
+            for _ in range(nsets)]
+    C = ctx.alm2cl([a.alm for a in alms], lmax)
+    assert C.shape == (nsets, nsets, lmax + 1)
+    for i in range(nsets):
+        for j in range(nsets):
+            want = pkg.alm2cl(alms[i], alms[j])
+            assert np.abs(C[i, j] - want).max() <= 1e-13 * np.abs(want).max(), (i, j)
+    D = ctx.alm2cl([alms[0].alm, alms[0].alm], lmax)
+    for i in range(2):
+        for j in range(2):
+            assert np.array_equal(D[i, j], C[0, 0])
+
+
+def test_alm2cl_device_limits(pkg, ctx):
+    """More than 8 sets, no set, a length that does not match lmax and a
+    negative lmax are refused."""
+    a = np.zeros(6, dtype=np.complex128)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.alm2cl([a] * 9, 2)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.alm2cl([], 2)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.alm2cl([a[:5]], 2)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.alm2cl([a[:0]], -1)
