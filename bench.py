@@ -531,9 +531,11 @@ def run_master(args, pkg) -> dict:
     ratio = float(np.median(tt[band] / cl[:lmax + 1][100:2000]))
     return _line("master: nine decoupled spectra of two (I, Q, U) nside-1024 maps at lmax 3071",
                  1.0 / dt, "calls/s", args, dt * 1e3,
-                 {"workload": "master_nside1024_lmax3071: map2alm_pol of two maps, map2alm of "
-                              "four masks, alm2cl, the five-block mode-coupling pass, five "
-                              "n = 3072 and two n = 6144 LU solves", "nside": nside,
+                 {"workload": "master_nside1024_lmax3071: psx_master_spectra (masking, the "
+                              "polarised analysis of both maps, the analysis of four masks, "
+                              "their spectra), the five-block mode-coupling pass, three "
+                              "n = 3070 LU solves (TE/TB and ET/BT two right-hand sides each) "
+                              "and two n = 6140", "nside": nside,
                   "lmax": lmax},
                  data="synthetic skies (Gaussian alms, C_l = (l + 10)^-2.5, E x 0.1, B x 0.01) "
                       "and seeded nside-1024 cut-sky masks (seeds 0-3)",

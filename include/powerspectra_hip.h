@@ -213,6 +213,22 @@ int psx_mask_spectra(psx_ctx* ctx, int nside, int lmax, int niter, int nmaps,
  * cls[(i nsets + j)(lmax+1) + l] = alm2cl(a_i, a_j)[l] for every i, j, summed
  * as in psx_mask_spectra.  lmax 0..8191. */
 int psx_alm2cl(psx_ctx* ctx, int lmax, int nsets, const double* const* alms, double* cls);
+/* The spectra `master` decouples (reference src/modecoupling.jl:320-330: the
+ * masking and map2alm calls; the alm2cl of mcm's masks, :197, and of the
+ * pseudo-spectra in maskedalm2spectra, :355,363-364), from two polarised maps
+ * and their masks in one call: maps[0..5] = I1, Q1, U1, I2, Q2, U2 and
+ * masks[0..3] = T1, P1, T2, P2, ring order, 12 nside^2 doubles each, anywhere
+ * in host memory (a mask address given twice is copied and transformed once).
+ * On the device: I times its T mask and Q, U times the P mask (skipped when
+ * already_masked != 0), the polarised analysis of both maps and the scalar
+ * analysis of the masks (niter refinements each, as psx_map2alm_pol and
+ * psx_map2alm), and their cross-spectra, summed as in psx_mask_spectra:
+ *   pcl[(x 3 + y)(lmax+1) + l] = alm2cl(X of map 1, Y of map 2)[l], x, y over T, E, B;
+ *   W[(a 2 + b)(lmax+1) + l]   = alm2cl(mask a of map 1, mask b of map 2)[l],
+ *                                a, b over T, P.
+ * Only the spectra come back to the host.  Limits as psx_map2alm. */
+int psx_master_spectra(psx_ctx* ctx, int nside, int lmax, int niter, const double* const* maps,
+                       const double* const* masks, int already_masked, double* pcl, double* W);
 /* Polarised analysis (Healpix.jl map2alm of a PolarizedHealpixMap, as master
  * calls it, reference src/modecoupling.jl:328-329): per map the I, Q, U rings
  * (3 x 12 nside^2 doubles, consecutive) into T, E, B a_lm (3 x nalm complex,

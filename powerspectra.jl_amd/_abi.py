@@ -98,6 +98,8 @@ SIGNATURES = {
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2cl": (_int, [_vp, _int, _int, ctypes.POINTER(_vp), _dp]),
+    "psx_master_spectra": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_vp), _int, _dp, _dp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2map_pol": (_int, [_vp, _int, _int, _int, _dp, _dp]),
     "psx_server_create": (_int, [_vp, _int, ctypes.c_size_t, _int, ctypes.POINTER(_vp)]),
@@ -356,6 +358,28 @@ class Context:
         out = np.empty((n, n, lmax + 1))
         self.check(self.lib.psx_alm2cl(self._h, lmax, n, ptrs, dptr(out)), "psx_alm2cl")
         return out
+
+    def master_spectra(self, maps, masks, nside: int, lmax: int, niter: int = 3,
+                       already_masked: bool = False):
+        """(pcl[x, y, l], W[a, b, l]): the pseudo-spectra of map 1's x in (T, E, B)
+        with map 2's y, and the spectra of mask a in (T, P) of map 1 with mask b
+        of map 2, from maps = (I1, Q1, U1, I2, Q2, U2) and masks = (T1, P1, T2,
+        P2), all on the device (psx_master_spectra)."""
+        npix = 12 * nside * nside
+        M = [np.ascontiguousarray(m, dtype=np.float64) for m in maps]
+        K = [np.ascontiguousarray(m, dtype=np.float64) for m in masks]
+        if len(M) != 6 or len(K) != 4:
+            raise ArgumentError("six map fields and four masks are required")
+        if any(a.ndim != 1 or a.size != npix for a in M + K):
+            raise ArgumentError("maps and masks must have 12 nside^2 pixels")
+        mp = (_vp * 6)(*[a.ctypes.data for a in M])
+        kp = (_vp * 4)(*[a.ctypes.data for a in K])
+        pcl = np.empty((3, 3, lmax + 1))
+        W = np.empty((2, 2, lmax + 1))
+        self.check(self.lib.psx_master_spectra(self._h, nside, lmax, niter, mp, kp,
+                                               int(bool(already_masked)), dptr(pcl), dptr(W)),
+                   "psx_master_spectra")
+        return pcl, W
 
     def map2alm_pol(self, iqu, nside: int, lmax: int, niter: int = 3) -> np.ndarray:
         """(T, E, B) a_lm, shape (nmaps, 3, nalm), of (I, Q, U) ring-ordered

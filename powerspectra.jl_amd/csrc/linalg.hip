@@ -3,8 +3,9 @@
 // (src/covariance.jl:8-14).  Julia's `lu` is LAPACK getrf with partial
 // pivoting and throws SingularException on an exact zero pivot.  The
 // factorisation is this file's blocked getrf (HIP leaf kernels + rocBLAS
-// updates), the triangular solves rocSOLVER's getrs on its LAPACK-layout
-// output.
+// updates) in LAPACK layout; the solves apply its interchanges (a gather for
+// many columns, rocSOLVER's laswp for a few) and run this file's forward and
+// back substitution (rocBLAS updates between diagonal blocks).
 #include "psx_internal.h"
 
 #include <utility>

@@ -944,6 +944,13 @@ __global__ void __launch_bounds__(256) residual(const double* __restrict__ a,
   if (i < n) out[i] = a[i] - b[i];
 }
 
+// x[i] *= w[i]: a map times its mask, in place
+__global__ void __launch_bounds__(256) apply_mask(double* __restrict__ x,
+                                                  const double* __restrict__ w, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= w[i];
+}
+
 // The polar DFTs with LDS tables sized for the longest ring of the grid
 // (4 nside - 4 pixels): 64 KB for nside <= 1024, 128 KB up to 2048.
 void launch_polar_dft(int ns, hipStream_t s, RingArgs ra, const double* map, double2* X) {
@@ -1555,6 +1562,66 @@ int psx_alm2cl(psx_ctx* h, int lmax, int nsets, const double* const* alms, doubl
       h2d_matrix(c, da + 2 * nalm * i, 2 * nalm, alms[i], 2 * nalm, 2 * nalm, 1);
     cross_spectra_dev(c, lmax, nsets, da, dc);
     d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
+  });
+}
+
+int psx_master_spectra(psx_ctx* h, int nside, int lmax, int niter, const double* const* maps,
+                       const double* const* masks, int already_masked, double* pcl, double* W) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  return guarded(c, [&] {
+    PSX_REQUIRE(maps && masks && pcl && W && niter >= 0, PSX_ERR_ARGUMENT, "bad arguments");
+    for (int i = 0; i < 6; ++i) PSX_REQUIRE(maps[i], PSX_ERR_ARGUMENT, "null map pointer");
+    for (int a = 0; a < 4; ++a) PSX_REQUIRE(masks[a], PSX_ERR_ARGUMENT, "null mask pointer");
+    check_sht_args(nside, lmax);
+    PSX_HIP(hipSetDevice(c->device));
+    const int64_t npix = 12LL * nside * nside;
+    const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+    const int64_t L = lmax + 1;
+    // distinct masks (by address), each copied and transformed once
+    const double* uniq[4];
+    int slot[4], nu = 0;
+    for (int a = 0; a < 4; ++a) {
+      int k = 0;
+      while (k < nu && uniq[k] != masks[a]) ++k;
+      if (k == nu) uniq[nu++] = masks[a];
+      slot[a] = k;
+    }
+    const size_t mb = sizeof(double) * npix * 6, wb = sizeof(double) * npix * nu,
+                 ab = 2 * sizeof(double) * nalm * 6, wab = 2 * sizeof(double) * nalm * nu,
+                 cb = sizeof(double) * 36 * L, wcb = sizeof(double) * nu * nu * L;
+    XferBuf x(c, xfer_bytes({mb, wb, ab, wab, cb, wcb}));
+    double* dm = x.take(mb);
+    double* dw = x.take(wb);
+    double* da = x.take(ab);
+    double* dwa = x.take(wab);
+    double* dc = x.take(cb);
+    double* dwc = x.take(wcb);
+    for (int i = 0; i < 6; ++i) h2d_matrix(c, dm + i * npix, npix, maps[i], npix, npix, 1);
+    for (int k = 0; k < nu; ++k) h2d_matrix(c, dw + k * npix, npix, uniq[k], npix, npix, 1);
+    if (!already_masked) {
+      // map m's I by its T mask, Q and U by its P mask
+      for (int i = 0; i < 6; ++i) {
+        const int a = 2 * (i / 3) + (i % 3 == 0 ? 0 : 1);
+        hipLaunchKernelGGL(apply_mask, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+                           c->stream, dm + i * npix, (const double*)(dw + slot[a] * npix), npix);
+        PSX_HIP(hipGetLastError());
+      }
+    }
+    map2alm_pol_dev(c, nside, lmax, niter, 2, dm, da);
+    map2alm_dev(c, nside, lmax, niter, nu, dw, dwa);
+    cross_spectra_dev(c, lmax, 6, da, dc);
+    cross_spectra_dev(c, lmax, nu, dwa, dwc);
+    std::vector<double> hc(36 * L), hw((size_t)nu * nu * L);
+    d2h_matrix(c, hc.data(), 36 * L, dc, 36 * L, 36 * L, 1);
+    d2h_matrix(c, hw.data(), (int64_t)nu * nu * L, dwc, (int64_t)nu * nu * L,
+               (int64_t)nu * nu * L, 1);
+    // map 1's X with map 2's Y; mask a of map 1 with mask b of map 2
+    for (int xx = 0; xx < 3; ++xx)
+      for (int yy = 0; yy < 3; ++yy)
+        std::copy_n(hc.data() + (xx * 6 + 3 + yy) * L, L, pcl + (xx * 3 + yy) * L);
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        std::copy_n(hw.data() + (slot[a] * nu + slot[2 + b]) * L, L, W + (a * 2 + b) * L);
   });
 }
 

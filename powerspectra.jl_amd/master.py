@@ -84,10 +84,10 @@ def master(map1, maskT1, maskP1, map2, maskT2, maskP2, *, already_masked: bool =
     """Decoupled spectra of two polarised maps (I, Q, U) with their masks
     (src/modecoupling.jl:317-341).  `transform` = (pol_map2alm, map2alm):
     the polarised and scalar analyses to use (default: the GPU transforms at
-    lmax = 3 nside - 1 with three refinements, the masked maps analysed in one
-    psx_map2alm_pol call, their spectra summed on the device, and the masks
-    -- each distinct one once -- through psx_mask_spectra)."""
-    from .sht import HealpixMap, map2alm, map2alm_pol, nside2lmax
+    lmax = 3 nside - 1 with three refinements: masking, both analyses and the
+    spectra in one device call, psx_master_spectra, each distinct mask
+    transformed once)."""
+    from .sht import HealpixMap, nside2lmax
     px = lambda m: np.asarray(m.pixels if isinstance(m, HealpixMap) else m, dtype=np.float64)
     mT1, mP1, mT2, mP2 = maskT1, maskP1, maskT2, maskP2
     if transform is not None:
@@ -102,29 +102,13 @@ def master(map1, maskT1, maskP1, map2, maskT2, maskP2, *, already_masked: bool =
                                  lmin=lmin, ctx=ctx)
     ctx = ctx or default_context()
     nside = HealpixMap(px(map1[0])).nside
-    npix = 12 * nside * nside
-    lmax = nside2lmax(nside)
-    iqu = np.empty((2, 3, npix))
-    for i, (m, mT, mP) in enumerate(((map1, mT1, mP1), (map2, mT2, mP2))):
-        for k, (x, w) in enumerate(zip(m, (mT, mP, mP))):
-            if already_masked:
-                iqu[i, k] = px(x)
-            else:
-                np.multiply(px(x), px(w), out=iqu[i, k])
-    teb = ctx.map2alm_pol(iqu, nside, lmax, 3)
-    del iqu
-    P = ctx.alm2cl([teb[0, 0], teb[0, 1], teb[0, 2], teb[1, 0], teb[1, 1], teb[1, 2]], lmax)
-    uniq, slot = [], {}
-    for name, w in (("T1", mT1), ("P1", mP1), ("T2", mT2), ("P2", mP2)):
-        k = next((u for u, v in enumerate(uniq) if v is w), None)
-        if k is None:
-            uniq.append(w)
-            k = len(uniq) - 1
-        slot[name] = k
-    W = ctx.mask_spectra(np.stack([px(w) for w in uniq]), nside, lmax, 3)
+    pcl, W = ctx.master_spectra([px(x) for x in (*map1, *map2)],
+                                [px(w) for w in (mT1, mP1, mT2, mP2)], nside, nside2lmax(nside),
+                                3, already_masked)
+    wi = {"T1": 0, "P1": 1, "T2": 0, "P2": 1}
     pi = {"T": 0, "E": 1, "B": 2}
-    return _decoupled(lambda a, b: W[slot[a], slot[b]], lambda X, Y: P[pi[X], 3 + pi[Y]],
-                      lmin, lmax, ctx)
+    return _decoupled(lambda a, b: W[wi[a], wi[b]], lambda X, Y: pcl[pi[X], pi[Y]],
+                      lmin, lmax=nside2lmax(nside), ctx=ctx)
 
 
 def binning_matrix(left_bins, right_bins, weight_function, lmax: int | None = None) -> np.ndarray:

@@ -1,4 +1,5 @@
-"""GPU parity: FP64 LU decoupling (rocSOLVER) through the C ABI.
+"""GPU parity: FP64 LU (the library's blocked getrf and substitution, rocBLAS
+updates) and decoupling through the C ABI.
 
 Reference: decouple_covmat (src/covariance.jl:8-14) and `\\`
 (src/blockspectralmatrix.jl:101-129).
@@ -105,6 +106,21 @@ def test_lu_factor_singular(ctx, pkg):
         ctx.lu_factor(np.ones((40, 40)))
 
 
+@pytest.mark.parametrize("n,k", [(2100, 2000), (3100, 5), (4100, 3500), (4100, 700)])
+def test_lu_factor_zero_column_position(ctx, pkg, n, k):
+    """An exactly zero column k stays zero through every update, so the first
+    zero pivot is column k + 1 (LAPACK's info): found by the one-workgroup
+    leaf (n = 2100), by the per-column leaf (n = 3100, k = 5), and by each of
+    them after the other has run (n = 4100).  The solve refuses it too."""
+    rng = np.random.default_rng(n + k)
+    A = rng.standard_normal((n, n))
+    A[:, k] = 0.0
+    with pytest.raises(pkg.SingularException, match=rf"position {k + 1} "):
+        ctx.lu_factor(A)
+    with pytest.raises(pkg.SingularException, match=rf"position {k + 1} "):
+        ctx.lu_solve(A, np.ones(n))
+
+
 def test_lu_factor_structured_pivots(ctx):
     """Pivoting that moves rows across leaves and blocks: a matrix whose
     largest entry of every column sits in the bottom rows (LAPACK swaps it up
@@ -142,9 +158,10 @@ def test_lu_solve_device_buffers(ctx):
 
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 255, 256, 257, 320, 511, 777, 2100])
 def test_lu_solve_substitution_sizes(ctx, n):
-    """Solves with 1-4 right-hand sides run the library's blocked forward and
-    back substitution (256-row steps, 64-row diagonal sub-blocks); 5 go
-    through rocSOLVER's getrs.  Both against LAPACK (scipy lu_solve)."""
+    """Solves with 1-4 right-hand sides run the library's forward and back
+    substitution (128-row diagonal blocks, one launch per block); 5 run its
+    many-right-hand-side triangular solve (64-row diagonal blocks, rocBLAS
+    updates).  Both against LAPACK (scipy lu_solve)."""
     import scipy.linalg as sl
     rng = np.random.default_rng(100 + n)
     A = rng.standard_normal((n, n)) + np.diag(rng.uniform(1.0, 2.0, n)) * np.sqrt(n)

@@ -274,6 +274,40 @@ def test_alm2cl_device_vs_host(pkg, ctx, lmax, nsets):
             assert np.array_equal(D[i, j], C[0, 0])
 
 
+@pytest.mark.parametrize("nside,lmax,repeat", [(8, 20, False), (16, 47, True)])
+def test_master_spectra_vs_separate_calls(pkg, ctx, nside, lmax, repeat):
+    """psx_master_spectra (masking, both analyses and the spectra on the
+    device) against the same steps as separate calls: host products,
+    psx_map2alm_pol, psx_alm2cl and psx_mask_spectra.  A mask address given
+    twice (repeat) is transformed once; masked maps with already_masked give
+    the same result bit for bit."""
+    rng = np.random.default_rng(nside)
+    maps = [rng.standard_normal(12 * nside * nside) for _ in range(6)]
+    masks = [_smooth_map(nside, 60 + s) for s in range(4)]
+    if repeat:
+        masks[3] = masks[0]
+    pcl, W = ctx.master_spectra(maps, masks, nside, lmax, 2)
+    which = (0, 1, 1, 2, 3, 3)
+    masked = [m * masks[which[i]] for i, m in enumerate(maps)]
+    teb = ctx.map2alm_pol(np.stack(masked).reshape(2, 3, -1), nside, lmax, 2)
+    P = ctx.alm2cl([teb[0, 0], teb[0, 1], teb[0, 2], teb[1, 0], teb[1, 1], teb[1, 2]], lmax)
+    C = ctx.mask_spectra(np.stack(masks), nside, lmax, 2)
+    for x in range(3):
+        for y in range(3):
+            want = P[x, 3 + y]
+            assert np.abs(pcl[x, y] - want).max() <= 1e-13 * np.abs(want).max(), (x, y)
+    for a in range(2):
+        for b in range(2):
+            want = C[a, 2 + b]
+            assert np.abs(W[a, b] - want).max() <= 1e-13 * np.abs(want).max(), (a, b)
+    pcl2, W2 = ctx.master_spectra(masked, masks, nside, lmax, 2, already_masked=True)
+    assert np.array_equal(pcl2, pcl) and np.array_equal(W2, W)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.master_spectra(maps[:5], masks, nside, lmax)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.master_spectra(maps, masks, nside, 4 * nside + 1)
+
+
 def test_alm2cl_device_limits(pkg, ctx):
     """More than 8 sets, no set, a length that does not match lmax and a
     negative lmax are refused."""
