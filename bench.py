@@ -535,7 +535,7 @@ def run_master(args, pkg) -> dict:
                               "polarised analysis of both maps, the analysis of four masks, "
                               "their spectra), the five-block mode-coupling pass, three "
                               "n = 3070 LU solves (TE/TB and ET/BT two right-hand sides each) "
-                              "and two n = 6140", "nside": nside,
+                              "and one n = 6140 (EE/BB and EB/BE)", "nside": nside,
                   "lmax": lmax},
                  data="synthetic skies (Gaussian alms, C_l = (l + 10)^-2.5, E x 0.1, B x 0.01) "
                       "and seeded nside-1024 cut-sky masks (seeds 0-3)",

@@ -3,7 +3,9 @@
 Mirrors src/modecoupling.jl:343-380 (maskedalm2spectra) and :317-341
 (master), and src/util.jl:3-18 (binning_matrix).  The five mode-coupling
 blocks come from one fused pass (psx_mcm_set: TT, TE (= TB), ET (= BT), ++,
---), the solves from the GPU LU.
+--), the solves from the GPU LU: four factorisations (TT; TE for TE and TB;
+ET for ET and BT; the 2n x 2n EE/BB matrix for EE/BB and EB/BE) where the
+reference factors seven matrices.
 """
 from __future__ import annotations
 
@@ -11,7 +13,7 @@ import numpy as np
 
 from ._abi import ArgumentError, Context, default_context
 from .alm import Alm, alm2cl
-from .blockspectralmatrix import block_vector, hvcat, solve, spectra
+from .blockspectralmatrix import BlockSpectralMatrix, hvcat, solve
 from .modecoupling import mcm_set
 from .spectralarray import SpectralArray, SpectralVector
 
@@ -52,10 +54,21 @@ def _decoupled(W, P, lmin: int, lmax: int, ctx: Context) -> dict:
                   ctx=ctx)
         for k, XY in enumerate(pair):
             out[XY] = SpectralArray(np.ascontiguousarray(X.parent[:, k]), firsts=(lmin,))
+    # EE/BB and EB/BE are decoupled jointly (src/modecoupling.jl:362-377) by
+    # [M++ M--; M-- M++] and [M++ -M--; -M-- M++].  The second is D A D for the
+    # first, A, with D = diag(I, -I): [EB; BE] solves it exactly when
+    # [EB; -BE] solves A against [pEB; -pBE].  Sign flips of rows and columns
+    # change neither partial pivoting's choices nor any magnitude, so one
+    # factorisation of A, with two right-hand sides, serves both systems.
+    n = lmax - lmin + 1
     M_EE_BB = hvcat([[Mpp, Mmm], [Mmm, Mpp]])
-    M_EB_BE = hvcat([[Mpp, -Mmm], [-Mmm, Mpp]])
-    out["EE"], out["BB"] = spectra(solve(M_EE_BB, block_vector(pcl("EE"), pcl("BB")), ctx=ctx))
-    out["EB"], out["BE"] = spectra(solve(M_EB_BE, block_vector(pcl("EB"), pcl("BE")), ctx=ctx))
+    rhs = np.empty((2 * n, 2), order="F")
+    rhs[:n, 0], rhs[n:, 0] = pcl("EE").parent, pcl("BB").parent
+    rhs[:n, 1], rhs[n:, 1] = pcl("EB").parent, -pcl("BE").parent
+    X = solve(M_EE_BB, BlockSpectralMatrix(rhs, M_EE_BB.n_ells, [range(1, 3)]), ctx=ctx).parent
+    vec = lambda v: SpectralArray(np.ascontiguousarray(v), firsts=(lmin,))
+    out["EE"], out["BB"] = vec(X[:n, 0]), vec(X[n:, 0])
+    out["EB"], out["BE"] = vec(X[:n, 1]), vec(-X[n:, 1])
     return out
 
 
