@@ -469,6 +469,32 @@ def run_quickpol(args, pkg) -> dict:
                                                        "term (sqrt and div count 1)"}})
 
 
+def run_binned(args, pkg) -> dict:
+    """Binned TT mode-coupling matrix P M P^T at lmax (src/util.jl:3-18 bins of
+    width 30 from l = 2, weights l (l + 1)) through psx_mcm_binned, the block
+    kept on the device, beside psx_mcm copying the full block to the host."""
+    ctx = pkg.Context(0)
+    lmax = args.lmax
+    W = synthetic_W(lmax + 1, 1, False)
+    left = np.arange(2, lmax - 28, 30)
+    P = pkg.binning_matrix(left, left + 29, lambda l: l * (l + 1.0), lmax)
+    Q = np.asfortranarray(P.T)
+    P = np.asfortranarray(P)
+    full = np.empty((lmax + 1, lmax + 1), order="F")
+    binned = lambda: ctx.mcm_binned(0, 0, lmax, W, P, Q)
+    dt, _ = _timed(ctx, binned, [], args.steps, args.warmup)
+    dt_full, _ = _timed(ctx, lambda: ctx.mcm(0, 0, lmax, W, out=full), [], args.steps,
+                        args.warmup)
+    err = float(np.abs(binned() - P @ full @ Q).max() / np.abs(P @ full @ Q).max())
+    ctx.close()
+    return _line(f"binned TT mode-coupling matrices per second at lmax {lmax}",
+                 args.steps / dt, "calls/s", args, dt / args.steps * 1e3,
+                 {"workload": f"mcm_binned_lmax{lmax}: TT block, {P.shape[0]} bins of width 30, "
+                              "P M P^T on the device", "lmax": lmax, "bins": int(P.shape[0])},
+                 full_block_with_copy_ms=dt_full / args.steps * 1e3,
+                 check={"max_rel_diff_vs_host_product": err})
+
+
 def run_e2e(args, pkg) -> dict:
     """The headline workload through the host-pointer entry point psx_mcm_set:
     W copied in, five blocks computed and copied out each step (PCIe
@@ -702,7 +728,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
                     choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
-                             "sht_pol", "quickpol", "master"],
+                             "sht_pol", "quickpol", "master", "binned"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -718,7 +744,8 @@ def main() -> int:
     if args.workload != "mcm_set":
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
               "e2e": run_e2e, "serve": run_serve, "sht": run_sht, "sht_pol": run_sht_pol,
-              "quickpol": run_quickpol, "master": run_master}[args.workload]
+              "quickpol": run_quickpol, "master": run_master,
+              "binned": run_binned}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]

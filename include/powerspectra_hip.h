@@ -118,6 +118,15 @@ int psx_mcm(psx_ctx* ctx, int kind, int lmin, int lmax,
 int psx_mcm_d(psx_ctx* ctx, int kind, int lmin, int lmax,
               const double* dW, int64_t LW, double* dM, int64_t ldM);
 
+/* Binned mode-coupling matrix P M Q (the binning of reference src/util.jl:3-18
+ * applied to one block of psx_mcm): P is nbl x n (leading dimension nbl),
+ * Q is n x nbr (leading dimension n), n = lmax-lmin+1, both over l = lmin..lmax;
+ * out is nbl x nbr (leading dimension ldo).  M stays on the device: only the
+ * binned matrix is copied back. */
+int psx_mcm_binned(psx_ctx* ctx, int kind, int lmin, int lmax, const double* W, int64_t LW,
+                   const double* P, int nbl, const double* Q, int nbr, double* out,
+                   int64_t ldo);
+
 /* Fused spin-2 pair: one pass fills M⁺⁺ and M⁻⁻ from the same W (the two
  * reference passes inner_mcm⁺⁺! and inner_mcm⁻⁻! of src/modecoupling.jl:216-217
  * or :238-239 share their (0 -2 2) family). */

@@ -12,7 +12,7 @@ from .blockspectralmatrix import (BlockSpectralMatrix, block_vector, getblock, h
                                   spectra, vcat)
 from .covariance import coupledcov, coupledcovTEEE, decouple_covmat
 from .master import binning_matrix, maskedalm2spectra, master
-from .modecoupling import mask, mcm, mcm_from_spectrum, mcm_set, scale
+from .modecoupling import mask, mcm, mcm_binned, mcm_from_spectrum, mcm_set, scale
 from .sht import HealpixMap, alm2map, alm2map_pol, map2alm, map2alm_pol, nside2lmax, nside2npix
 from .spectralarray import (SpectralArray, SpectralVector, inv, parent, spectralones,
                             spectralzeros)
@@ -26,7 +26,8 @@ __all__ = [
     "SpectralWorkspace", "SpectrumName", "alm2cl", "alm2map", "alm2map_pol", "binning_matrix",
     "block_vector", "coupledcov",
     "coupledcovTEEE", "decouple_covmat", "default_context", "getblock", "hcat", "hvcat", "inv",
-    "map2alm", "map2alm_pol", "mask", "maskedalm2spectra", "master", "mcm", "mcm_from_spectrum",
+    "map2alm", "map2alm_pol", "mask", "maskedalm2spectra", "master", "mcm", "mcm_binned",
+    "mcm_from_spectrum",
     "mcm_set", "nside2lmax", "nside2npix",
     "ku", "parent", "quickpolW", "quickpolXi", "scale", "solve", "spectra", "spectralones", "spectralzeros", "vcat",
 ]

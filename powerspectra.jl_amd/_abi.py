@@ -98,6 +98,8 @@ SIGNATURES = {
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2cl": (_int, [_vp, _int, _int, ctypes.POINTER(_vp), _dp]),
+    "psx_mcm_binned": (_int, [_vp, _int, _int, _int, _dp, _i64, _dp, _int, _dp, _int, _dp,
+                              _i64]),
     "psx_master_spectra": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp),
                                   ctypes.POINTER(_vp), _int, _dp, _dp]),
     "psx_map2alm_pol": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
@@ -252,6 +254,22 @@ class Context:
             out = np.empty((n, n), order="F")
         self.check(self.lib.psx_mcm(self._h, kind, lmin, lmax, dptr(W), W.size, dptr(out), n),
                    "psx_mcm")
+        return out
+
+    def mcm_binned(self, kind: int, lmin: int, lmax: int, W: np.ndarray, P: np.ndarray,
+                   Q: np.ndarray) -> np.ndarray:
+        """P M Q for one mode-coupling block M over l = lmin..lmax: P is
+        (nbl, n), Q is (n, nbr); M stays on the device."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        n = _block_size(lmin, lmax)
+        P = np.asfortranarray(P, dtype=np.float64)
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        if P.ndim != 2 or Q.ndim != 2 or P.shape[1] != n or Q.shape[0] != n:
+            raise AxisError(f"binning matrices must be (nbl, {n}) and ({n}, nbr)")
+        out = np.empty((P.shape[0], Q.shape[1]), order="F")
+        self.check(self.lib.psx_mcm_binned(self._h, kind, lmin, lmax, dptr(W), W.size, dptr(P),
+                                           P.shape[0], dptr(Q), Q.shape[1], dptr(out),
+                                           P.shape[0]), "psx_mcm_binned")
         return out
 
     def mcm_pm(self, lmin: int, lmax: int, W: np.ndarray):

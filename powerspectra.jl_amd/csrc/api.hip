@@ -175,6 +175,37 @@ int psx_mcm(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t L
   });
 }
 
+int psx_mcm_binned(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t LW,
+                   const double* P, int nbl, const double* Q, int nbr, double* out,
+                   int64_t ldo) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, lmax - lmin + 1);
+    PSX_REQUIRE(W && P && Q && out, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_REQUIRE(nbl >= 1 && nbr >= 1, PSX_ERR_ARGUMENT, "need at least one bin on each side");
+    PSX_REQUIRE(ldo >= nbl, PSX_ERR_AXES, "leading dimension of the binned matrix below nbl");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n,
+                 pb = sizeof(double) * nbl * (size_t)n, qb = sizeof(double) * n * (size_t)nbr,
+                 ob = sizeof(double) * nbl * (size_t)nbr;
+    XferBuf x(c, xfer_bytes({wb, mb, pb, qb, pb, ob}));
+    double* dW = x.take(wb);
+    double* dM = x.take(mb);
+    double* dP = x.take(pb);
+    double* dQ = x.take(qb);
+    double* dT = x.take(pb);
+    double* dO = x.take(ob);
+    h2d(c, dW, W, wb);
+    h2d(c, dP, P, pb);
+    h2d(c, dQ, Q, qb);
+    run_contraction(c, mcm_request(kind, lmin, lmax, dW, LW, dM, n));
+    dgemm_dev(c, nbl, n, n, dP, nbl, dM, n, dT, nbl);     // P M
+    dgemm_dev(c, nbl, nbr, n, dT, nbl, dQ, n, dO, nbl);   // (P M) Q
+    d2h_matrix(c, out, ldo, dO, nbl, nbl, nbr);
+  });
+}
+
 int psx_mcm_pm_d(psx_ctx* h, int lmin, int lmax, const double* dW, int64_t LW,
                  double* dMpp, double* dMmm, int64_t ldM) {
   Ctx* c = C(h);

@@ -40,6 +40,14 @@ void release_blas(Ctx* c) {
                 std::string(#call) + " failed: " + rocblas_status_to_string(s_)); \
   } while (0)
 
+void dgemm_dev(Ctx* c, int m, int n, int k, const double* dA, int64_t lda, const double* dB,
+               int64_t ldb, double* dC, int64_t ldc) {
+  const double one = 1.0, zero = 0.0;
+  PSX_ROC(rocblas_dgemm(blas(c), rocblas_operation_none, rocblas_operation_none, m, n, k, &one,
+                        dA, (rocblas_int)lda, dB, (rocblas_int)ldb, &zero, dC,
+                        (rocblas_int)ldc));
+}
+
 // ---- LU with partial pivoting ---------------------------------------------
 // Right-looking blocked getrf.  Outer blocks of kLuPanel columns end with a
 // trailing update by rocBLAS (dtrsm + dgemm of depth kLuPanel).  Inside a

@@ -235,6 +235,10 @@ int mcm_family(int kind);
 void cov_layout(int kind, ContractRequest& rq, const double* const* W);
 
 // ---- linear algebra ------------------------------------------------------
+// C (m x n) = A (m x k) B (k x n), column-major FP64, rocBLAS on the context
+// stream
+void dgemm_dev(Ctx* c, int m, int n, int k, const double* dA, int64_t lda, const double* dB,
+               int64_t ldb, double* dC, int64_t ldc);
 void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB,
                   int64_t ldb, int nrhs);  // dA overwritten by LU
 void decouple_dev(Ctx* c, int n, const double* dY, double* dB1, double* dB2,

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ._abi import MCM_00, MCM_02, MCM_MM, MCM_PP, ArgumentError, Context, default_context
+from ._abi import MCM_00, MCM_02, MCM_MM, MCM_PP, ArgumentError, AxisError, Context, default_context
 from .alm import Alm, alm2cl
 from .blockspectralmatrix import BlockSpectralMatrix, hvcat
 from .spectralarray import SpectralArray, SpectralVector
@@ -65,6 +65,30 @@ def mcm_from_spectrum(spec, V, *, lmin: int = 0, lmax: int | None = None,
             return hvcat([[Mpp, Mmm], [Mmm, Mpp]])
         return hvcat([[Mpp, -Mmm], [-Mmm, Mpp]])
     raise ArgumentError(f"{spec} not a valid spectrum.")
+
+
+def mcm_binned(spec, V, P, Q=None, *, lmin: int = 0, lmax: int | None = None,
+               ctx: Context | None = None) -> np.ndarray:
+    """P M Q for the single block M = mcm_from_spectrum(spec, V) (TT, TE/ET/TB/BT,
+    M++ or M--), with M kept on the device.  P is a binning matrix as
+    binning_matrix (src/util.jl:3-18) returns it, one column per l = 0..lmax;
+    Q (default P^T) has one row per l.  Columns of P and rows of Q below lmin
+    are dropped, as M starts at lmin."""
+    ctx = ctx or default_context()
+    V = np.asarray(V.parent if isinstance(V, SpectralArray) else V, dtype=np.float64)
+    if lmax is None:
+        lmax = V.size - 1
+    if V.size < lmax + 1:
+        raise ArgumentError(f"mask spectrum has {V.size} entries, lmax {lmax} needs {lmax + 1}")
+    name = _name(spec)
+    if name not in _KIND:
+        raise ArgumentError(f"{spec}: binned matrices are for single blocks")
+    P = np.asarray(P, dtype=np.float64)
+    Q = P.T if Q is None else np.asarray(Q, dtype=np.float64)
+    if P.ndim != 2 or Q.ndim != 2 or P.shape[1] != lmax + 1 or Q.shape[0] != lmax + 1:
+        raise AxisError(f"binning matrices need lmax + 1 = {lmax + 1} columns (P) and rows (Q)")
+    return ctx.mcm_binned(_KIND[name], lmin, lmax, np.ascontiguousarray(V[: lmax + 1]),
+                          P[:, lmin:], Q[lmin:, :])
 
 
 def mcm(spec, a1, a2=None, *, lmin: int = 0, lmax: int | None = None,

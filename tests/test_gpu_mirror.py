@@ -2,7 +2,7 @@
 decouple_covmat / `\\`), against the oracle on the same mask spectra."""
 import numpy as np
 import pytest
-from helpers import RTOL_FP64, norm_err, rel_err, spectra_set
+from helpers import RTOL_FP64, mask_like_W, norm_err, rel_err, spectra_set
 
 pytestmark = pytest.mark.gpu
 
@@ -242,3 +242,36 @@ def test_master_default_path_matches_alm_route(pkg, ctx, masks):
         want = ref[k].parent
         assert np.abs(out[k].parent - want).max() <= 1e-10 * np.abs(want).max(), k
         assert np.array_equal(pre[k].parent, out[k].parent), k
+
+
+@pytest.mark.parametrize("spec,lmin,lmax", [("TT", 0, 300), ("TE", 2, 257), ("M++", 2, 130),
+                                            ("M--", 2, 130)])
+def test_mcm_binned_matches_host_product(pkg, ctx, spec, lmin, lmax):
+    """mcm_binned (psx_mcm_binned: the block kept on the device, then two
+    rocBLAS GEMMs) against P M Q on the host from the unbinned block, with
+    binning_matrix's bins (src/util.jl:3-18) and a flat unbinning Q, and
+    against P M P^T when Q is left out."""
+    V = mask_like_W(lmax + 1, seed=lmax)
+    left = np.arange(2, lmax - 20, 30)
+    P = pkg.binning_matrix(left, left + 29, lambda l: l * (l + 1.0), lmax)
+    Q = (P > 0).T.astype(np.float64)
+    M = pkg.mcm_from_spectrum(spec, V, lmin=lmin, lmax=lmax, ctx=ctx).parent
+    got = pkg.mcm_binned(spec, V, P, Q, lmin=lmin, lmax=lmax, ctx=ctx)
+    want = P[:, lmin:] @ M @ Q[lmin:, :]
+    assert got.shape == (P.shape[0], Q.shape[1])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    got = pkg.mcm_binned(spec, V, P, lmin=lmin, lmax=lmax, ctx=ctx)
+    want = P[:, lmin:] @ M @ P[:, lmin:].T
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_mcm_binned_arguments(pkg, ctx):
+    """A 2n x 2n block name, and binning matrices of the wrong l length, are refused."""
+    V = mask_like_W(101, seed=1)
+    P = pkg.binning_matrix(np.arange(2, 90, 10), np.arange(11, 100, 10), lambda l: 1.0, 100)
+    with pytest.raises(pkg.ArgumentError):
+        pkg.mcm_binned("EE_BB", V, P, lmax=100, ctx=ctx)
+    with pytest.raises(pkg.AxisError):
+        pkg.mcm_binned("TT", V, P[:, :-1], lmax=100, ctx=ctx)
+    with pytest.raises(pkg.AxisError):
+        ctx.mcm_binned(0, 0, 100, V, P, P[:, :-1].T)
