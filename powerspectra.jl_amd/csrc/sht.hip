@@ -689,7 +689,13 @@ __device__ __forceinline__ void pol_rescale(double& x1, double& x2, double& y1, 
 // Spin-2 analysis partial sums: part[comp][chunk][idx(l, m)], comp 0 = E, 1 = B,
 // of sum over the chunk's pairs of [2 lambda+ Q + 2i lambda- U] (E) and
 // [2 lambda+ U - 2i lambda- Q] (B); F holds Q+, Q-, U+, U- at stride fs.
-__global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
+// Both spin-2 Legendre kernels ask for 3 waves per SIMD (168 VGPRs): at their
+// natural 175 VGPRs only 2 waves fit, too few to cover the latency of the
+// dependent FP64 chains of the four recurrences.  The price is a few spilled
+// values: 8 scratch accesses per round of kLChunk2 steps in the analysis, none
+// inside the synthesis loop.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
                                                             int64_t fs, double2* __restrict__ part) {
   __shared__ double red[kLChunk2][4][256];
   const int m = blockIdx.y;
@@ -789,7 +795,8 @@ __global__ void __launch_bounds__(256) legendre_analysis_s2(PolArgs A, const dou
 // parity, lambda- terms to the other (north = even + odd, south = even - odd).
 // a_E, a_B and the coefficients of kLChunk2 steps are read ahead with scalar
 // loads.
-__global__ void __launch_bounds__(256) legendre_synthesis_s2(PolArgs A,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+legendre_synthesis_s2(PolArgs A,
                                                              const double2* __restrict__ aE,
                                                              const double2* __restrict__ aB,
                                                              double2* __restrict__ GQ,
