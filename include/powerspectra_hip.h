@@ -181,6 +181,16 @@ int psx_lu_solve(psx_ctx* ctx, int n, const double* A, int64_t lda,
  * the solution; enqueued on the context stream, returns when done. */
 int psx_lu_solve_d(psx_ctx* ctx, int n, double* dA, int64_t lda, double* dB, int64_t ldb,
                    int nrhs);
+/* The joint EE/BB (sign = +1) or EB/BE (sign = -1) decoupling solve
+ * [M++ s M--; s M-- M++] X = B of maskedalm2spectra (reference
+ * src/modecoupling.jl:362-377, the blocks of mcm(:EE_BB) and mcm(:EB_BE),
+ * :210-224,235-244): Mpp, Mmm are n x n (leading dimension ldM), B is 2n x nrhs
+ * (leading dimension ldb >= 2n) and is overwritten by X.  The 2n x 2n matrix
+ * is assembled on the device, so only M++ and M-- cross the bus; sign = -1
+ * solves the sign = +1 matrix against D B and returns D X, D = diag(I, -I),
+ * which is exact.  LU with partial pivoting as psx_lu_solve. */
+int psx_lu_solve_pm(psx_ctx* ctx, int n, int sign, const double* Mpp, const double* Mmm,
+                    int64_t ldM, double* B, int64_t ldb, int nrhs);
 /* `lu(A)` (reference src/blockspectralmatrix.jl:89-99, Julia's LAPACK getrf
  * with partial pivoting): LU (n x n, leading dimension ldlu) receives the
  * unit-lower L below the diagonal and U on and above it, ipiv[n] the 1-based

@@ -98,6 +98,7 @@ SIGNATURES = {
     "psx_quickpol_xi_d": (_int, [_vp] + [_int] * 7 + [_vp, _i64, _vp]),
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2cl": (_int, [_vp, _int, _int, ctypes.POINTER(_vp), _dp]),
+    "psx_lu_solve_pm": (_int, [_vp, _int, _int, _dp, _dp, _i64, _dp, _i64, _int]),
     "psx_mcm_binned": (_int, [_vp, _int, _int, _int, _dp, _i64, _dp, _int, _dp, _int, _dp,
                               _i64]),
     "psx_master_spectra": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp),
@@ -479,11 +480,29 @@ class Context:
         vec = np.ndim(B) == 1
         X = np.array(B, dtype=np.float64, order="F", copy=True)
         if vec:
-            X = X.reshape(n, 1, order="F")
+            X = X.reshape(-1, 1, order="F")
         if X.shape[0] != n:
             raise AxisError("right-hand side length does not match the matrix")
         self.check(self.lib.psx_lu_solve(self._h, n, dptr(A), n, dptr(X), n, X.shape[1]),
                    "psx_lu_solve")
+        return X[:, 0] if vec else X
+
+    def lu_solve_pm(self, Mpp, Mmm, B, sign: int = 1):
+        """[M++ s M--; s M-- M++] \\ B with s = sign (+1: EE/BB, -1: EB/BE), the
+        2n x 2n matrix assembled on the device (psx_lu_solve_pm)."""
+        Mpp = np.asfortranarray(Mpp, dtype=np.float64)
+        Mmm = np.asfortranarray(Mmm, dtype=np.float64)
+        n = Mpp.shape[0]
+        if Mpp.shape != (n, n) or Mmm.shape != (n, n):
+            raise AxisError("lu_solve_pm needs two square blocks of one size")
+        vec = np.ndim(B) == 1
+        X = np.array(B, dtype=np.float64, order="F", copy=True)
+        if vec:
+            X = X.reshape(-1, 1, order="F")
+        if X.shape[0] != 2 * n:
+            raise AxisError("right-hand side length is not twice the block size")
+        self.check(self.lib.psx_lu_solve_pm(self._h, n, int(sign), dptr(Mpp), dptr(Mmm), n,
+                                            dptr(X), 2 * n, X.shape[1]), "psx_lu_solve_pm")
         return X[:, 0] if vec else X
 
 

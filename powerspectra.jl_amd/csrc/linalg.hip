@@ -921,6 +921,16 @@ void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ld
   PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
+// B[r0 + i, j] = -B[r0 + i, j] for i < nr, j < gridDim.y
+__global__ void __launch_bounds__(256) negate_rows(double* __restrict__ B, int64_t ldb, int r0,
+                                                   int nr) {
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i < nr) {
+    double* p = B + (int64_t)blockIdx.y * ldb + r0 + i;
+    *p = -*p;
+  }
+}
+
 static void transpose(Ctx* c, int n, const double* dIn, double* dOut) {
   const double one = 1.0, zero = 0.0;
   PSX_ROC(rocblas_dgeam(blas(c), rocblas_operation_transpose, rocblas_operation_none, n, n,
@@ -969,6 +979,49 @@ int psx_lu_solve(psx_ctx* hctx, int n, const double* A, int64_t lda, double* B,
     if (nrhs > 0) h2d_matrix(c, dB, n, B, ldb, n, nrhs);
     lu_solve_dev(c, n, dA, n, dB, n, nrhs);
     if (nrhs > 0) d2h_matrix(c, B, ldb, dB, n, n, nrhs);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int psx_lu_solve_pm(psx_ctx* hctx, int n, int sign, const double* Mpp, const double* Mmm,
+                    int64_t ldM, double* B, int64_t ldb, int nrhs) {
+  Ctx* c = reinterpret_cast<Ctx*>(hctx);
+  return guarded(c, [&] {
+    PSX_REQUIRE(n >= 1 && nrhs >= 1 && ldM >= n && ldb >= 2 * (int64_t)n, PSX_ERR_ARGUMENT,
+                "bad sizes for lu_solve_pm");
+    PSX_REQUIRE(sign == 1 || sign == -1, PSX_ERR_ARGUMENT, "sign must be +1 or -1");
+    PSX_REQUIRE(Mpp && Mmm && B, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_REQUIRE(2 * (int64_t)n <= INT32_MAX / 2, PSX_ERR_ARGUMENT, "n too large");
+    PSX_HIP(hipSetDevice(c->device));
+    const int N = 2 * n;
+    const size_t ab = sizeof(double) * N * (size_t)N, bb = sizeof(double) * N * (size_t)nrhs;
+    XferBuf x(c, xfer_bytes({ab, bb}));
+    double* dA = x.take(ab);
+    double* dB = x.take(bb);
+    // [M++ M--; M-- M++]: each block crosses the bus once, its twin is a
+    // device copy
+    const int64_t off = (int64_t)n * N + n;   // the bottom-right block
+    h2d_matrix(c, dA, N, Mpp, ldM, n, n);
+    h2d_matrix(c, dA + n, N, Mmm, ldM, n, n);
+    PSX_HIP(hipMemcpy2DAsync(dA + off, sizeof(double) * N, dA, sizeof(double) * N,
+                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, c->stream));
+    PSX_HIP(hipMemcpy2DAsync(dA + (int64_t)n * N, sizeof(double) * N, dA + n,
+                             sizeof(double) * N, sizeof(double) * n, n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    h2d_matrix(c, dB, N, B, ldb, N, nrhs);
+    // [M++ -M--; -M-- M++] = D [M++ M--; M-- M++] D, D = diag(I, -I): solve
+    // the first against D B and return D X
+    const dim3 ng((unsigned)((n + 255) / 256), (unsigned)nrhs);
+    if (sign < 0) {
+      hipLaunchKernelGGL(negate_rows, ng, dim3(256), 0, c->stream, dB, (int64_t)N, n, n);
+      PSX_HIP(hipGetLastError());
+    }
+    lu_solve_dev(c, N, dA, N, dB, N, nrhs);
+    if (sign < 0) {
+      hipLaunchKernelGGL(negate_rows, ng, dim3(256), 0, c->stream, dB, (int64_t)N, n, n);
+      PSX_HIP(hipGetLastError());
+    }
+    d2h_matrix(c, B, ldb, dB, N, N, nrhs);
     PSX_HIP(hipStreamSynchronize(c->stream));
   });
 }

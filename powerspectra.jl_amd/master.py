@@ -13,7 +13,7 @@ import numpy as np
 
 from ._abi import ArgumentError, Context, default_context
 from .alm import Alm, alm2cl
-from .blockspectralmatrix import BlockSpectralMatrix, hvcat, solve
+from .blockspectralmatrix import solve
 from .modecoupling import mcm_set
 from .spectralarray import SpectralArray, SpectralVector
 
@@ -60,12 +60,12 @@ def _decoupled(W, P, lmin: int, lmax: int, ctx: Context) -> dict:
     # [EB; -BE] solves A against [pEB; -pBE].  Sign flips of rows and columns
     # change neither partial pivoting's choices nor any magnitude, so one
     # factorisation of A, with two right-hand sides, serves both systems.
+    # The 2n x 2n matrix is assembled on the device (psx_lu_solve_pm).
     n = lmax - lmin + 1
-    M_EE_BB = hvcat([[Mpp, Mmm], [Mmm, Mpp]])
     rhs = np.empty((2 * n, 2), order="F")
     rhs[:n, 0], rhs[n:, 0] = pcl("EE").parent, pcl("BB").parent
     rhs[:n, 1], rhs[n:, 1] = pcl("EB").parent, -pcl("BE").parent
-    X = solve(M_EE_BB, BlockSpectralMatrix(rhs, M_EE_BB.n_ells, [range(1, 3)]), ctx=ctx).parent
+    X = ctx.lu_solve_pm(Mpp.parent, Mmm.parent, rhs, 1)
     vec = lambda v: SpectralArray(np.ascontiguousarray(v), firsts=(lmin,))
     out["EE"], out["BB"] = vec(X[:n, 0]), vec(X[n:, 0])
     out["EB"], out["BE"] = vec(X[:n, 1]), vec(-X[n:, 1])

@@ -67,6 +67,8 @@ def test_singular(ctx, pkg):
         ctx.lu_solve(A, np.ones(4))
     with pytest.raises(pkg.SingularException):
         ctx.decouple_covmat(np.eye(4), np.zeros((4, 4)), np.eye(4))
+    with pytest.raises(pkg.AxisError):
+        ctx.lu_solve(np.eye(4), np.ones(3))
 
 
 @pytest.mark.parametrize("n", [1, 5, 16, 17, 32, 33, 256, 257, 289, 300, 513, 1000, 1025, 2100,
@@ -260,3 +262,37 @@ def test_decouple_shared_matrix_factored_once(ctx):
     apart = ctx.decouple_covmat(Y, M, np.array(M, order="F", copy=True))
     assert np.array_equal(same, apart)
     assert np.linalg.norm(M @ same @ M.T - Y) / np.linalg.norm(Y) < 1e-11
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (300, 1), (300, 3), (1700, 2)])
+def test_lu_solve_pm(ctx, pkg, n, nrhs):
+    """psx_lu_solve_pm: [M++ s M--; s M-- M++] X = B with the 2n x 2n matrix
+    assembled on the device equals psx_lu_solve on the matrix assembled on the
+    host, bit for bit for s = +1, and for s = -1, where it solves the s = +1
+    matrix against D B (D = diag(I, -I)) exactly; both against numpy."""
+    rng = np.random.default_rng(n + nrhs)
+    Mpp = np.eye(n) * 2.0 + rng.standard_normal((n, n)) / np.sqrt(n)
+    Mmm = 0.3 * rng.standard_normal((n, n)) / np.sqrt(n)
+    B = rng.standard_normal((2 * n, nrhs))
+    for sign in (1, -1):
+        A = np.block([[Mpp, sign * Mmm], [sign * Mmm, Mpp]])
+        X = ctx.lu_solve_pm(Mpp, Mmm, B, sign)
+        assert np.array_equal(X, ctx.lu_solve(A, B)), sign
+        R = np.linalg.solve(A, B)
+        assert np.linalg.norm(X - R) / np.linalg.norm(R) < 1e-12, sign
+    x = ctx.lu_solve_pm(Mpp, Mmm, B[:, 0], -1)
+    assert x.shape == (2 * n,) and np.array_equal(x, ctx.lu_solve_pm(Mpp, Mmm, B[:, :1], -1)[:, 0])
+
+
+def test_lu_solve_pm_arguments(ctx, pkg):
+    """A sign other than +-1, blocks of different sizes, a right-hand side of
+    the wrong length, and M++ = M-- (a singular block matrix) are refused."""
+    A = np.eye(4)
+    with pytest.raises(pkg.ArgumentError):
+        ctx.lu_solve_pm(A, A, np.ones(8), 0)
+    with pytest.raises(pkg.AxisError):
+        ctx.lu_solve_pm(A, np.eye(3), np.ones(8))
+    with pytest.raises(pkg.AxisError):
+        ctx.lu_solve_pm(A, A, np.ones(7))
+    with pytest.raises(pkg.SingularException):
+        ctx.lu_solve_pm(A, A, np.ones(8), 1)
