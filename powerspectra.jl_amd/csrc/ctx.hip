@@ -298,6 +298,11 @@ namespace psx {
 
 namespace {
 constexpr size_t kDirectBytes = size_t(1) << 20;
+// A contiguous range longer than a stage buffer (one long column, or columns
+// packed on both sides: a map, a set of a_lm) is recut into columns of this
+// many doubles (512 KB), so that it goes through the staged pipeline and its
+// host copies spread over threads.
+constexpr int64_t kLineDoubles = int64_t(1) << 16;
 
 void ensure_stage(Ctx* c) {
   if (c->stage[0]) return;
@@ -388,6 +393,18 @@ void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2]
     const D2HMatrix& x = m[i];
     if (x.rows <= 0 || x.cols <= 0) continue;
     const size_t colb = sizeof(double) * (size_t)x.rows;
+    if (colb > kStageBytes && (x.cols == 1 || (x.ldh == x.rows && x.ldd == x.rows))) {
+      const int64_t total = x.rows * x.cols, full = total / kLineDoubles;
+      const int64_t rem = total % kLineDoubles;
+      const int64_t cpc = (int64_t)(kStageBytes / (sizeof(double) * kLineDoubles));
+      for (int64_t j0 = 0; j0 < full; j0 += cpc)
+        ch.push_back({x.host + j0 * kLineDoubles, kLineDoubles, x.dev + j0 * kLineDoubles,
+                      kLineDoubles, kLineDoubles, std::min(cpc, full - j0), x.after});
+      if (rem)
+        ch.push_back({x.host + full * kLineDoubles, rem, x.dev + full * kLineDoubles, rem, rem,
+                      1, x.after});
+      continue;
+    }
     // (a piece that waits for an event is always staged: a direct copy to
     // pageable memory would hold this thread until the event fires)
     if ((!x.after && colb * (size_t)x.cols <= kDirectBytes) || colb > kStageBytes) {
@@ -440,6 +457,13 @@ void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ld
                 int64_t rows, int64_t cols) {
   const size_t colb = sizeof(double) * (size_t)rows;
   if (rows <= 0 || cols <= 0) return;
+  if (colb > kStageBytes && (cols == 1 || (ldd == rows && ldh == rows))) {
+    const int64_t total = rows * cols, full = total / kLineDoubles, rem = total % kLineDoubles;
+    if (full) h2d_matrix(c, dev, kLineDoubles, host, kLineDoubles, kLineDoubles, full);
+    if (rem)
+      h2d_matrix(c, dev + full * kLineDoubles, rem, host + full * kLineDoubles, rem, rem, 1);
+    return;
+  }
   if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
     PSX_HIP(hipMemcpy2DAsync(dev, ldd * sizeof(double), host, ldh * sizeof(double), colb,
                              (size_t)cols, hipMemcpyHostToDevice, c->stream));

@@ -253,11 +253,13 @@ def test_mask_spectra_config_masks_and_limits(pkg, ctx):
         ctx.mask_spectra(np.zeros((9, 12 * 4 * 4)), 4, 8, 0)
 
 
-@pytest.mark.parametrize("lmax,nsets", [(0, 1), (5, 2), (100, 6), (767, 8)])
+@pytest.mark.parametrize("lmax,nsets", [(0, 1), (5, 2), (100, 6), (767, 8), (2900, 2)])
 def test_alm2cl_device_vs_host(pkg, ctx, lmax, nsets):
     """psx_alm2cl (all cross-spectra of coefficient sets anywhere in host
     memory) against the host alm2cl of the same sets; a set passed twice gives
-    its auto-spectrum, bit for bit, in all four entries."""
+    its auto-spectrum, bit for bit, in all four entries.  At lmax 2900 each
+    set is 64 MB, longer than a staging buffer and not a whole number of the
+    512 KB lines it is recut into."""
     rng = np.random.default_rng(lmax + nsets)
     nalm = (lmax + 1) * (lmax + 2) // 2
     alms = [pkg.Alm(lmax, lmax, rng.standard_normal(nalm) + 1j * rng.standard_normal(nalm))
