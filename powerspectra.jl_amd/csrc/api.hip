@@ -197,8 +197,8 @@ int psx_mcm_binned(psx_ctx* h, int kind, int lmin, int lmax, const double* W, in
     double* dT = x.take(pb);
     double* dO = x.take(ob);
     h2d(c, dW, W, wb);
-    h2d(c, dP, P, pb);
-    h2d(c, dQ, Q, qb);
+    h2d_matrix(c, dP, nbl, P, nbl, nbl, n);
+    h2d_matrix(c, dQ, n, Q, n, n, nbr);
     run_contraction(c, mcm_request(kind, lmin, lmax, dW, LW, dM, n));
     dgemm_dev(c, nbl, n, n, dP, nbl, dM, n, dT, nbl);     // P M
     dgemm_dev(c, nbl, nbr, n, dT, nbl, dQ, n, dO, nbl);   // (P M) Q
