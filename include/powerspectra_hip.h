@@ -85,6 +85,9 @@ typedef struct psx_ctx psx_ctx;
 /* Create a context on HIP device `device`.  `hbm_budget` bounds the bytes the
  * context may hold for workspaces and resident spectra (0 = 90% of free HBM). */
 int psx_ctx_create(int device, size_t hbm_budget, psx_ctx** out);
+/* Waits for the context's work and frees everything it holds.  Refused with
+ * PSX_ERR_ARGUMENT (nothing destroyed) while a server created on this context
+ * is alive: destroy servers first (psx_server_destroy). */
 int psx_ctx_destroy(psx_ctx* ctx);
 int psx_ctx_synchronize(psx_ctx* ctx);
 /* Human-readable message for the last error on this context (never NULL). */
@@ -112,7 +115,10 @@ int psx_memcpy_d2h(psx_ctx* ctx, void* dst, const void* src, size_t bytes);
  * M[l2,l1] = (2l1+1) Ξ for lmin <= l1 <= l2 <= lmax, with Ξ the projector of
  * src/modecoupling.jl:3-66 summed over l3 <= min(l1+l2, LW-1).
  * The reference passes V = alm2cl(...)[0:lmax], i.e. LW = lmax+1
- * (src/modecoupling.jl:197); LW is a free parameter here. */
+ * (src/modecoupling.jl:197); LW is a free parameter here.
+ * Every mode-coupling and covariance entry point accepts lmax <= 8191: up
+ * to there the integer polynomials of the closed-form 3j products are exact
+ * doubles, so each term carries a few ulp (PSX_ERR_ARGUMENT above). */
 int psx_mcm(psx_ctx* ctx, int kind, int lmin, int lmax,
             const double* W, int64_t LW, double* M, int64_t ldM);
 int psx_mcm_d(psx_ctx* ctx, int kind, int lmin, int lmax,
@@ -314,7 +320,8 @@ typedef struct psx_request {
 } psx_request;
 
 /* slots: concurrent requests (1..16); arena_bytes: device memory for requests
- * (taken from the context budget); lmax_max: largest lmax accepted. */
+ * (taken from the context budget); lmax_max: largest lmax accepted (<= 8191).
+ * The server lives on its context: psx_ctx_destroy refuses while it exists. */
 int psx_server_create(psx_ctx* ctx, int slots, size_t arena_bytes, int lmax_max,
                       psx_server** out);
 /* Waits for admitted work, drops queued requests, frees everything. */

@@ -198,8 +198,10 @@ class Context:
             raise _ERRORS.get(st, PowerSpectraError)(f"{what}: {msg} (status {st})")
 
     def close(self):
+        """Destroy the context.  Refused (ArgumentError, the context stays
+        usable) while a Server created on it is still open."""
         if getattr(self, "_h", None):
-            self.lib.psx_ctx_destroy(self._h)
+            self.check(self.lib.psx_ctx_destroy(self._h), "psx_ctx_destroy")
             self._h = None
 
     def __del__(self):

@@ -6,6 +6,7 @@ on a machine without a GPU.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import subprocess
@@ -37,14 +38,22 @@ def sources() -> list[str]:
 
 def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
     hipcc = os.path.join(ROCM, "bin", "hipcc")
-    objs = []
+    jobs = []
     for src in sources():
         obj = os.path.join(CSRC, os.path.basename(src)[:-4] + ".o")
-        cmd = [hipcc, *HIPCC_FLAGS, *(extra_flags or []), "-c", src, "-o", obj]
+        jobs.append(([hipcc, *HIPCC_FLAGS, *(extra_flags or []), "-c", src, "-o", obj], obj))
+
+    def compile_one(job):
+        cmd, obj = job
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+        return obj
+
+    # one compiler process per source file, a few at a time
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        objs = list(ex.map(compile_one, jobs))
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
            f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", "-lrocfft", f"-Wl,-rpath,{ROCM}/lib"]

@@ -50,6 +50,7 @@ Ctx* C(psx_ctx* h) { return reinterpret_cast<Ctx*>(h); }
 
 void check_block(int lmin, int lmax, int64_t LW, int64_t ld) {
   PSX_REQUIRE(lmin >= 0 && lmax >= lmin, PSX_ERR_ARGUMENT, "need 0 <= lmin <= lmax");
+  PSX_REQUIRE(lmax <= 8191, PSX_ERR_ARGUMENT, "lmax above 8191 is not supported");
   PSX_REQUIRE(LW >= 1, PSX_ERR_ARGUMENT, "W must have at least one entry");
   PSX_REQUIRE(ld >= lmax - lmin + 1, PSX_ERR_AXES,
               "leading dimension smaller than lmax-lmin+1");

@@ -24,9 +24,10 @@
 //   odd:   z_{l1-c-1}   * z_c      *  z_{l2-c-1}   * igamma_{S-c} *  What_{l3}
 // with delta_g = 1/((2g+1) gamma_g), igamma_g = 1/gamma_g and
 // What_l = (2l+1) W_l / (4 pi).  The main kernel spends 2 FP64 VALU ops per
-// (0 0 0)^2 term (mul + fma) -- no sqrt, no division, no recurrence.  All
-// integer-valued polynomials (P, Q, D0, D1) are exact in FP64 for
-// lmax <= ~6000.
+// (0 0 0)^2 term (mul + fma) -- no sqrt, no division, no recurrence.  The
+// integer polynomials P, D0, D1 and kappa = 1 + 2 P1 P2 are exact doubles for
+// lmax <= 8191 (2 P1 P2 <= 2 (8191 * 8192)^2 < 2^53; at 8192 it is not), and
+// Q = E^2 - kappa and D1^2 take one rounding each; lmax is capped there.
 //
 // Work decomposition: one wavefront owns a tile of 64 consecutive l1 (lanes)
 // times R consecutive anti-diagonals S = S0 .. S0+R-1 (register slots); for a
@@ -51,6 +52,7 @@ namespace psx {
 namespace {
 
 constexpr int kWOff = 192;   // What index of l3 = 0
+constexpr int kLmaxCap = 8191;  // exactness bound of kappa, see above
 constexpr double kInv4Pi = 0.07957747154594766788444188168625718;  // 1/(4 pi)
 
 // A tile: lanes l1 = x0 .. x0+63 on anti-diagonals S0 .. S0+R-1; the tile
@@ -759,7 +761,7 @@ int* run_contraction_on(Ctx* c, const ContractRequest& rq, hipStream_t s,
   PSX_REQUIRE(nv >= 1 && nv <= kMaxV, PSX_ERR_ARGUMENT, "bad number of W spectra");
   PSX_REQUIRE(rq.lmin >= 0 && rq.lmax >= rq.lmin, PSX_ERR_ARGUMENT, "bad lmin/lmax");
   PSX_REQUIRE(rq.LW >= 1 && rq.LW < (1 << 24), PSX_ERR_ARGUMENT, "bad W length");
-  PSX_REQUIRE(rq.lmax <= 16383, PSX_ERR_ARGUMENT, "lmax above 16383 is not supported");
+  PSX_REQUIRE(rq.lmax <= kLmaxCap, PSX_ERR_ARGUMENT, "lmax above 8191 is not supported");
   // l3 never exceeds l1 + l2 <= 2 lmax: longer W spectra are truncated
   ContractRequest rqe = rq;
   rqe.LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
@@ -895,7 +897,7 @@ void run_contraction_to_host(Ctx* c, const ContractRequest& rq, double* const* h
     const int nv = nv_of(rq);
     PSX_REQUIRE(nv >= 1 && nv <= kMaxV, PSX_ERR_ARGUMENT, "bad number of W spectra");
     PSX_REQUIRE(rq.LW >= 1 && rq.LW < (1 << 24), PSX_ERR_ARGUMENT, "bad W length");
-    PSX_REQUIRE(rq.lmax <= 16383, PSX_ERR_ARGUMENT, "lmax above 16383 is not supported");
+    PSX_REQUIRE(rq.lmax <= kLmaxCap, PSX_ERR_ARGUMENT, "lmax above 8191 is not supported");
     ContractRequest rqe = rq;
     rqe.LW = std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
     err = run_contraction_checked(c, rqe, c->stream, scratch, c->scratch_bytes, kMaxBands,

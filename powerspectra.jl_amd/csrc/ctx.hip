@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -96,7 +97,9 @@ TimedLaunch::~TimedLaunch() noexcept(false) {
     (void)hipEventRecord(e1, c->stream);
     c->timers[name].pending.emplace_back(e0, e1);
   }
-  if (le != hipSuccess)
+  // while another error unwinds the stack (a failed call inside the timed
+  // scope), that error is the one reported: a second throw would terminate
+  if (le != hipSuccess && std::uncaught_exceptions() == 0)
     throw Error{PSX_ERR_DEVICE, std::string("launch of ") + name + ": " +
                                     hipGetErrorString(le)};
 }
@@ -119,12 +122,10 @@ void resolve_timers(Ctx* c) {
 
 // gamma_m = prod_{k<=m} (2k-1)/(2k) accumulated in long double (64-bit
 // mantissa); the rounding to double of each entry is then the dominant error.
-void ensure_tables(Ctx* c, int nmax_index) {
-  Tables& t = c->tables;
-  if (t.n >= nmax_index + 2 * kTabOff) return;
-  int n = nmax_index + 2 * kTabOff;
-  n = (n + 1023) & ~1023;
-  std::vector<double> G(n, 0.0), Z(n, 0.0), D(n, 0.0), IG(n, 0.0);
+int table_entries(int nmax_index) { return (nmax_index + 2 * kTabOff + 1023) & ~1023; }
+
+void build_tables_host(int n, double* G, double* Z, double* D, double* IG) {
+  for (int i = 0; i < n; ++i) G[i] = Z[i] = D[i] = IG[i] = 0.0;
   long double g = 1.0L;
   for (int m = 0; m + kTabOff < n; ++m) {
     if (m > 0) g = g * (long double)(2 * m - 1) / (long double)(2 * m);
@@ -134,6 +135,14 @@ void ensure_tables(Ctx* c, int nmax_index) {
     D[i] = (double)(1.0L / ((long double)(2 * m + 1) * g));
     IG[i] = (double)(1.0L / g);
   }
+}
+
+void ensure_tables(Ctx* c, int nmax_index) {
+  Tables& t = c->tables;
+  if (t.n >= nmax_index + 2 * kTabOff) return;
+  const int n = table_entries(nmax_index);
+  std::vector<double> G(n), Z(n), D(n), IG(n);
+  build_tables_host(n, G.data(), Z.data(), D.data(), IG.data());
   // other streams (a server's slots) may read the old tables
   PSX_HIP(hipDeviceSynchronize());
   if (t.dG) {
@@ -191,17 +200,24 @@ int psx_ctx_destroy(psx_ctx* h) {
   if (!c) return PSX_ERR_ARGUMENT;
   {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (c->live_servers > 0) {
+      c->last_error = "context still has " + std::to_string(c->live_servers) +
+                      " live server(s): destroy them first (psx_server_destroy)";
+      return PSX_ERR_ARGUMENT;
+    }
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     try {
       resolve_timers(c);
     } catch (...) {
     }
+    // library handles first, while their stream and the memory they may
+    // still reference exist; then the plans, the memory and the streams
+    release_blas(c);
     release_tile_cache(c);
     release_sht_plans(c);
     for (auto& kv : c->allocations) (void)hipFree(kv.first);
     c->allocations.clear();
-    release_blas(c);
     release_stage(c);
     (void)hipStreamDestroy(c->stream);
   }

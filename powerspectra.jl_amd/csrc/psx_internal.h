@@ -95,6 +95,10 @@ struct Ctx {
   hipStream_t xstream = nullptr;
   // one event per l1 band of a banded contraction
   hipEvent_t band_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // servers created on this context and not yet destroyed: their worker
+  // threads, arenas and resident spectra live on the context, so it is not
+  // destroyed while any is alive
+  int live_servers = 0;
 };
 constexpr size_t kMaxBands = 5;
 void ensure_band_events(Ctx* c);
@@ -132,6 +136,10 @@ inline size_t xfer_bytes(std::initializer_list<size_t> pieces) {
   return t;
 }
 void ensure_tables(Ctx* c, int nmax_index);
+// entries per table covering indices up to nmax_index, and the host-side
+// build of the four tables (n entries each, index m at m + kTabOff)
+int table_entries(int nmax_index);
+void build_tables_host(int n, double* G, double* Z, double* D, double* IG);
 void release_blas(Ctx* c);
 void release_stage(Ctx* c);
 void release_tile_cache(Ctx* c);

@@ -421,7 +421,7 @@ int psx_server_create(psx_ctx* hctx, int slots, size_t arena_bytes, int lmax_max
   S->ctx = c;
   const int st = sguard(S, [&] {
     PSX_REQUIRE(slots >= 1 && slots <= 16, PSX_ERR_ARGUMENT, "slots must be 1..16");
-    PSX_REQUIRE(lmax_max >= 0 && lmax_max <= 16383, PSX_ERR_ARGUMENT, "lmax_max must be 0..16383");
+    PSX_REQUIRE(lmax_max >= 0 && lmax_max <= 8191, PSX_ERR_ARGUMENT, "lmax_max must be 0..8191");
     PSX_REQUIRE(arena_bytes >= 4096, PSX_ERR_ARGUMENT, "arena too small");
     S->lmax_max = lmax_max;
     {
@@ -456,6 +456,10 @@ int psx_server_create(psx_ctx* hctx, int slots, size_t arena_bytes, int lmax_max
     delete S;
     return st;
   }
+  {
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->live_servers++;
+  }
   *out = reinterpret_cast<psx_server*>(S);
   return PSX_OK;
 }
@@ -481,6 +485,10 @@ int psx_server_destroy(psx_server* hs) {
   }
   S->cv_done.notify_all();
   free_server(S);
+  {
+    std::lock_guard<std::recursive_mutex> lk(S->ctx->mu);
+    S->ctx->live_servers--;
+  }
   delete S;
   return PSX_OK;
 }

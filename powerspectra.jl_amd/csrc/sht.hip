@@ -1152,6 +1152,12 @@ void release_sht_plans(Ctx* c) {
       ++it;
     }
   }
+  // the library's setup is paired with its cleanup once no plan is left, so
+  // nothing of it is left for the process's exit to tear down
+  if (g_plans.empty() && g_fft_ready) {
+    (void)rocfft_cleanup();
+    g_fft_ready = false;
+  }
 }
 
 namespace {

@@ -199,3 +199,18 @@ def test_errors_stay_with_their_request(pkg, server):
     for t in (good1, bad, good2):
         server.release(t)
     server.evict(500)
+
+
+def test_context_close_refused_while_server_alive(pkg):
+    c = pkg.Context(0)
+    s = pkg.Server(c, slots=1, arena_bytes=1 << 24, lmax_max=64)
+    try:
+        with pytest.raises(pkg.ArgumentError, match="live server"):
+            c.close()
+        # the context is still usable
+        W = mask_like_W(65, seed=3)
+        c.mcm(0, 0, 64, W)
+    finally:
+        s.close()
+    c.close()
+    assert c.handle is None

@@ -261,6 +261,17 @@ def test_binning_matrix(pkg):
     assert np.allclose(P, [[1 / 3] * 3 + [0] * 3, [0] * 3 + [1 / 3] * 3])
 
 
+def test_binning_matrix_reference_kat(pkg):
+    """The reference's own known answer (test/test_util.jl:6-14):
+    binning_matrix([0,1,2], [0,1,3], l -> 1; lmax=3)."""
+    from powerspectra_amd.master import binning_matrix
+    P = binning_matrix([0, 1, 2], [0, 1, 3], lambda l: 1, lmax=3)
+    ref = np.array([[1.0, 0.0, 0.0, 0.0],
+                    [0.0, 1.0, 0.0, 0.0],
+                    [0.0, 0.0, 0.5, 0.5]])
+    assert P.shape == ref.shape and np.allclose(P, ref, rtol=0, atol=1e-15)
+
+
 def test_cut_sky_mask(pkg):
     """Seeded masks: 0 inside the cut and at hole centres, 1 away from both,
     a cosine ramp over the apodisation width, reproducible by seed."""
