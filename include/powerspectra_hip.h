@@ -123,6 +123,16 @@ int psx_mcm(psx_ctx* ctx, int kind, int lmin, int lmax,
             const double* W, int64_t LW, double* M, int64_t ldM);
 int psx_mcm_d(psx_ctx* ctx, int kind, int lmin, int lmax,
               const double* dW, int64_t LW, double* dM, int64_t ldM);
+/* psx_mcm_d without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Same kernels, launch geometry and
+ * bits as psx_mcm_d; everything beyond dW and dM lives in one static device
+ * array of 4 MiB per device, set up by psx_ctx_create and held until the
+ * process ends.  Limits: lmax <= 8191 (PSX_ERR_ARGUMENT above).  Every call
+ * on a device shares that device's array: calls that overlap on one device
+ * (several contexts or threads) race.  Enqueued on the context stream. */
+int psx_mcm_d_noalloc(psx_ctx* ctx, int kind, int lmin, int lmax,
+                      const double* dW, int64_t LW, double* dM, int64_t ldM);
 
 /* Binned mode-coupling matrix P M Q (the binning of reference src/util.jl:3-18
  * applied to one block of psx_mcm): P is nbl x n (leading dimension nbl),

@@ -130,6 +130,7 @@ SIGNATURES = {
     "psx_memcpy_d2h": (_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "psx_mcm": (_int, [_vp, _int, _int, _int, _dp, _i64, _dp, _i64]),
     "psx_mcm_d": (_int, [_vp, _int, _int, _int, _vp, _i64, _vp, _i64]),
+    "psx_mcm_d_noalloc": (_int, [_vp, _int, _int, _int, _vp, _i64, _vp, _i64]),
     "psx_mcm_pm": (_int, [_vp, _int, _int, _dp, _i64, _dp, _dp, _i64]),
     "psx_mcm_pm_d": (_int, [_vp, _int, _int, _vp, _i64, _vp, _vp, _i64]),
     "psx_mcm_set": (_int, [_vp, _int, _int, _i64] + [_dp] * 4 + [_dp] * 5 + [_i64]),

@@ -159,6 +159,17 @@ int psx_mcm_d(psx_ctx* h, int kind, int lmin, int lmax, const double* dW,
   });
 }
 
+int psx_mcm_d_noalloc(psx_ctx* h, int kind, int lmin, int lmax, const double* dW,
+                      int64_t LW, double* dM, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    PSX_REQUIRE(dW && dM, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    run_contraction_noalloc(c, mcm_request(kind, lmin, lmax, dW, LW, dM, ldM));
+  });
+}
+
 int psx_mcm(psx_ctx* h, int kind, int lmin, int lmax, const double* W, int64_t LW,
             double* M, int64_t ldM) {
   Ctx* c = C(h);

@@ -933,4 +933,150 @@ void run_contraction_to_host(Ctx* c, const ContractRequest& rq, double* const* h
   }
 }
 
+// ---- allocation-free mode-coupling path -----------------------------------
+// psx_mcm_d_noalloc runs the psx_mcm_d kernels with everything they need
+// beyond the caller's W and M in one file-scope device array: the four gamma
+// tables (built for lmax <= 8191 and copied in once per device at context
+// creation), the launch's tile list (copied in when it changes) and the
+// per-call record table.  Host code never passes the array's name to a
+// kernel: the device address of each device's copy comes from
+// hipGetSymbolAddress, looked up once per device.  Every call on a device
+// shares that device's array and overwrites it, so calls that overlap on one
+// device (from several contexts or threads) race.
+namespace {
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+// largest table index a call with lmax <= kLmaxCap touches (as ensure_tables
+// is asked for by run_contraction_checked with LW <= 2 lmax + 1)
+constexpr int kNoAllocTabIndex = (2 * kLmaxCap + 1) + 2 * kLmaxCap + 512;
+
+struct NoAllocDevice {
+  char* base = nullptr;  // device address of this device's `scratch`
+  int ntab = 0;          // entries per gamma table at the front of the array
+  bool have_tiles = false;
+  TileKey tiles_key{};   // tile list currently in the array
+};
+
+NoAllocDevice g_noalloc[kNoAllocDevices];
+std::mutex g_noalloc_mu;
+
+struct HostTiles {
+  std::vector<Tile> tiles;
+  Geometry geo;
+};
+std::map<TileKey, HostTiles> g_noalloc_tiles;  // host-side tile lists
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+void noalloc_init_device(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  PSX_REQUIRE(device >= 0 && device < kNoAllocDevices, PSX_ERR_ARGUMENT,
+              "device index beyond the allocation-free path's table");
+  NoAllocDevice& d = g_noalloc[device];
+  if (d.base) return;
+  void* p = nullptr;
+  PSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PSX_REQUIRE(p != nullptr, PSX_ERR_DEVICE, "hipGetSymbolAddress returned a null address");
+  const int n = table_entries(kNoAllocTabIndex);
+  const size_t tb = sizeof(double) * (size_t)n;
+  PSX_REQUIRE(4 * tb <= kNoAllocBytes, PSX_ERR_DEVICE, "static scratch smaller than its tables");
+  std::vector<double> t(4 * (size_t)n);
+  build_tables_host(n, t.data(), t.data() + n, t.data() + 2 * (size_t)n,
+                    t.data() + 3 * (size_t)n);
+  PSX_HIP(hipMemcpy(p, t.data(), 4 * tb, hipMemcpyHostToDevice));
+  d.ntab = n;
+  d.base = (char*)p;
+}
+
+void run_contraction_noalloc(Ctx* c, const ContractRequest& rq0) {
+  PSX_REQUIRE(rq0.epi == EPI_MCM && nv_of(rq0) == 1, PSX_ERR_ARGUMENT,
+              "the allocation-free path computes one mode-coupling block");
+  PSX_REQUIRE(rq0.lmin >= 0 && rq0.lmax >= rq0.lmin, PSX_ERR_ARGUMENT, "bad lmin/lmax");
+  PSX_REQUIRE(rq0.lmax <= kLmaxCap, PSX_ERR_ARGUMENT,
+              "psx_mcm_d_noalloc: lmax above 8191 is beyond its static scratch");
+  PSX_REQUIRE(rq0.LW >= 1 && rq0.LW < (1 << 24), PSX_ERR_ARGUMENT, "bad W length");
+  ContractRequest rq = rq0;
+  rq.LW = std::min<int64_t>(rq0.LW, 2 * (int64_t)rq0.lmax + 1);
+  const int n = rq.lmax - rq.lmin + 1;
+  PSX_REQUIRE(rq.ldo >= n, PSX_ERR_AXES, "leading dimension smaller than lmax-lmin+1");
+  const Config cf = pick_config(rq);
+
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  PSX_REQUIRE(c->device >= 0 && c->device < kNoAllocDevices && g_noalloc[c->device].base,
+              PSX_ERR_DEVICE, "static scratch address not set up for this device");
+  NoAllocDevice& d = g_noalloc[c->device];
+  const TileKey key{rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves, 1};
+  auto it = g_noalloc_tiles.find(key);
+  if (it == g_noalloc_tiles.end()) {
+    HostTiles ht;
+    ht.tiles = make_tiles(rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves);
+    ht.geo = check_geometry(ht.tiles, rq.lmax, (int)rq.LW, cf.R, 0);
+    it = g_noalloc_tiles.emplace(key, std::move(ht)).first;
+  }
+  const HostTiles& ht = it->second;
+  const Geometry& g = ht.geo;
+  PSX_REQUIRE(g.tab_need + kTabOff < d.ntab, PSX_ERR_ARGUMENT, "tile geometry: table overflow");
+
+  // [G Z D IG tables][tile list][record table][flag]
+  const RecLayout rl = rec_layout(rq);
+  const size_t tab_bytes = 4 * sizeof(double) * (size_t)d.ntab;
+  const size_t tile_bytes = sizeof(Tile) * ht.tiles.size();
+  const size_t rec_bytes = (size_t)rl.rs * g.wh_len * sizeof(double);
+  const size_t need = align256(tab_bytes) + align256(tile_bytes) + align256(rec_bytes) + 256;
+  PSX_REQUIRE(need <= kNoAllocBytes, PSX_ERR_BUDGET,
+              "psx_mcm_d_noalloc: the call needs " + std::to_string(need) +
+                  " bytes of the " + std::to_string(kNoAllocBytes) + "-byte static scratch");
+  const double* tab = (const double*)d.base;
+  Tile* dtiles = (Tile*)(d.base + align256(tab_bytes));
+  double* rec = (double*)(d.base + align256(tab_bytes) + align256(tile_bytes));
+
+  if (!d.have_tiles || d.tiles_key < key || key < d.tiles_key) {
+    PSX_HIP(hipMemcpyAsync(dtiles, ht.tiles.data(), tile_bytes, hipMemcpyHostToDevice,
+                           c->stream));
+    d.have_tiles = true;
+    d.tiles_key = key;
+  }
+
+  RecArgs ra{};
+  ra.rec = rec;
+  ra.W[0] = rq.dW[0];
+  ra.len = g.wh_len;
+  ra.LW = (int)rq.LW;
+  ra.nv = rl.nv;
+  ra.ne = rl.ne;
+  ra.rs = rl.rs;
+  ra.ft = rl.ft;
+  ra.ft1 = rl.ft1;
+  ra.err = nullptr;
+  hipLaunchKernelGGL(record_kernel, dim3((unsigned)((g.wh_len + 255) / 256)), dim3(256), 0,
+                     c->stream, ra);
+  PSX_HIP(hipGetLastError());
+
+  KArgs ka{};
+  ka.tiles = dtiles;
+  ka.ntiles = (int)ht.tiles.size();
+  ka.lmin = rq.lmin;
+  ka.lmax = rq.lmax;
+  ka.LW = (int)rq.LW;
+  ka.G = tab;
+  ka.Z = tab + d.ntab;
+  ka.D = tab + 2 * (size_t)d.ntab;
+  ka.IG = tab + 3 * (size_t)d.ntab;
+  ka.rec = rec;
+  ka.ldo = rq.ldo;
+  ka.out[0] = rq.dout[0];
+  ka.err = nullptr;
+  const unsigned blocks = (unsigned)((ka.ntiles + cf.waves - 1) / cf.waves);
+  hipLaunchKernelGGL(cf.fn, dim3(blocks), dim3(64 * cf.waves), 0, c->stream, ka);
+  PSX_HIP(hipGetLastError());
+}
+
 }  // namespace psx

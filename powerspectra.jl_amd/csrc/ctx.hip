@@ -191,6 +191,13 @@ int psx_ctx_create(int device, size_t hbm_budget, psx_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     c->cu_count = prop.multiProcessorCount;
+  try {
+    noalloc_init_device(device);
+  } catch (const Error&) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PSX_ERR_DEVICE;
+  }
   *out = reinterpret_cast<psx_ctx*>(c);
   return PSX_OK;
 }

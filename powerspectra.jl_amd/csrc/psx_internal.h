@@ -140,6 +140,10 @@ void ensure_tables(Ctx* c, int nmax_index);
 // build of the four tables (n entries each, index m at m + kTabOff)
 int table_entries(int nmax_index);
 void build_tables_host(int n, double* G, double* Z, double* D, double* IG);
+// The allocation-free mode-coupling path (psx_mcm_d_noalloc): the device
+// address of this device's static scratch array, looked up once per device
+// (called by psx_ctx_create with the device current).
+void noalloc_init_device(int device);
 void release_blas(Ctx* c);
 void release_stage(Ctx* c);
 void release_tile_cache(Ctx* c);
@@ -223,6 +227,9 @@ struct ContractRequest {
 };
 
 void run_contraction(Ctx* c, const ContractRequest& rq);
+// The same through the device's static scratch array only: no device
+// allocation (psx_mcm_d_noalloc).
+void run_contraction_noalloc(Ctx* c, const ContractRequest& rq);
 // Enqueue a contraction on stream s with caller-provided scratch (at least
 // contraction_scratch_bytes(rq)).  Returns the device address of the
 // covariance domain-error flag inside the scratch (nonzero after the kernel
