@@ -74,9 +74,15 @@ enum psx_cov_kind {
 int psx_cov_arity(int kind, int* nW, int* nspectra, int* nratios);
 
 enum psx_precision {
-  PSX_FP64 = 0,   /* all arithmetic in FP64                                   */
-  PSX_MIXED = 1   /* reserved: FP32-MFMA covariance contraction; not available,
-                     psx_cov returns PSX_ERR_ARGUMENT                          */
+  PSX_FP64 = 0,      /* FP64 on the vector ALUs (the reference's arithmetic)     */
+  PSX_MIXED = 1,     /* the contraction against the W spectra on the int8 matrix
+                        cores: the 3j products split exactly into six int8
+                        digits, the W spectra into seven, 27 digit products
+                        accumulated exactly in int32 and combined in FP64
+                        (family products and epilogues stay FP64); error near
+                        1e-14 of the element's terms                          */
+  PSX_FP64_MFMA = 2  /* the same contraction on the FP64 matrix cores
+                        (v_mfma_f64_16x16x4): FP64 products and sums            */
 };
 
 /* ---- context ----------------------------------------------------------- */
@@ -172,7 +178,8 @@ int psx_mcm_set_d(psx_ctx* ctx, int lmin, int lmax, int64_t LW,
  * pointers in exactly the reference loop's argument order; psx_cov_arity
  * gives their counts.  Spectra and ratios are 0-based with length >= lmax+1,
  * W has length LW (reference: workspace.lmax+1).  Fills C[l1,l2] = C[l2,l1]
- * for lmin <= l1 <= l2 <= lmax.  `precision` must be PSX_FP64. */
+ * for lmin <= l1 <= l2 <= lmax.  `precision`: PSX_FP64, or PSX_MIXED /
+ * PSX_FP64_MFMA through the matrix-core pass of psx_cov_batch (not TETE). */
 int psx_cov(psx_ctx* ctx, int kind, int lmin, int lmax,
             const double* const* spectra, const double* const* ratios,
             const double* const* W, int64_t LW, int precision,
@@ -181,6 +188,37 @@ int psx_cov_d(psx_ctx* ctx, int kind, int lmin, int lmax,
               const double* const* dspectra, const double* const* dratios,
               const double* const* dW, int64_t LW, int precision,
               double* dC, int64_t ldC);
+
+/* ---- batches: many W spectra against one family in one pass ------------ */
+/* K blocks of one mode-coupling kind from K W spectra (K = 1..64), the
+ * result of K psx_mcm calls (reference inner_mcm*! of
+ * src/modecoupling.jl:78-159 once per W): every family product is computed
+ * once and contracted against all K spectra.  precision: PSX_FP64 (vector
+ * FP64, groups of up to 8 spectra per launch), PSX_MIXED (int8 matrix cores,
+ * exact digit products; the tests bound the error per element at 1e-9) or
+ * PSX_FP64_MFMA (FP64 matrix cores).  W[k] has LW
+ * entries, M[k] is n x n with leading dimension ldM. */
+int psx_mcm_batch(psx_ctx* ctx, int kind, int lmin, int lmax, int K, const double* const* W,
+                  int64_t LW, int precision, double* const* M, int64_t ldM);
+int psx_mcm_batch_d(psx_ctx* ctx, int kind, int lmin, int lmax, int K,
+                    const double* const* dW, int64_t LW, int precision, double* const* dM,
+                    int64_t ldM);
+/* nblk covariance blocks of one kind (the result of nblk psx_cov calls,
+ * reference loop_cov*! of src/covariance.jl:92-446): spectra, ratios and W
+ * hold the blocks' arguments one block after another, each block in the
+ * reference loop's order with psx_cov_arity's counts (nblk * nS spectra,
+ * nblk * nR ratios, nblk * nW W spectra); C[b] is block b.  The matrix-core
+ * precisions contract up to 64 W spectra per pass (8 TTTT or EEEE blocks,
+ * 16 TTTE, TEEE or TEEE-Planck blocks, 16 TTEE blocks) and take every kind
+ * but TETE, whose two families need two passes (PSX_ERR_ARGUMENT). */
+int psx_cov_batch(psx_ctx* ctx, int kind, int lmin, int lmax, int nblk,
+                  const double* const* spectra, const double* const* ratios,
+                  const double* const* W, int64_t LW, int precision, double* const* C,
+                  int64_t ldC);
+int psx_cov_batch_d(psx_ctx* ctx, int kind, int lmin, int lmax, int nblk,
+                    const double* const* dspectra, const double* const* dratios,
+                    const double* const* dW, int64_t LW, int precision, double* const* dC,
+                    int64_t ldC);
 
 /* ---- FP64 dense linear algebra ----------------------------------------- */
 /* decouple_covmat (reference src/covariance.jl:8-14): out = B1^{-1} Y B2^{-T},

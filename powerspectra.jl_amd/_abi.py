@@ -30,7 +30,7 @@ MCM_00, MCM_02, MCM_PP, MCM_MM = 0, 1, 2, 3
 # enum psx_cov_kind
 COV_TTTT, COV_EEEE, COV_TTTE, COV_TETE, COV_TEEE, COV_TEEE_PLANCK, COV_TTEE = range(7)
 # enum psx_precision
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_MFMA = 0, 1, 2
 # enum psx_request_type
 REQ_MCM, REQ_MCM_PM, REQ_MCM_SET, REQ_COV = 0, 1, 2, 3
 # enum psx_sched
@@ -140,6 +140,16 @@ SIGNATURES = {
     "psx_cov_d": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                          ctypes.POINTER(_vp), _i64, _int, _vp, _i64]),
     "psx_decouple_covmat": (_int, [_vp, _int, _dp, _dp, _dp, _dp]),
+    "psx_mcm_batch": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dp), _i64, _int,
+                             ctypes.POINTER(_dp), _i64]),
+    "psx_mcm_batch_d": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_vp), _i64, _int,
+                               ctypes.POINTER(_vp), _i64]),
+    "psx_cov_batch": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_dp),
+                             ctypes.POINTER(_dp), ctypes.POINTER(_dp), _i64, _int,
+                             ctypes.POINTER(_dp), _i64]),
+    "psx_cov_batch_d": (_int, [_vp, _int, _int, _int, _int, ctypes.POINTER(_vp),
+                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64, _int,
+                               ctypes.POINTER(_vp), _i64]),
     "psx_lu_solve": (_int, [_vp, _int, _dp, _i64, _dp, _i64, _int]),
 }
 
@@ -333,6 +343,56 @@ class Context:
         self.check(self.lib.psx_cov(self._h, kind, lmin, lmax, sp, rp, wp, LW, precision,
                                     dptr(out), n), "psx_cov")
         return out
+
+    def mcm_batch(self, kind: int, lmin: int, lmax: int, Ws, precision: int = FP64):
+        """K mode-coupling blocks of one kind from K W spectra in one pass
+        (psx_mcm_batch); returns K n x n Fortran-ordered arrays."""
+        Ws = [np.ascontiguousarray(w, dtype=np.float64) for w in Ws]
+        if not Ws:
+            raise ArgumentError("at least one W spectrum")
+        LW = Ws[0].size
+        if any(w.size != LW for w in Ws):
+            raise ArgumentError("all W spectra must have the same length")
+        n = _block_size(lmin, lmax)
+        outs = [np.empty((n, n), order="F") for _ in Ws]
+        wp = (_dp * len(Ws))(*[dptr(w) for w in Ws])
+        op = (_dp * len(Ws))(*[dptr(o) for o in outs])
+        self.check(self.lib.psx_mcm_batch(self._h, kind, lmin, lmax, len(Ws), wp, LW, precision,
+                                          op, n), "psx_mcm_batch")
+        return outs
+
+    def cov_batch(self, kind: int, lmin: int, lmax: int, spectra, ratios, W,
+                  precision: int = FP64):
+        """Covariance blocks of one kind (psx_cov_batch): spectra[b], ratios[b]
+        and W[b] are block b's arguments in the reference loop's order."""
+        nW, nS, nR = cov_arity(kind)
+        nb = len(W)
+        _block_size(lmin, lmax)
+        if len(spectra) != nb or len(ratios) != nb:
+            raise ArgumentError("spectra, ratios and W must list the same blocks")
+        sp, rt, ws = [], [], []
+        for b in range(nb):
+            if len(W[b]) != nW or len(spectra[b]) != nS or len(ratios[b]) != nR:
+                raise ArgumentError(f"covariance kind {kind} takes {nW} W, {nS} spectra, "
+                                    f"{nR} ratios per block")
+            sp += [np.ascontiguousarray(x, dtype=np.float64) for x in spectra[b]]
+            rt += [np.ascontiguousarray(x, dtype=np.float64) for x in ratios[b]]
+            ws += [np.ascontiguousarray(x, dtype=np.float64) for x in W[b]]
+        for x in sp + rt:
+            if x.size < lmax + 1:
+                raise ArgumentError("spectra and noise ratios need at least lmax+1 entries")
+        LW = ws[0].size
+        if any(w.size != LW for w in ws):
+            raise ArgumentError("all W spectra must have the same length")
+        n = lmax - lmin + 1
+        outs = [np.empty((n, n), order="F") for _ in range(nb)]
+        spp = (_dp * max(len(sp), 1))(*[dptr(x) for x in sp])
+        rtp = (_dp * max(len(rt), 1))(*[dptr(x) for x in rt])
+        wsp = (_dp * len(ws))(*[dptr(x) for x in ws])
+        op = (_dp * nb)(*[dptr(o) for o in outs])
+        self.check(self.lib.psx_cov_batch(self._h, kind, lmin, lmax, nb, spp, rtp, wsp, LW,
+                                          precision, op, n), "psx_cov_batch")
+        return outs
 
     def map2alm(self, maps, nside: int, lmax: int, niter: int = 3) -> np.ndarray:
         """a_lm (complex, m-major, shape (nmaps, nalm)) of ring-ordered maps."""

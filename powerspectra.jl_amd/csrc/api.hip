@@ -1,7 +1,9 @@
 // extern "C" entry points for mode-coupling matrices and covariance blocks.
 #include "psx_internal.h"
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace psx {
 
@@ -112,10 +114,8 @@ ContractRequest cov_request(int kind, int lmin, int lmax, const double* const* d
   int nW = 0, nS = 0, nR = 0;
   PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
               "unknown covariance kind");
-  PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED, PSX_ERR_ARGUMENT,
-              "precision must be PSX_FP64 or PSX_MIXED");
-  PSX_REQUIRE(precision == PSX_FP64, PSX_ERR_ARGUMENT,
-              "PSX_MIXED covariance contraction is not available in this build");
+  PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED || precision == PSX_FP64_MFMA,
+              PSX_ERR_ARGUMENT, "precision must be PSX_FP64, PSX_MIXED or PSX_FP64_MFMA");
   PSX_REQUIRE(dW && dC && dspectra, PSX_ERR_ARGUMENT, "null pointer");
   for (int i = 0; i < nW; ++i) PSX_REQUIRE(dW[i], PSX_ERR_ARGUMENT, "null W spectrum");
   for (int i = 0; i < nS; ++i) PSX_REQUIRE(dspectra[i], PSX_ERR_ARGUMENT, "null spectrum");
@@ -132,6 +132,104 @@ ContractRequest cov_request(int kind, int lmin, int lmax, const double* const* d
   }
   rq.precision = precision;
   return rq;
+}
+
+void check_precision(int precision) {
+  PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED || precision == PSX_FP64_MFMA,
+              PSX_ERR_ARGUMENT, "precision must be PSX_FP64, PSX_MIXED or PSX_FP64_MFMA");
+}
+
+// K blocks of one mode-coupling kind on device buffers: FP64 on the vector
+// ALUs in groups of 8, 4, 2, 1 spectra per launch, or the matrix-core pass.
+void mcm_batch_dev(Ctx* c, int kind, int lmin, int lmax, int K, const double* const* dW,
+                   int64_t LW, int precision, double* const* dM, int64_t ldM) {
+  PSX_REQUIRE(K >= 1 && K <= 64, PSX_ERR_ARGUMENT, "between 1 and 64 W spectra");
+  PSX_REQUIRE(dW && dM, PSX_ERR_ARGUMENT, "null pointer");
+  for (int k = 0; k < K; ++k) PSX_REQUIRE(dW[k] && dM[k], PSX_ERR_ARGUMENT, "null pointer");
+  const int fam = mcm_family(kind);
+  if (precision == PSX_FP64) {
+    for (int k0 = 0; k0 < K;) {
+      const int left = K - k0;
+      const int g = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+      ContractRequest rq = block_request(lmin, lmax, LW, ldM);
+      rq.nfam[fam] = g;
+      for (int i = 0; i < g; ++i) {
+        rq.dW[i] = dW[k0 + i];
+        rq.dout[i] = dM[k0 + i];
+      }
+      run_contraction(c, rq);
+      k0 += g;
+    }
+    return;
+  }
+  MixRequest mr;
+  mr.lmin = lmin;
+  mr.lmax = lmax;
+  mr.LW = LW;
+  mr.fam = fam;
+  mr.ncol = K;
+  for (int k = 0; k < K; ++k) {
+    mr.dW[k] = dW[k];
+    mr.dout[k] = dM[k];
+  }
+  mr.epi = EPI_MCM;
+  mr.ldo = ldM;
+  mr.mode = precision;
+  run_mixed(c, mr);
+}
+
+// nblk covariance blocks of one kind on device buffers; the arrays hold the
+// blocks' spectra, ratios and W spectra one block after another, each in the
+// reference loop's argument order.
+void cov_batch_dev(Ctx* c, int kind, int lmin, int lmax, int nblk,
+                   const double* const* dspectra, const double* const* dratios,
+                   const double* const* dW, int64_t LW, int precision, double* const* dC,
+                   int64_t ldC) {
+  int nW = 0, nS = 0, nR = 0;
+  PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
+              "unknown covariance kind");
+  PSX_REQUIRE(nblk >= 1, PSX_ERR_ARGUMENT, "at least one block");
+  PSX_REQUIRE(dW && dC && dspectra && (nR == 0 || dratios), PSX_ERR_ARGUMENT, "null pointer");
+  if (precision == PSX_FP64) {
+    for (int b = 0; b < nblk; ++b)
+      run_contraction(c, cov_request(kind, lmin, lmax, dspectra + b * nS,
+                                     nR ? dratios + b * nR : nullptr, dW + b * nW, LW,
+                                     precision, dC[b], ldC));
+    return;
+  }
+  const int fam = mix_family_of_cov(kind);
+  const int per = std::min(16, 64 / nW);  // blocks per matrix-core pass
+  for (int b0 = 0; b0 < nblk; b0 += per) {
+    const int nb = std::min(per, nblk - b0);
+    MixRequest mr;
+    mr.lmin = lmin;
+    mr.lmax = lmax;
+    mr.LW = LW;
+    mr.fam = fam;
+    mr.ncol = nb * nW;
+    mr.epi = EPI_COV;
+    mr.cov_kind = kind;
+    mr.nblk = nb;
+    mr.nW = nW;
+    mr.ldo = ldC;
+    mr.mode = precision;
+    for (int b = 0; b < nb; ++b) {
+      const int B = b0 + b;
+      PSX_REQUIRE(dC[B], PSX_ERR_ARGUMENT, "null output");
+      for (int w = 0; w < nW; ++w) {
+        PSX_REQUIRE(dW[B * nW + w], PSX_ERR_ARGUMENT, "null W spectrum");
+        mr.dW[b * nW + w] = dW[B * nW + w];
+      }
+      for (int i = 0; i < 4; ++i) {
+        mr.dspec[b][i] = i < nS ? dspectra[B * nS + i] : nullptr;
+        mr.dratio[b][i] = i < nR ? dratios[B * nR + i] : nullptr;
+        if (i < nS) PSX_REQUIRE(mr.dspec[b][i], PSX_ERR_ARGUMENT, "null spectrum");
+        if (i < nR) PSX_REQUIRE(mr.dratio[b][i], PSX_ERR_ARGUMENT, "null noise ratio");
+      }
+      mr.dout[b] = dC[B];
+    }
+    run_mixed(c, mr);
+  }
 }
 
 }  // namespace
@@ -326,6 +424,11 @@ int psx_cov_d(psx_ctx* h, int kind, int lmin, int lmax, const double* const* dsp
     ContractRequest rq =
         cov_request(kind, lmin, lmax, dspectra, dratios, dW, LW, precision, dC, ldC);
     PSX_HIP(hipSetDevice(c->device));
+    if (precision != PSX_FP64) {
+      double* const outs[1] = {dC};
+      cov_batch_dev(c, kind, lmin, lmax, 1, dspectra, dratios, dW, LW, precision, outs, ldC);
+      return;
+    }
     run_contraction(c, rq);
   });
 }
@@ -367,7 +470,127 @@ int psx_cov(psx_ctx* h, int kind, int lmin, int lmax, const double* const* spect
       dw[i] = (const double*)p;
     }
     ContractRequest rq = cov_request(kind, lmin, lmax, ds, dr, dw, LW, precision, dC, n);
+    if (precision != PSX_FP64) {
+      double* const outs[1] = {dC};
+      cov_batch_dev(c, kind, lmin, lmax, 1, ds, dr, dw, LW, precision, outs, n);
+      d2h_matrix(c, Cm, ldC, dC, n, n, n);
+      return;
+    }
     run_contraction_to_host(c, rq, &Cm, ldC);
+  });
+}
+
+
+int psx_mcm_batch_d(psx_ctx* h, int kind, int lmin, int lmax, int K, const double* const* dW,
+                    int64_t LW, int precision, double* const* dM, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    check_precision(precision);
+    PSX_HIP(hipSetDevice(c->device));
+    mcm_batch_dev(c, kind, lmin, lmax, K, dW, LW, precision, dM, ldM);
+  });
+}
+
+int psx_mcm_batch(psx_ctx* h, int kind, int lmin, int lmax, int K, const double* const* W,
+                  int64_t LW, int precision, double* const* M, int64_t ldM) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldM);
+    check_precision(precision);
+    PSX_REQUIRE(K >= 1 && K <= 64, PSX_ERR_ARGUMENT, "between 1 and 64 W spectra");
+    PSX_REQUIRE(W && M, PSX_ERR_ARGUMENT, "null pointer");
+    for (int k = 0; k < K; ++k) PSX_REQUIRE(W[k] && M[k], PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    const size_t wb = sizeof(double) * LW, mb = sizeof(double) * n * (size_t)n;
+    std::vector<size_t> pieces((size_t)2 * K);
+    for (int k = 0; k < K; ++k) {
+      pieces[k] = wb;
+      pieces[K + k] = mb;
+    }
+    size_t total = 0;
+    for (size_t b : pieces) total += (b + 255) & ~size_t(255);
+    XferBuf x(c, total);
+    std::vector<const double*> dW(K);
+    std::vector<double*> dM(K);
+    for (int k = 0; k < K; ++k) {
+      double* d = x.take(wb);
+      h2d(c, d, W[k], wb);
+      dW[k] = d;
+    }
+    for (int k = 0; k < K; ++k) dM[k] = x.take(mb);
+    mcm_batch_dev(c, kind, lmin, lmax, K, dW.data(), LW, precision, dM.data(), n);
+    std::vector<D2HMatrix> out;
+    for (int k = 0; k < K; ++k) out.push_back({M[k], ldM, dM[k], n, n, n});
+    d2h_matrices(c, out.data(), K);
+  });
+}
+
+int psx_cov_batch_d(psx_ctx* h, int kind, int lmin, int lmax, int nblk,
+                    const double* const* dspectra, const double* const* dratios,
+                    const double* const* dW, int64_t LW, int precision, double* const* dC,
+                    int64_t ldC) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldC);
+    check_precision(precision);
+    PSX_HIP(hipSetDevice(c->device));
+    cov_batch_dev(c, kind, lmin, lmax, nblk, dspectra, dratios, dW, LW, precision, dC, ldC);
+  });
+}
+
+int psx_cov_batch(psx_ctx* h, int kind, int lmin, int lmax, int nblk,
+                  const double* const* spectra, const double* const* ratios,
+                  const double* const* W, int64_t LW, int precision, double* const* Cm,
+                  int64_t ldC) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, ldC);
+    check_precision(precision);
+    int nW = 0, nS = 0, nR = 0;
+    PSX_REQUIRE(psx_cov_arity(kind, &nW, &nS, &nR) == PSX_OK, PSX_ERR_ARGUMENT,
+                "unknown covariance kind");
+    PSX_REQUIRE(nblk >= 1 && nblk <= 64, PSX_ERR_ARGUMENT, "between 1 and 64 blocks");
+    PSX_REQUIRE(spectra && W && Cm && (nR == 0 || ratios), PSX_ERR_ARGUMENT, "null pointer");
+    for (int i = 0; i < nblk * nS; ++i) PSX_REQUIRE(spectra[i], PSX_ERR_ARGUMENT, "null spectrum");
+    for (int i = 0; i < nblk * nR; ++i)
+      PSX_REQUIRE(ratios[i], PSX_ERR_ARGUMENT, "null noise ratio");
+    for (int i = 0; i < nblk * nW; ++i) PSX_REQUIRE(W[i], PSX_ERR_ARGUMENT, "null W spectrum");
+    for (int b = 0; b < nblk; ++b) PSX_REQUIRE(Cm[b], PSX_ERR_ARGUMENT, "null output");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1;
+    const size_t sb = sizeof(double) * (lmax + 1), wb = sizeof(double) * LW;
+    const size_t cb = sizeof(double) * n * (size_t)n;
+    const size_t total = (size_t)nblk * (nS + nR) * ((sb + 255) & ~size_t(255)) +
+                         (size_t)nblk * nW * ((wb + 255) & ~size_t(255)) +
+                         (size_t)nblk * ((cb + 255) & ~size_t(255));
+    XferBuf x(c, total);
+    std::vector<const double*> ds((size_t)nblk * nS), dr((size_t)nblk * nR),
+        dw((size_t)nblk * nW);
+    std::vector<double*> dc(nblk);
+    for (size_t i = 0; i < ds.size(); ++i) {
+      double* d = x.take(sb);
+      h2d(c, d, spectra[i], sb);
+      ds[i] = d;
+    }
+    for (size_t i = 0; i < dr.size(); ++i) {
+      double* d = x.take(sb);
+      h2d(c, d, ratios[i], sb);
+      dr[i] = d;
+    }
+    for (size_t i = 0; i < dw.size(); ++i) {
+      double* d = x.take(wb);
+      h2d(c, d, W[i], wb);
+      dw[i] = d;
+    }
+    for (int b = 0; b < nblk; ++b) dc[b] = x.take(cb);
+    cov_batch_dev(c, kind, lmin, lmax, nblk, ds.data(), nR ? dr.data() : nullptr, dw.data(), LW,
+                  precision, dc.data(), n);
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<D2HMatrix> out;
+    for (int b = 0; b < nblk; ++b) out.push_back({Cm[b], ldC, dc[b], n, n, n});
+    d2h_matrices(c, out.data(), nblk);
   });
 }
 

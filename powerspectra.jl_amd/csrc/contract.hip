@@ -43,6 +43,7 @@
 // Lane tables are read with vector memory loads (in-order vmcnt), uniform
 // ones with scalar loads, so neither waits on the other.
 #include "psx_internal.h"
+#include "cov_epi.h"
 
 #include <algorithm>
 #include <cmath>
@@ -91,65 +92,6 @@ __device__ __forceinline__ T sload(const T* p, int64_t i) {
 }
 
 __device__ __forceinline__ int ufl(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// ---- covariance epilogues (reference src/covariance.jl) -------------------
-// xi[] is in family order; see cov_layout() for the W -> xi mapping.
-__device__ __forceinline__ double sqc(double x, bool& bad) {
-  if (x < 0) bad = true;
-  return sqrt(x);
-}
-
-template <int COVK>
-__device__ __forceinline__ double cov_combine(int a, int b, const double* xi,
-                                              const KArgs& A, bool& bad) {
-  const double* const* S = A.spec;
-  const double* const* R = A.ratio;
-  if constexpr (COVK == PSX_COV_TTTT || COVK == PSX_COV_EEEE) {
-    // :110-118 / :171-179
-    const double *ip = S[0], *jq = S[1], *iq = S[2], *jp = S[3];
-    const double *rip = R[0], *rjq = R[1], *riq = R[2], *rjp = R[3];
-    return sqc(ip[a] * ip[b] * jq[a] * jq[b], bad) * xi[0] +
-           sqc(iq[a] * iq[b] * jp[a] * jp[b], bad) * xi[1] +
-           sqc(ip[a] * ip[b], bad) * xi[2] * rjq[a] * rjq[b] +
-           sqc(jq[a] * jq[b], bad) * xi[3] * rip[a] * rip[b] +
-           sqc(iq[a] * iq[b], bad) * xi[4] * rjp[a] * rjp[b] +
-           sqc(jp[a] * jp[b], bad) * xi[5] * riq[a] * riq[b] +
-           xi[6] * rip[a] * rjq[a] * rip[b] * rjq[b] +
-           xi[7] * riq[a] * rjp[a] * riq[b] * rjp[b];
-  } else if constexpr (COVK == PSX_COV_TTTE) {
-    // :225-230
-    const double *TTip = S[0], *TTjp = S[1], *TEiq = S[2], *TEjq = S[3];
-    const double *rip = R[0], *rjp = R[1];
-    return (sqc(TTip[a] * TTip[b], bad) * (TEjq[a] + TEjq[b]) * xi[0] +
-            sqc(TTjp[a] * TTjp[b], bad) * (TEiq[a] + TEiq[b]) * xi[1] +
-            (TEjq[a] + TEjq[b]) * xi[2] * rip[a] * rip[b] +
-            (TEiq[a] + TEiq[b]) * xi[3] * rjp[a] * rjp[b]) /
-           2;
-  } else if constexpr (COVK == PSX_COV_TETE) {
-    // :292-297; xi[0] = Ξ_TT(W2), xi[1..4] = Ξ_TE(W1, W3, W4, W5)
-    const double *TTip = S[0], *EEjq = S[1], *TEiq = S[2], *TEjp = S[3];
-    const double *rTTip = R[0], *rPPjq = R[1];
-    return sqc(TTip[a] * TTip[b] * EEjq[a] * EEjq[b], bad) * xi[1] +
-           0.5 * (TEiq[a] * TEjp[b] + TEjp[a] * TEiq[b]) * xi[0] +
-           sqc(TTip[a] * TTip[b], bad) * xi[2] * rPPjq[a] * rPPjq[b] +
-           sqc(EEjq[a] * EEjq[b], bad) * xi[3] * rTTip[a] * rTTip[b] +
-           xi[4] * rTTip[a] * rTTip[b] * rPPjq[a] * rPPjq[b];
-  } else if constexpr (COVK == PSX_COV_TEEE || COVK == PSX_COV_TEEE_PLANCK) {
-    // :362-367 / :392-397
-    const double *EEjq = S[0], *EEjp = S[1], *TEip = S[2], *TEiq = S[3];
-    const double *rjq = R[0], *rjp = R[1];
-    return (sqc(EEjq[a] * EEjq[b], bad) * (TEip[a] + TEip[b]) * xi[0] +
-            sqc(EEjp[a] * EEjp[b], bad) * (TEiq[a] + TEiq[b]) * xi[1] +
-            (TEip[a] + TEip[b]) * xi[2] * rjq[a] * rjq[b] +
-            (TEiq[a] + TEiq[b]) * xi[3] * rjp[a] * rjp[b]) /
-           2;
-  } else {  // PSX_COV_TTEE, :438-441
-    const double *TEip = S[0], *TEiq = S[1], *TEjq = S[2], *TEjp = S[3];
-    return ((TEip[a] * TEjq[b] + TEjq[a] * TEip[b]) * xi[0] +
-            (TEiq[a] * TEjp[b] + TEjp[a] * TEiq[b]) * xi[1]) /
-           2;
-  }
-}
 
 // ---- the contraction kernel ---------------------------------------------
 // Uniform per-l3 data lives in one interleaved record table: record l3 holds
@@ -331,7 +273,7 @@ contract_kernel(KArgs A) {
         A.out[k][i21] = (2.0 * L1 + 1.0) * xi[k];
       }
     } else {
-      const double v = cov_combine<COVK>(l1, l2, xi, A, bad);
+      const double v = cov_combine<COVK>(l1, l2, xi, A.spec, A.ratio, bad);
       A.out[0][i12] = v;
       A.out[0][i21] = v;
     }
@@ -705,6 +647,27 @@ Config pick_config(const ContractRequest& rq) {
       return cfg<PSX_R_SET, 1, 2, 1, 1, EPI_MCM, 0, PSX_WPE_SET, PSX_WAVES_SET>("psx_mcm_set");
     if (f[0] == 1 && f[1] == 1 && f[2] == 1 && f[3] == 1)
       return cfg<PSX_R_SET, 1, 1, 1, 1, EPI_MCM, 0, PSX_WPE_SET>("psx_mcm_set1");
+    // groups of 2, 4 or 8 spectra of one family (batches of blocks in FP64)
+    if (f[1] == 0 && f[2] == 0 && f[3] == 0) {
+      if (f[0] == 2) return cfg<4, 2, 0, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[0] == 4) return cfg<4, 4, 0, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[0] == 8) return cfg<4, 8, 0, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+    }
+    if (f[0] == 0 && f[2] == 0 && f[3] == 0) {
+      if (f[1] == 2) return cfg<4, 0, 2, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[1] == 4) return cfg<4, 0, 4, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[1] == 8) return cfg<4, 0, 8, 0, 0, EPI_MCM, 0>("psx_mcm_group");
+    }
+    if (f[0] == 0 && f[1] == 0 && f[3] == 0) {
+      if (f[2] == 2) return cfg<4, 0, 0, 2, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[2] == 4) return cfg<4, 0, 0, 4, 0, EPI_MCM, 0>("psx_mcm_group");
+      if (f[2] == 8) return cfg<4, 0, 0, 8, 0, EPI_MCM, 0>("psx_mcm_group");
+    }
+    if (f[0] == 0 && f[1] == 0 && f[2] == 0) {
+      if (f[3] == 2) return cfg<4, 0, 0, 0, 2, EPI_MCM, 0>("psx_mcm_group");
+      if (f[3] == 4) return cfg<4, 0, 0, 0, 4, EPI_MCM, 0>("psx_mcm_group");
+      if (f[3] == 8) return cfg<4, 0, 0, 0, 8, EPI_MCM, 0>("psx_mcm_group");
+    }
   } else {
     constexpr int RC = PSX_R_COV;
     switch (rq.cov_kind) {

@@ -222,6 +222,7 @@ int psx_ctx_destroy(psx_ctx* h) {
     // still reference exist; then the plans, the memory and the streams
     release_blas(c);
     release_tile_cache(c);
+    release_mix_tiles(c);
     release_sht_plans(c);
     for (auto& kv : c->allocations) (void)hipFree(kv.first);
     c->allocations.clear();

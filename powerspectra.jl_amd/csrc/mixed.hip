@@ -1,0 +1,824 @@
+// Many-W contraction on the matrix cores: K mask spectra contracted against
+// one family of 3j products in one pass, for batches of mode-coupling blocks
+// (K blocks of one kind from K W spectra) and of covariance blocks (nblk
+// blocks of one kind, nW W spectra each).  The reference contracts each W
+// separately inside its per-pair loops (src/modecoupling.jl:3-66,78-159;
+// src/covariance.jl:92-446); here every family product is computed once and
+// multiplied into all K spectra as a matrix product.
+//
+// Per anti-diagonal S = l1 + l2 the contraction is a true GEMM:
+//   X_k(l1, S) = sum_c A(l1, c) B_k(l3),   l3 = S - 2c (- 1 for the odd family)
+// with A the family product of the closed forms in contract.hip and
+// B_k = What_k = (2 l3 + 1) W_k(l3) / (4 pi).  Rows are 16 consecutive l1
+// (one MFMA row group), the inner index is c, columns are the K spectra.
+//
+// Precision PSX_MIXED runs the product on v_mfma_i32_16x16x64_i8 with an
+// error-free split of both operands into balanced base-256 digits
+// (d_j in [-128, 127], x = sum_j d_j 256^j exactly):
+//   * a power-of-two envelope env(l3) >= max_k |What_k(l3)| / c_k (c_k a
+//     power of two per spectrum) is moved from B into A:
+//     A' = A env,  B'_k = What_k / (c_k env) in [-1, 1],
+//     so B' has no dynamic range left across l3 and takes one fixed-point
+//     scale 2^54 for all l3 (7 digits: a spectrum far below the envelope
+//     at some l3 keeps its relative precision there);
+//   * A' takes one exponent per row over its whole family: the exact maximum
+//     of |A'| over the row (a first pass over c), scaled to < 2^46 (6
+//     digits, split on the fly);
+//   * the 27 digit products with s + t >= 5 accumulate in int32 (exact:
+//     below 2^30 per diagonal at lmax 8191) over the whole family, seven
+//     diagonals per output;
+//   * the diagonals are combined in FP64 at the end.
+// The dropped products (s + t <= 4) and the rounding of A' to 46 and B' to
+// 54 bits put the error near 1e-14 of the element's terms (emulated against
+// long double on mask-like spectra before this kernel was written: 15
+// products of five digits each left 6e-10 of the element, 21 of six 3e-12;
+// with six digits for both, a TE block whose terms cancel 1400x still lost
+// 1e-12 of its terms to B' quantised against the envelope; measured per
+// block against the FP64 kernels and the oracle by the tests).
+//
+// Precision PSX_FP64_MFMA runs the same tiles on v_mfma_f64_16x16x4_f64 with
+// A and B in FP64 (no splitting): exact FP64 products, a different
+// summation order than the VALU kernels.
+#include "cov_epi.h"
+#include "psx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace psx {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxCols = 64;     // W spectra in one pass
+constexpr int kMaxBlk = 16;      // covariance blocks in one pass
+constexpr int kNd = 6;           // int8 digits of A' (split on the fly)
+constexpr int kNdB = 7;          // int8 digits of B' (precomputed per call)
+constexpr int kDmin = 5;         // digit products with s + t >= kDmin are kept
+constexpr int kNdiag = (kNd - 1) + (kNdB - 1) - kDmin + 1;  // diagonals s + t = 5 .. 11
+constexpr int kFix = 46;         // fixed-point bits of A' (|x| <= 2^46)
+constexpr int kFixB = 54;        // fixed-point bits of B' (|x| <= 2^54)
+// 1.5 * 2^52 rounds a double to an integer in the low mantissa bits;
+// 0x808080808080 biases the 48-bit integer so that its bytes are the
+// balanced digits plus 128
+constexpr double kMagic = 6755399441055744.0 + (double)0x808080808080ull;
+constexpr double kInv4PiM = 0.07957747154594766788444188168625718;
+
+enum MixMode : int { MIX_I8 = 0, MIX_F64 = 1 };
+
+struct MTile {
+  int S, x0, lo, hi;  // anti-diagonal, first row of the row groups, owned rows
+};
+
+struct MixArgs {
+  const MTile* tiles;
+  int ntiles;
+  int lmin, lmax, LW;
+  const double* G;
+  const double* Z;
+  const double* D;
+  const double* IG;
+  const double* env;        // env[l3] for l3 < envlen
+  const v4i* bdig;          // MIX_I8: digits, [t][p][h/16][kpad] x 16 bytes
+  const double* bf64;       // MIX_F64: What' table, [p][h][kpad]
+  int hblocks;              // 16-h blocks per (digit, parity)
+  int kpad;                 // columns padded to a multiple of 16
+  int ncol;                 // K
+  const double* colscale;   // c_k
+  double* const* out;       // device table: EPI_MCM K outputs, EPI_COV nblk outputs
+  int64_t ldo;
+  int nW;                   // covariance: W spectra per block
+  const double* const* spec;   // device table [blk][4]
+  const double* const* ratio;  // device table [blk][4]
+  int* err;
+};
+
+__device__ __forceinline__ int ufl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Family product (without the kappa factor of the epilogue) at row l1,
+// anti-diagonal S and summation index c from the staged factors:
+// u1 = T[l1 - c (- 1)], u2 = T[l2 - c (- 1)], v = (row-independent part) env.
+// The spin-2 polynomials (exact integers for lmax <= 8191):
+//   E = P1 + P2 - 1 - l3(l3 + 1),  Q = E^2 - kappa  (02: p Q, ++: p Q^2)
+//   D1 = P1 + P2 - l3 (l3 + 1) - 2  (--: z-product D1^2, l3 odd)
+template <int FAM>
+__device__ __forceinline__ double fam_value(double u1, double u2, double v, double sig,
+                                            double kap, int l3) {
+  const double p = (u1 * u2) * v;
+  if constexpr (FAM == FAM_00) {
+    return p;
+  } else if constexpr (FAM == FAM_02 || FAM == FAM_PE) {
+    const double T = (double)l3 * (double)(l3 + 1);
+    const double E = sig - T;
+    const double Q = fma(E, E, -kap);
+    return FAM == FAM_02 ? p * Q : (p * Q) * Q;
+  } else {
+    // l3 is the odd index here: D1 = P1 + P2 - l3 (l3 + 1) - 2
+    const double T1 = (double)l3 * (double)(l3 + 1) + 1.0;
+    const double D1 = sig - T1;
+    return p * (D1 * D1);
+  }
+}
+
+// bytes of the biased 48-bit integer of x (|x| <= 2^46)
+__device__ __forceinline__ void split_words(double x, uint32_t& lo, uint32_t& hi) {
+  const double y = x + kMagic;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(y);
+  lo = (uint32_t)b;
+  hi = (uint32_t)(b >> 32);
+}
+
+// Digit words of four values a, b, c, d (their split words): word t holds
+// digit t of a, b, c, d in bytes 0..3, as signed bytes.
+__device__ __forceinline__ void transpose4(uint32_t la, uint32_t lb, uint32_t lc, uint32_t ld,
+                                           uint32_t ha, uint32_t hb, uint32_t hc, uint32_t hd,
+                                           uint32_t w[kNd]) {
+  // v_perm_b32(s0, s1, sel): byte k of the result = byte sel_k of {s0:s1}
+  // (bytes 0-3 from s1, 4-7 from s0)
+  const uint32_t ab_lo = __builtin_amdgcn_perm(lb, la, 0x05010400u);  // a0 b0 a1 b1
+  const uint32_t ab_hi = __builtin_amdgcn_perm(lb, la, 0x07030602u);  // a2 b2 a3 b3
+  const uint32_t cd_lo = __builtin_amdgcn_perm(ld, lc, 0x05010400u);
+  const uint32_t cd_hi = __builtin_amdgcn_perm(ld, lc, 0x07030602u);
+  w[0] = __builtin_amdgcn_perm(cd_lo, ab_lo, 0x05040100u) ^ 0x80808080u;  // a0 b0 c0 d0
+  w[1] = __builtin_amdgcn_perm(cd_lo, ab_lo, 0x07060302u) ^ 0x80808080u;  // a1 b1 c1 d1
+  w[2] = __builtin_amdgcn_perm(cd_hi, ab_hi, 0x05040100u) ^ 0x80808080u;
+  w[3] = __builtin_amdgcn_perm(cd_hi, ab_hi, 0x07060302u) ^ 0x80808080u;
+  const uint32_t h_ab = __builtin_amdgcn_perm(hb, ha, 0x05010400u);  // a4 b4 a5 b5
+  const uint32_t h_cd = __builtin_amdgcn_perm(hd, hc, 0x05010400u);
+  w[4] = __builtin_amdgcn_perm(h_cd, h_ab, 0x05040100u) ^ 0x80808080u;
+  w[5] = __builtin_amdgcn_perm(h_cd, h_ab, 0x07060302u) ^ 0x80808080u;
+}
+
+// ---- B side: column scales, envelope, digits ------------------------------
+struct PrepArgs {
+  const double* const* W;   // device table of K column sources
+  int K, kpad, LW;
+  double* colscale;
+  double* env;
+  int envlen;
+  int8_t* bdig;             // MIX_I8
+  double* bf64;             // MIX_F64
+  int hblocks;
+  int* err;                 // covariance flag, cleared here
+};
+
+__device__ __forceinline__ double what_m(const double* W, int l3, int LW) {
+  return (l3 >= 0 && l3 < LW) ? (2.0 * l3 + 1.0) * W[l3] * kInv4PiM : 0.0;
+}
+
+// c_k = smallest power of two >= max_l3 |What_k(l3)| (1 for an all-zero column)
+__global__ void __launch_bounds__(256) mix_colscale_kernel(PrepArgs a) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  if (k == 0 && threadIdx.x == 0) *a.err = 0;
+  const double* W = a.W[k];
+  double m = 0.0;
+  for (int l = threadIdx.x; l < a.LW; l += 256) m = fmax(m, fabs(what_m(W, l, a.LW)));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int e = 0;
+    const double mm = red[0];
+    (void)frexp(mm, &e);
+    a.colscale[k] = mm > 0.0 ? ldexp(1.0, e) : 1.0;
+  }
+}
+
+// one thread per l3: env(l3) and the column digits / scaled values
+__global__ void __launch_bounds__(256) mix_bprep_kernel(PrepArgs a, int mode) {
+  const int l3 = blockIdx.x * 256 + threadIdx.x;
+  if (l3 >= a.envlen) return;
+  double r = 0.0;
+  for (int k = 0; k < a.K; ++k) r = fmax(r, fabs(what_m(a.W[k], l3, a.LW)) / a.colscale[k]);
+  int e = 0;
+  (void)frexp(r, &e);
+  const double env = r > 0.0 ? ldexp(1.0, e) : 0.0;
+  a.env[l3] = env;
+  const int p = l3 & 1, h = l3 >> 1;
+  for (int k = 0; k < a.kpad; ++k) {
+    double b = 0.0;
+    if (k < a.K && env > 0.0) b = what_m(a.W[k], l3, a.LW) / (a.colscale[k] * env);
+    if (mode == MIX_I8) {
+      // balanced base-256 digits of the integer nearest b 2^54 (|b| <= 1)
+      long long nb = llrint(ldexp(b, kFixB));
+      for (int t = 0; t < kNdB; ++t) {
+        const long long d = ((nb + 128) & 255) - 128;
+        nb = (nb - d) >> 8;
+        const size_t off = ((((size_t)t * 2 + p) * a.hblocks + (h >> 4)) * a.kpad + k) * 16 +
+                           (h & 15);
+        a.bdig[off] = (int8_t)d;
+      }
+    } else {
+      a.bf64[((size_t)p * (a.hblocks * 16) + h) * a.kpad + k] = b;
+    }
+  }
+}
+
+// ---- the contraction -------------------------------------------------------
+// One wave: anti-diagonal S, rows x0 .. x0 + 16 RG - 1 (RG row groups of 16),
+// all NCG column groups of 16.  Lane (j, g) = (lane % 16, lane / 16).
+template <int MODE, int FAM, int RG, int NCG, int EPI, int COVK>
+__global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
+  constexpr int kOdd = FAM == FAM_PO ? 1 : 0;
+  constexpr int kUw = 16 * (RG - 1) + 80;  // staged window of each lane factor
+  // per wave: lane-factor windows u1, u2 and the row-independent factor v
+  __shared__ double lds_u[4][2][kUw];
+  __shared__ double lds_v[4][64];
+  __shared__ double lds_x[4][16][17];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = ufl(threadIdx.x >> 6);
+  const int tile = ufl(blockIdx.x * 4 + wave);
+  if (tile >= A.ntiles) return;
+  const int j = lane & 15, g = lane >> 4;
+  const int S = __builtin_amdgcn_readfirstlane(A.tiles[tile].S);
+  const int x0 = __builtin_amdgcn_readfirstlane(A.tiles[tile].x0);
+  const int own_lo = __builtin_amdgcn_readfirstlane(A.tiles[tile].lo);
+  const int own_hi = __builtin_amdgcn_readfirstlane(A.tiles[tile].hi);
+  if (own_hi < own_lo) return;
+
+  // l3 = 2h + p = S - 2c - kOdd,  c = H - h
+  const int p = (S - kOdd) & 1;
+  const int H = (S - kOdd - p) >> 1;
+  const int hb0 = max(0, H - (x0 + 16 * RG - 1)) & ~15;
+  const int clo = max(0, (S - kOdd - (A.LW - 1) + 1) >> 1);
+  const int hend = H - clo;
+  const int nsteps = hend >= hb0 ? (hend - hb0 + 64) >> 6 : 0;
+
+  const double* T = kOdd ? A.Z : A.G;           // lane factors (index - kOdd)
+  const double* Tc = kOdd ? A.Z : A.G;          // gamma_c / z_c
+  const double* Ts = kOdd ? A.IG : A.D;         // delta_{S-c} / igamma_{S-c}
+  double* su1 = lds_u[wave][0];
+  double* su2 = lds_u[wave][1];
+  double* sv = lds_v[wave];
+
+  // spin-2 polynomial constants per (row group) of this lane's row j
+  double sig[RG], kap[RG];
+#pragma unroll
+  for (int rg = 0; rg < RG; ++rg) {
+    const double L1 = (double)(x0 + 16 * rg + j), L2 = (double)(S - (x0 + 16 * rg + j));
+    const double P1 = L1 * (L1 + 1.0), P2 = L2 * (L2 + 1.0);
+    sig[rg] = P1 + P2 - 1.0;
+    kap[rg] = 1.0 + 2.0 * P1 * P2;
+  }
+
+  // stage the factors of one K-step (h = hb .. hb + 63)
+  auto stage = [&](int hb) {
+    // u1[q] = T[x0 - H + hb + q - kOdd], u2[q] = T[S - x0 - 16 RG + 1 - H + hb + q - kOdd]
+    const int b1 = x0 - H + hb - kOdd + kTabOff;
+    const int b2 = S - x0 - 16 * RG + 1 - H + hb - kOdd + kTabOff;
+    for (int q = lane; q < kUw; q += 64) {
+      su1[q] = T[b1 + q];
+      su2[q] = T[b2 + q];
+    }
+    {
+      const int h = hb + lane;
+      const int c = H - h;
+      const int l3 = 2 * h + p;
+      sv[lane] = (Tc[kTabOff + c] * Ts[kTabOff + S - c]) * A.env[l3];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto unstage = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  // ---- pass 1: exact maximum of |A'| per row (MIX_I8) ----
+  double rscale[RG];
+  int rexp[RG];
+#pragma unroll
+  for (int rg = 0; rg < RG; ++rg) {
+    rscale[rg] = 1.0;
+    rexp[rg] = 0;
+  }
+  if constexpr (MODE == MIX_I8) {
+    double rmax[RG];
+#pragma unroll
+    for (int rg = 0; rg < RG; ++rg) rmax[rg] = 0.0;
+    for (int st = 0; st < nsteps; ++st) {
+      const int hb = hb0 + 64 * st;
+      stage(hb);
+#pragma unroll
+      for (int rg = 0; rg < RG; ++rg) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int q = 16 * g + i;
+          const double a = fam_value<FAM>(su1[16 * rg + j + q], su2[16 * (RG - 1 - rg) + 15 - j + q],
+                                          sv[q], sig[rg], kap[rg], 2 * (hb + q) + p);
+          rmax[rg] = fmax(rmax[rg], fabs(a));
+        }
+      }
+      unstage();
+    }
+#pragma unroll
+    for (int rg = 0; rg < RG; ++rg) {
+      double m = rmax[rg];
+      m = fmax(m, __shfl_xor(m, 16));
+      m = fmax(m, __shfl_xor(m, 32));
+      int e = 0;
+      (void)frexp(m, &e);
+      rexp[rg] = m > 0.0 ? e : 0;
+      rscale[rg] = ldexp(1.0, kFix - rexp[rg]);
+    }
+  }
+
+  // ---- pass 2: the products ----
+  // MIX_I8: acc[rg][cg][d] = sum over digit pairs with s + t = d + kDmin
+  // MIX_F64: acc[rg][cg][0] in FP64
+  v4i iacc[MODE == MIX_I8 ? RG : 1][MODE == MIX_I8 ? NCG : 1][kNdiag];
+  v4d dacc[MODE == MIX_F64 ? RG : 1][MODE == MIX_F64 ? NCG : 1];
+  if constexpr (MODE == MIX_I8) {
+#pragma unroll
+    for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+      for (int cg = 0; cg < NCG; ++cg)
+#pragma unroll
+        for (int d = 0; d < kNdiag; ++d) iacc[rg][cg][d] = v4i{0, 0, 0, 0};
+  } else {
+#pragma unroll
+    for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+      for (int cg = 0; cg < NCG; ++cg) dacc[rg][cg] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
+
+  for (int st = 0; st < nsteps; ++st) {
+    const int hb = hb0 + 64 * st;
+    stage(hb);
+    if constexpr (MODE == MIX_I8) {
+      // B digits of this K-step: lane (j, g) holds h = hb + 16 g .. + 15 of column 16 cg + j
+      v4i bd[NCG][kNdB];
+#pragma unroll
+      for (int cg = 0; cg < NCG; ++cg)
+#pragma unroll
+        for (int t = 0; t < kNdB; ++t)
+          bd[cg][t] = A.bdig[(((size_t)t * 2 + p) * A.hblocks + (hb >> 4) + g) * A.kpad +
+                             16 * cg + j];
+#pragma unroll
+      for (int rg = 0; rg < RG; ++rg) {
+        uint32_t w[4][kNd];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          uint32_t lo[4], hi[4];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int q = 16 * g + 4 * qq + b;
+            const double a =
+                fam_value<FAM>(su1[16 * rg + j + q], su2[16 * (RG - 1 - rg) + 15 - j + q], sv[q],
+                               sig[rg], kap[rg], 2 * (hb + q) + p);
+            split_words(a * rscale[rg], lo[b], hi[b]);
+          }
+          transpose4(lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3], w[qq]);
+        }
+        v4i ad[kNd];
+#pragma unroll
+        for (int t = 0; t < kNd; ++t)
+          ad[t] = v4i{(int)w[0][t], (int)w[1][t], (int)w[2][t], (int)w[3][t]};
+#pragma unroll
+        for (int cg = 0; cg < NCG; ++cg) {
+#pragma unroll
+          for (int s = 0; s < kNd; ++s)
+#pragma unroll
+            for (int t = 0; t < kNdB; ++t) {
+              if (s + t < kDmin) continue;
+              iacc[rg][cg][s + t - kDmin] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  ad[s], bd[cg][t], iacc[rg][cg][s + t - kDmin], 0, 0, 0);
+            }
+        }
+      }
+    } else {
+      // FP64: 16 MFMA steps of 4 c; lane (j, g) supplies h = hb + 4 m + g
+#pragma unroll 4
+      for (int m = 0; m < 16; ++m) {
+        const int q = 4 * m + g;
+        double bv[NCG];
+#pragma unroll
+        for (int cg = 0; cg < NCG; ++cg)
+          bv[cg] = A.bf64[((size_t)p * (A.hblocks * 16) + hb + q) * A.kpad + 16 * cg + j];
+#pragma unroll
+        for (int rg = 0; rg < RG; ++rg) {
+          const double a =
+              fam_value<FAM>(su1[16 * rg + j + q], su2[16 * (RG - 1 - rg) + 15 - j + q], sv[q],
+                             sig[rg], kap[rg], 2 * (hb + q) + p);
+#pragma unroll
+          for (int cg = 0; cg < NCG; ++cg)
+            dacc[rg][cg] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[cg], dacc[rg][cg], 0, 0, 0);
+        }
+      }
+    }
+    unstage();
+  }
+
+  // ---- epilogue: lane (j, g) holds four rows of column 16 cg + j: rows
+  // 4 g + i in register i for the int8 MFMA, rows 4 i + g for the FP64 one ----
+  const int lmin = A.lmin, lmax = A.lmax;
+  bool bad = false;
+  double* sx = &lds_x[wave][0][0];
+#pragma unroll
+  for (int rg = 0; rg < RG; ++rg) {
+    int er[4], ri[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ri[i] = MODE == MIX_I8 ? 4 * g + i : 4 * i + g;
+      er[i] = MODE == MIX_I8 ? __shfl(rexp[rg], ri[i]) : 0;
+    }
+#pragma unroll
+    for (int cg = 0; cg < NCG; ++cg) {
+      const int k = 16 * cg + j;
+      const double ck = k < A.ncol ? A.colscale[k] : 0.0;
+      double xv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double x;
+        if constexpr (MODE == MIX_I8) {
+          const v4i* c = iacc[rg][cg];
+          x = (double)c[kNdiag - 1][i];
+#pragma unroll
+          for (int d = kNdiag - 2; d >= 0; --d) x = fma(x, 256.0, (double)c[d][i]);
+          // sum A' B' = X 256^kDmin 2^(e - kFix - kFixB);  Xi = that times c_k
+          x = ldexp(x, 8 * kDmin + er[i] - kFix - kFixB) * ck;
+        } else {
+          x = dacc[rg][cg][i] * ck;
+        }
+        xv[i] = x;
+      }
+      if constexpr (EPI == EPI_MCM) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int l1 = x0 + 16 * rg + ri[i];
+          const int l2 = S - l1;
+          if (k >= A.ncol || l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax)
+            continue;
+          const double L1 = (double)l1, L2 = (double)l2;
+          double kk = 1.0;
+          if constexpr (FAM != FAM_00) {
+            const double pi1 = (L1 - 1.0) * L1 * (L1 + 1.0) * (L1 + 2.0);
+            const double pi2 = (L2 - 1.0) * L2 * (L2 + 1.0) * (L2 + 2.0);
+            const bool spin2 = (l1 >= 2) && (l2 >= 2);
+            if constexpr (FAM == FAM_02) kk = spin2 ? 1.0 / (2.0 * sqrt(pi1 * pi2)) : 0.0;
+            else kk = spin2 ? 1.0 / (4.0 * pi1 * pi2) : 0.0;
+          }
+          double* o = A.out[k];
+          o[(int64_t)(l1 - lmin) + (int64_t)(l2 - lmin) * A.ldo] = (2.0 * L2 + 1.0) * kk * xv[i];
+          o[(int64_t)(l2 - lmin) + (int64_t)(l1 - lmin) * A.ldo] = (2.0 * L1 + 1.0) * kk * xv[i];
+        }
+      } else {
+        // covariance: rows x columns of this column group through LDS, then
+        // one (row, block) per lane
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sx[ri[i] * 17 + j] = xv[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        constexpr int kNW = COVK == PSX_COV_TTTT || COVK == PSX_COV_EEEE ? 8
+                            : COVK == PSX_COV_TTEE                       ? 2
+                                                                         : 4;
+        constexpr int kBpc = 16 / kNW;  // blocks per column group
+        for (int pr = lane; pr < 16 * kBpc; pr += 64) {
+          const int r = pr & 15, bb = pr >> 4;
+          const int blk = (16 * cg) / kNW + bb;
+          const int l1 = x0 + 16 * rg + r;
+          const int l2 = S - l1;
+          if (blk * kNW >= A.ncol || l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 ||
+              l2 > lmax)
+            continue;
+          const double L1 = (double)l1, L2 = (double)l2;
+          double kk = 1.0;
+          if constexpr (FAM != FAM_00) {
+            const double pi1 = (L1 - 1.0) * L1 * (L1 + 1.0) * (L1 + 2.0);
+            const double pi2 = (L2 - 1.0) * L2 * (L2 + 1.0) * (L2 + 2.0);
+            const bool spin2 = (l1 >= 2) && (l2 >= 2);
+            if constexpr (FAM == FAM_02) kk = spin2 ? 1.0 / (2.0 * sqrt(pi1 * pi2)) : 0.0;
+            else kk = spin2 ? 1.0 / (4.0 * pi1 * pi2) : 0.0;
+          }
+          double xi[kNW];
+#pragma unroll
+          for (int w = 0; w < kNW; ++w) xi[w] = kk * sx[r * 17 + bb * kNW + w];
+          const double v = cov_combine<COVK>(l1, l2, xi, A.spec + 4 * blk, A.ratio + 4 * blk, bad);
+          double* o = A.out[blk];
+          o[(int64_t)(l1 - lmin) + (int64_t)(l2 - lmin) * A.ldo] = v;
+          o[(int64_t)(l2 - lmin) + (int64_t)(l1 - lmin) * A.ldo] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  }
+  if constexpr (EPI == EPI_COV) {
+    if (bad) atomicOr(A.err, 1);
+  }
+}
+
+// ---- tiles -------------------------------------------------------------------
+// For each anti-diagonal S the owned rows l1 in [max(lmin, S - lmax), min(S/2, lmax)]
+// are cut into chunks of `rows` from the diagonal down; a workgroup takes the
+// same chunk of 4 consecutive anti-diagonals; workgroups longest first, laid
+// out in runs per XCD as the VALU kernels' tiles.
+std::vector<MTile> make_mix_tiles(int lmin, int lmax, int LW, int odd, int rows) {
+  struct TC {
+    MTile t;
+    int cost;
+  };
+  std::vector<std::vector<TC>> chunks;
+  for (int S = 2 * lmin; S <= 2 * lmax; ++S) {
+    const int hi = std::min(S / 2, lmax);
+    const int lo = std::max(lmin, S - lmax);
+    int k = 0;
+    for (int top = hi; top >= lo; top -= rows, ++k) {
+      MTile t;
+      t.S = S;
+      t.x0 = std::max(top - rows + 1, 0);
+      t.lo = std::max(lo, top - rows + 1);
+      t.hi = top;
+      const int p = (S - odd) & 1, H = (S - odd - p) >> 1;
+      const int hb0 = std::max(0, H - (t.x0 + rows - 1)) & ~15;
+      const int clo = std::max(0, (S - odd - (LW - 1) + 1) >> 1);
+      const int hend = H - clo;
+      const int cost = hend >= hb0 ? (hend - hb0 + 64) >> 6 : 0;
+      if ((int)chunks.size() <= k) chunks.resize(k + 1);
+      chunks[k].push_back({t, cost});
+    }
+  }
+  struct WG {
+    int first, n, cost;
+  };
+  std::vector<TC> flat;
+  std::vector<WG> wgs;
+  for (auto& ck : chunks)
+    for (size_t i = 0; i < ck.size(); i += 4) {
+      WG g{(int)flat.size(), 0, 0};
+      for (size_t q = i; q < std::min(ck.size(), i + 4); ++q) {
+        flat.push_back(ck[q]);
+        g.n++;
+        g.cost = std::max(g.cost, ck[q].cost);
+      }
+      wgs.push_back(g);
+    }
+  int maxc = 1;
+  for (const WG& g : wgs) maxc = std::max(maxc, g.cost);
+  const int bucket = std::max(1, maxc / 32);
+  std::stable_sort(wgs.begin(), wgs.end(),
+                   [bucket](const WG& a, const WG& b) { return a.cost / bucket > b.cost / bucket; });
+  constexpr int kXcds = 8, kRun = 8;
+  const int nwg = (int)wgs.size();
+  std::vector<MTile> out;
+  out.reserve((size_t)nwg * 4);
+  for (int b = 0; (int)out.size() < nwg * 4 && b <= 4 * nwg + 64 * kXcds * kRun; ++b) {
+    const int x = b % kXcds, m = b / kXcds;
+    const long w = (long)(m / kRun) * kXcds * kRun + (long)x * kRun + m % kRun;
+    if (w >= nwg) continue;
+    const WG& g = wgs[w];
+    for (int q = 0; q < 4; ++q)
+      out.push_back(q < g.n ? flat[g.first + q].t : MTile{2 * lmin, 0, 0, -1});
+  }
+  return out;
+}
+
+// Host check of every table index a tile set can touch (the kernel's access
+// list: staged lane factors T[l - c - odd] over the window, T[c], Ts[S - c],
+// env[l3], B rows h) against the buffer sizes.
+void check_mix_geometry(const std::vector<MTile>& tiles, int lmax, int LW, int odd, int RG,
+                        int ntab, int hblocks, int envlen) {
+  for (const MTile& t : tiles) {
+    if (t.hi < t.lo) continue;
+    PSX_REQUIRE(t.x0 >= 0 && t.lo >= t.x0 && t.hi <= t.x0 + 16 * RG - 1 && t.hi <= lmax,
+                PSX_ERR_ARGUMENT, "mixed tile geometry: bad row range");
+    const int S = t.S, p = (S - odd) & 1, H = (S - odd - p) >> 1;
+    const int hb0 = std::max(0, H - (t.x0 + 16 * RG - 1)) & ~15;
+    const int clo = std::max(0, (S - odd - (LW - 1) + 1) >> 1);
+    const int hend = H - clo;
+    if (hend < hb0) continue;
+    const int nsteps = (hend - hb0 + 64) >> 6;
+    const int hlast = hb0 + 64 * nsteps - 1;
+    const int kUw = 16 * (RG - 1) + 80;
+    const long u1lo = (long)t.x0 - H + hb0 - odd, u1hi = (long)t.x0 - H + hlast - 63 - odd + kUw - 1;
+    const long b2 = (long)S - t.x0 - 16 * RG + 1 - H - odd;
+    const long u2lo = b2 + hb0, u2hi = b2 + hlast - 63 + kUw - 1;
+    const long clo_i = (long)H - hlast, chi_i = (long)H - hb0;
+    const long slo = (long)S - chi_i, shi = (long)S - clo_i;
+    const long lo = std::min({u1lo, u2lo, clo_i, slo}), hi = std::max({u1hi, u2hi, chi_i, shi});
+    PSX_REQUIRE(lo >= -kTabOff && hi + kTabOff < ntab, PSX_ERR_ARGUMENT,
+                "mixed tile geometry: table index out of range");
+    PSX_REQUIRE(2L * hlast + p < envlen && (hlast >> 4) < hblocks && hb0 >= 0, PSX_ERR_ARGUMENT,
+                "mixed tile geometry: column table out of range");
+  }
+}
+
+struct MixTileKey {
+  int lmin, lmax, LW, odd, rows;
+  bool operator<(const MixTileKey& o) const {
+    return std::tie(lmin, lmax, LW, odd, rows) < std::tie(o.lmin, o.lmax, o.LW, o.odd, o.rows);
+  }
+};
+struct MixTiles {
+  MTile* d = nullptr;
+  int n = 0;
+};
+std::map<std::pair<void*, MixTileKey>, MixTiles> g_mix_tiles;
+std::mutex g_mix_mu;
+
+using MixFn = void (*)(MixArgs);
+
+template <int MODE, int FAM, int RG, int EPI, int COVK>
+MixFn mix_fn(int ncg) {
+  switch (ncg) {
+    case 1: return mix_kernel<MODE, FAM, RG, 1, EPI, COVK>;
+    case 2: return mix_kernel<MODE, FAM, RG, 2, EPI, COVK>;
+    case 3: return mix_kernel<MODE, FAM, RG, 3, EPI, COVK>;
+    default: return mix_kernel<MODE, FAM, RG, 4, EPI, COVK>;
+  }
+}
+
+#ifndef PSX_MIX_RG
+#define PSX_MIX_RG 2
+#endif
+
+template <int MODE>
+MixFn pick_mix(int fam, int epi, int covk, int ncg) {
+  constexpr int RG = PSX_MIX_RG;
+  if (epi == EPI_MCM) {
+    switch (fam) {
+      case FAM_00: return mix_fn<MODE, FAM_00, RG, EPI_MCM, 0>(ncg);
+      case FAM_02: return mix_fn<MODE, FAM_02, RG, EPI_MCM, 0>(ncg);
+      case FAM_PE: return mix_fn<MODE, FAM_PE, RG, EPI_MCM, 0>(ncg);
+      case FAM_PO: return mix_fn<MODE, FAM_PO, RG, EPI_MCM, 0>(ncg);
+    }
+  } else {
+    switch (covk) {
+      case PSX_COV_TTTT: return mix_fn<MODE, FAM_00, RG, EPI_COV, PSX_COV_TTTT>(ncg);
+      case PSX_COV_EEEE: return mix_fn<MODE, FAM_PE, RG, EPI_COV, PSX_COV_EEEE>(ncg);
+      case PSX_COV_TTTE: return mix_fn<MODE, FAM_00, RG, EPI_COV, PSX_COV_TTTE>(ncg);
+      case PSX_COV_TEEE: return mix_fn<MODE, FAM_02, RG, EPI_COV, PSX_COV_TEEE>(ncg);
+      case PSX_COV_TEEE_PLANCK:
+        return mix_fn<MODE, FAM_PE, RG, EPI_COV, PSX_COV_TEEE_PLANCK>(ncg);
+      case PSX_COV_TTEE: return mix_fn<MODE, FAM_00, RG, EPI_COV, PSX_COV_TTEE>(ncg);
+    }
+  }
+  throw Error{PSX_ERR_ARGUMENT, "the matrix-core contraction does not take this kind"};
+}
+
+size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+void release_mix_tiles(Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_mix_mu);
+  for (auto it = g_mix_tiles.begin(); it != g_mix_tiles.end();) {
+    if (it->first.first == (void*)c)
+      it = g_mix_tiles.erase(it);
+    else
+      ++it;
+  }
+}
+
+int mix_family_of_cov(int kind) {
+  switch (kind) {
+    case PSX_COV_TTTT: case PSX_COV_TTTE: case PSX_COV_TTEE: return FAM_00;
+    case PSX_COV_EEEE: case PSX_COV_TEEE_PLANCK: return FAM_PE;
+    case PSX_COV_TEEE: return FAM_02;
+  }
+  throw Error{PSX_ERR_ARGUMENT,
+              "the matrix-core contraction takes TTTT, EEEE, TTTE, TEEE, TEEE-Planck and TTEE "
+              "(TETE mixes two families)"};
+}
+
+// One pass of the matrix-core contraction on the context stream.
+void run_mixed(Ctx* c, const MixRequest& rq) {
+  const int K = rq.ncol;
+  PSX_REQUIRE(K >= 1 && K <= kMaxCols, PSX_ERR_ARGUMENT, "between 1 and 64 W spectra per pass");
+  PSX_REQUIRE(rq.lmin >= 0 && rq.lmax >= rq.lmin && rq.lmax <= 8191, PSX_ERR_ARGUMENT,
+              "need 0 <= lmin <= lmax <= 8191");
+  PSX_REQUIRE(rq.LW >= 1 && rq.LW < (1 << 24), PSX_ERR_ARGUMENT, "bad W length");
+  PSX_REQUIRE(rq.mode == PSX_MIXED || rq.mode == PSX_FP64_MFMA, PSX_ERR_ARGUMENT,
+              "matrix-core precision must be PSX_MIXED or PSX_FP64_MFMA");
+  const int mode = rq.mode == PSX_MIXED ? MIX_I8 : MIX_F64;
+  const int LW = (int)std::min<int64_t>(rq.LW, 2 * (int64_t)rq.lmax + 1);
+  const int n = rq.lmax - rq.lmin + 1;
+  PSX_REQUIRE(rq.ldo >= n, PSX_ERR_AXES, "leading dimension smaller than lmax-lmin+1");
+  const int nout = rq.epi == EPI_MCM ? K : rq.nblk;
+  if (rq.epi == EPI_COV) {
+    PSX_REQUIRE(rq.nblk >= 1 && rq.nblk <= kMaxBlk && rq.nblk * rq.nW == K, PSX_ERR_ARGUMENT,
+                "covariance batch: nblk x nW columns, at most 16 blocks");
+    PSX_REQUIRE(mix_family_of_cov(rq.cov_kind) == rq.fam, PSX_ERR_ARGUMENT,
+                "covariance kind and family disagree");
+  }
+  const int kpad = (K + 15) & ~15, ncg = kpad / 16;
+  const int odd = rq.fam == FAM_PO ? 1 : 0;
+  constexpr int RG = PSX_MIX_RG;
+  const int hblocks = (rq.lmax + 64 + 15) / 16 + 2;
+  const int envlen = 32 * hblocks;
+  ensure_tables(c, LW + 2 * rq.lmax + 512);
+
+  const MixTiles* mt;
+  {
+    std::lock_guard<std::mutex> lk(g_mix_mu);
+    auto key = std::make_pair((void*)c, MixTileKey{rq.lmin, rq.lmax, LW, odd, 16 * RG});
+    auto it = g_mix_tiles.find(key);
+    if (it == g_mix_tiles.end()) {
+      std::vector<MTile> h = make_mix_tiles(rq.lmin, rq.lmax, LW, odd, 16 * RG);
+      check_mix_geometry(h, rq.lmax, LW, odd, RG, table_entries(LW + 2 * rq.lmax + 512),
+                         hblocks, envlen);
+      MixTiles m;
+      m.n = (int)h.size();
+      m.d = (MTile*)ctx_alloc(c, sizeof(MTile) * std::max<size_t>(1, h.size()));
+      PSX_HIP(hipMemcpy(m.d, h.data(), sizeof(MTile) * h.size(), hipMemcpyHostToDevice));
+      it = g_mix_tiles.emplace(key, m).first;
+    }
+    mt = &it->second;
+  }
+  PSX_REQUIRE(c->tables.n >= table_entries(LW + 2 * rq.lmax + 512), PSX_ERR_ARGUMENT,
+              "tables smaller than the mixed geometry needs");
+
+  // scratch: [colscale][env][B table][pointer tables][flag]
+  const size_t b_cs = al256(sizeof(double) * kpad), b_env = al256(sizeof(double) * envlen);
+  const size_t b_tab = mode == MIX_I8 ? al256((size_t)kNdB * 2 * hblocks * kpad * 16)
+                                      : al256(sizeof(double) * 2 * (size_t)hblocks * 16 * kpad);
+  const size_t b_ptr = al256(sizeof(void*) * (kMaxCols + 2 * 4 * kMaxBlk + kMaxCols));
+  const size_t need = b_cs + b_env + b_tab + b_ptr + 256;
+  char* base = (char*)ctx_scratch(c, need);
+  double* colscale = (double*)base;
+  double* env = (double*)(base + b_cs);
+  char* tab = base + b_cs + b_env;
+  void** ptrs = (void**)(base + b_cs + b_env + b_tab);
+  int* err = (int*)(base + b_cs + b_env + b_tab + b_ptr);
+
+  // pointer tables: W sources, outputs, spectra, ratios (one copy)
+  std::vector<const void*> hp(kMaxCols + 2 * 4 * kMaxBlk + kMaxCols, nullptr);
+  for (int k = 0; k < K; ++k) hp[k] = rq.dW[k];
+  for (int k = 0; k < nout; ++k) hp[kMaxCols + k] = rq.dout[k];
+  if (rq.epi == EPI_COV)
+    for (int b = 0; b < rq.nblk; ++b)
+      for (int i = 0; i < 4; ++i) {
+        hp[2 * kMaxCols + 4 * b + i] = rq.dspec[b][i];
+        hp[2 * kMaxCols + 4 * kMaxBlk + 4 * b + i] = rq.dratio[b][i];
+      }
+  PSX_HIP(hipMemcpyAsync(ptrs, hp.data(), sizeof(void*) * hp.size(), hipMemcpyHostToDevice,
+                         c->stream));
+
+  PrepArgs pa{};
+  pa.W = (const double* const*)ptrs;
+  pa.K = K;
+  pa.kpad = kpad;
+  pa.LW = LW;
+  pa.colscale = colscale;
+  pa.env = env;
+  pa.envlen = envlen;
+  pa.bdig = mode == MIX_I8 ? (int8_t*)tab : nullptr;
+  pa.bf64 = mode == MIX_F64 ? (double*)tab : nullptr;
+  pa.hblocks = hblocks;
+  pa.err = err;
+  {
+    TimedLaunch tl(c, "psx_mix_prep");
+    hipLaunchKernelGGL(mix_colscale_kernel, dim3(K), dim3(256), 0, c->stream, pa);
+    hipLaunchKernelGGL(mix_bprep_kernel, dim3((envlen + 255) / 256), dim3(256), 0, c->stream, pa,
+                       mode);
+  }
+
+  MixArgs ma{};
+  ma.tiles = mt->d;
+  ma.ntiles = mt->n;
+  ma.lmin = rq.lmin;
+  ma.lmax = rq.lmax;
+  ma.LW = LW;
+  ma.G = c->tables.dG;
+  ma.Z = c->tables.dZ;
+  ma.D = c->tables.dD;
+  ma.IG = c->tables.dIG;
+  ma.env = env;
+  ma.bdig = mode == MIX_I8 ? (const v4i*)tab : nullptr;
+  ma.bf64 = mode == MIX_F64 ? (const double*)tab : nullptr;
+  ma.hblocks = hblocks;
+  ma.kpad = kpad;
+  ma.ncol = K;
+  ma.colscale = colscale;
+  ma.out = (double* const*)(ptrs + kMaxCols);
+  ma.ldo = rq.ldo;
+  ma.nW = rq.nW;
+  ma.spec = (const double* const*)(ptrs + 2 * kMaxCols);
+  ma.ratio = (const double* const*)(ptrs + 2 * kMaxCols + 4 * kMaxBlk);
+  ma.err = err;
+  const MixFn fn = mode == MIX_I8 ? pick_mix<MIX_I8>(rq.fam, rq.epi, rq.cov_kind, ncg)
+                                  : pick_mix<MIX_F64>(rq.fam, rq.epi, rq.cov_kind, ncg);
+  {
+    TimedLaunch tl(c, mode == MIX_I8 ? "psx_mix_i8" : "psx_mix_f64");
+    hipLaunchKernelGGL(fn, dim3((unsigned)((mt->n + 3) / 4)), dim3(256), 0, c->stream, ma);
+  }
+  if (rq.epi == EPI_COV) {
+    int h = 0;
+    PSX_HIP(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    PSX_REQUIRE(h == 0, PSX_ERR_DOMAIN, "sqrt of a negative spectrum product (Julia DomainError)");
+  }
+}
+
+}  // namespace psx

@@ -227,6 +227,28 @@ struct ContractRequest {
 };
 
 void run_contraction(Ctx* c, const ContractRequest& rq);
+
+// A matrix-core pass (mixed.hip): ncol W spectra against one family, into
+// ncol mode-coupling blocks (EPI_MCM) or nblk covariance blocks of nW
+// spectra each (EPI_COV), in precision PSX_MIXED or PSX_FP64_MFMA.
+struct MixRequest {
+  int lmin = 0, lmax = 0;
+  int64_t LW = 0;
+  int fam = FAM_00;
+  int ncol = 0;
+  const double* dW[64] = {};
+  EpiMode epi = EPI_MCM;
+  double* dout[64] = {};
+  int64_t ldo = 0;
+  int cov_kind = 0, nblk = 0, nW = 0;
+  const double* dspec[16][4] = {};
+  const double* dratio[16][4] = {};
+  int mode = PSX_MIXED;
+};
+void run_mixed(Ctx* c, const MixRequest& rq);
+// the one family a covariance kind contracts (ArgumentError for TETE)
+int mix_family_of_cov(int kind);
+void release_mix_tiles(Ctx* c);
 // The same through the device's static scratch array only: no device
 // allocation (psx_mcm_d_noalloc).
 void run_contraction_noalloc(Ctx* c, const ContractRequest& rq);
