@@ -26,6 +26,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -99,35 +100,84 @@ def read_traffic(name: str):
     return vals.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(lmax: int, LW: int, Ws, stride: int, nthreads: int):
-    """Time the CPU oracle (reference-algorithm restatement) on rows
+def physical_cores() -> int:
+    """Physical cores of the host (distinct (core, socket) pairs of lscpu)."""
+    try:
+        txt = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        pairs = {ln for ln in txt.splitlines() if ln and not ln.startswith("#")}
+        if pairs:
+            return len(pairs)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1
+
+
+def cpu_baseline(lmax: int, LW: int, Ws, stride: int, nthreads: int, runs: int = 3,
+                 stride_one: int = 128):
+    """Time the CPU oracle (reference-algorithm restatement) on the rows
     l1 = 0, stride, 2 stride, ... of the five blocks; the systematic row
-    sample scales the time of the full job by `stride`."""
+    sample scales the time of the full job by `stride`.  One oracle call per
+    (block, row), spread over `nthreads` host threads (the calls release the
+    GIL and write disjoint rows); the median of `runs` timed passes after a
+    warm-up pass.  The same on one thread over a sparser sample
+    (`stride_one`)."""
+    import concurrent.futures
     from oracle import oracle as O
 
     W_TT, W_TE, W_ET, W_PP = Ws
-    jobs = [(O.MCM_00, W_TT), (O.MCM_02, W_TE), (O.MCM_02, W_ET), (O.MCM_PP, W_PP),
-            (O.MCM_MM, W_PP)]
+    blocks = [(O.MCM_00, W_TT), (O.MCM_02, W_TE), (O.MCM_02, W_ET), (O.MCM_PP, W_PP),
+              (O.MCM_MM, W_PP)]
     n = lmax + 1
     M = np.full((n, n), np.nan, order="F")
-    rows = list(range(0, lmax + 1, stride))
-    # warm the library / thread pool
-    O.mcm_rows(0, 0, lmax, lmax, lmax, W_TT, nthreads=nthreads, out=M)
-    t0 = time.perf_counter()
-    for kind, W in jobs:
-        for r in rows:
-            O.mcm_rows(kind, 0, lmax, r, r, W, nthreads=nthreads, out=M)
-    dt = time.perf_counter() - t0
-    full_time = dt * stride
     elements = 5 * n * n
+
+    def one(job):
+        kind, W, r = job
+        O.mcm_rows(kind, 0, lmax, r, r, W, nthreads=1, out=M)
+
+    def timed(jobs, pool):
+        t0 = time.perf_counter()
+        if pool is None:
+            for jb in jobs:
+                one(jb)
+        else:
+            list(pool.map(one, jobs))
+        return time.perf_counter() - t0
+
+    def measure(st, pool):
+        rows = list(range(0, lmax + 1, st))
+        # largest rows first so the pool's tail is short
+        jobs = sorted([(k, W, r) for k, W in blocks for r in rows], key=lambda jb: -min(jb[2], lmax - jb[2]))
+        timed(jobs[: max(1, len(jobs) // 8)], pool)           # warm-up
+        ts = sorted(timed(jobs, pool) for _ in range(runs))
+        med = ts[len(ts) // 2]
+        return rows, ts, elements / (med * st)
+
+    with concurrent.futures.ThreadPoolExecutor(nthreads) as pool:
+        rows, ts, value = measure(stride, pool)
+    rows1, ts1, value1 = measure(stride_one, None)
+    phys = physical_cores()
     return {
-        "value": elements / full_time,
+        "value": value,
         "unit": "elements/s",
         "cores": nthreads,
         "kind": "port",
         "sample": (f"rows l1 = 0 mod {stride} ({len(rows)} of {n}) of all 5 blocks (TT, TE, ET, ++, --), "
-                   f"{dt:.1f} s measured, full-job time = {stride} x sample"),
-        "sample_seconds": round(dt, 3),
+                   f"one oracle call per (block, row) on {nthreads} threads; median of {runs} passes "
+                   f"after a warm-up ({', '.join(f'{t:.2f}' for t in ts)} s); full-job time = "
+                   f"{stride} x sample"),
+        "physical_cores": phys,
+        "threads_note": (f"{nthreads} threads = min(physical cores {phys}, CPUs available "
+                         f"{len(os.sched_getaffinity(0))}, 16 per GPU)"),
+        "one_core": {
+            "value": value1,
+            "unit": "elements/s",
+            "sample": (f"rows l1 = 0 mod {stride_one} ({len(rows1)} of {n}) of all 5 blocks on one "
+                       f"thread; median of {runs} passes after a warm-up "
+                       f"({', '.join(f'{t:.2f}' for t in ts1)} s); full-job time = "
+                       f"{stride_one} x sample"),
+        },
     }
 
 
@@ -172,6 +222,191 @@ def _kernel_roofline(name: str, ms: float, ev: int, od: int) -> dict:
     tf = FLOPS_PER_STEP[name] * steps / (ms * 1e-3) / 1e12
     return {"ms": ms, "tflops": tf, "frac": tf / FP64_PEAK_TFLOPS, "steps": steps,
             "flops_per_step": FLOPS_PER_STEP[name]}
+
+
+def rel_err(A, R, floor=1e-6) -> float:
+    """Max elementwise |A-R|/|R| over entries with |R| >= floor max|R|."""
+    scale = np.abs(R).max()
+    if scale == 0.0:
+        return float(np.abs(A).max())
+    sel = np.abs(R) >= floor * scale
+    return float((np.abs(A[sel] - R[sel]) / np.abs(R[sel])).max())
+
+
+def norm_err(A, R) -> float:
+    r = np.linalg.norm(R)
+    return float(np.linalg.norm(A - R) / r) if r > 0 else float(np.linalg.norm(A))
+
+
+# kernels each precision of a batch launches (HIP-event timers of the library)
+BATCH_KERNELS = {0: ["psx_mcm_group", "psx_mcm_00", "psx_mcm_02", "psx_mcm_pp", "psx_mcm_mm"],
+                 1: ["psx_mix_prep", "psx_mix_i8"], 2: ["psx_mix_prep", "psx_mix_f64"]}
+INT8_PEAK_TOPS = 5000.0      # MI355X dense int8 MFMA
+MIX_PRODUCTS = 27            # int8 digit products per (step, column): s + t >= 5 of 6 x 7 digits
+
+
+def batch_measure(ctx, abi, kind, lmax, Ws, steps, warmup, check=None):
+    """K blocks of one mode-coupling kind at lmax from K spectra, device
+    resident, in the three precisions: elements/s, kernel time per step, and
+    for PSX_MIXED and PSX_FP64_MFMA the error of the blocks in `check` (all by
+    default) against the FP64 blocks."""
+    K, LW, n = len(Ws), Ws[0].size, lmax + 1
+    L = ctx.lib
+    VP = ctypes.c_void_p
+    dW = [ctx.alloc(8 * LW) for _ in range(K)]
+    for d, w in zip(dW, Ws):
+        ctx.h2d(d, np.ascontiguousarray(w))
+    dM = [ctx.alloc(8 * n * n) for _ in range(K)]
+    Wp = (VP * K)(*[VP(p) for p in dW])
+    Mp = (VP * K)(*[VP(p) for p in dM])
+    check = list(range(K)) if check is None else list(check)
+    ev, od = csteps(0, lmax, LW)
+    csteps_k = od if kind == 3 else ev
+    out, ref = {}, {}
+    for prec, name in ((abi.FP64, "fp64"), (abi.MIXED, "mixed"), (abi.FP64_MFMA, "fp64_mfma")):
+        f = lambda: ctx.check(L.psx_mcm_batch_d(ctx.handle, kind, 0, lmax, K, Wp, LW, prec, Mp,
+                                                n), "psx_mcm_batch_d")
+        names = BATCH_KERNELS[prec]
+        for _ in range(warmup):
+            f()
+        ctx.synchronize()
+        for nm in names:
+            ctx.kernel_time(nm, reset=True)
+        ctx.set_timing(True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f()
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        ctx.set_timing(False)
+        kms = {}
+        for nm in names:
+            ms, cnt = ctx.kernel_time(nm, reset=True)
+            if cnt:
+                kms[nm] = ms / steps
+        blocks = {}
+        for k in check:
+            M = np.empty((n, n), order="F")
+            ctx.d2h(M, dM[k])
+            blocks[k] = M
+        r = {"elements_per_s": K * n * n / dt, "ms_per_step": dt * 1e3,
+             "kernel_ms_per_step": kms}
+        if prec == abi.FP64:
+            ref = blocks
+            kt = sum(kms.values())
+            r["fp64_tflops"] = 2.0 * K * csteps_k / (kt * 1e-3) / 1e12 if kt else None
+        else:
+            errs = [rel_err(blocks[k], ref[k]) for k in check]
+            nerr = [norm_err(blocks[k], ref[k]) for k in check]
+            r["max_rel_err"] = max(errs)
+            r["max_normwise_err"] = max(nerr)
+            r["per_block_max_rel_err"] = errs
+            main = "psx_mix_i8" if prec == abi.MIXED else "psx_mix_f64"
+            if main in kms:
+                kt = kms[main]
+                r["contraction_fp64_equiv_tflops"] = 2.0 * K * csteps_k / (kt * 1e-3) / 1e12
+                if prec == abi.MIXED:
+                    r["int8_tops"] = 2.0 * MIX_PRODUCTS * K * csteps_k / (kt * 1e-3) / 1e12
+                    r["int8_frac"] = r["int8_tops"] / INT8_PEAK_TOPS
+        out[name] = r
+        del blocks
+    for d in dW + dM:
+        ctx.free(d)
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out["speedup_fp64_mfma_vs_fp64"] = out["fp64"]["ms_per_step"] / out["fp64_mfma"]["ms_per_step"]
+    out.update({"kind": kind, "lmax": lmax, "K": K, "LW": LW, "checked_blocks": len(check)})
+    return out
+
+
+def mask_W_pool(ctx, nmasks: int, lmax: int, count: int):
+    """`count` distinct mask cross-spectra (autos and i < j pairs) of `nmasks`
+    seeded nside-1024 cut-sky masks (200 holes, seeds 0..nmasks-1), truncated
+    at lmax."""
+    from powerspectra_amd.masks import config_masks, mask_spectra
+    W = mask_spectra(config_masks(1024, 200, tuple(range(nmasks))), lmax, ctx=ctx)
+    keys = [(i, j) for i in range(nmasks) for j in range(i, nmasks)]
+    assert len(keys) >= count
+    return [np.ascontiguousarray(W[k][: lmax + 1]) for k in keys[:count]]
+
+
+def run_mixed(args, pkg) -> dict:
+    """FP64 against the matrix-core contractions on batches of TT blocks at
+    lmax 3071 (K = 8, 16, 32, 64 spectra from twelve seeded nside-1024 masks),
+    and on the covariance blocks TTTT of a 2 x 2 field set at lmax 2508 (8
+    blocks, 64 W spectra)."""
+    abi = pkg._abi
+    ctx = pkg.Context(0)
+    lmax = args.lmax
+    pool = mask_W_pool(ctx, 12, lmax, 64)
+    sweep = []
+    for K in (8, 16, 32, 64):
+        sweep.append(batch_measure(ctx, abi, 0, lmax, pool[:K], args.steps, args.warmup,
+                                   check=range(0, K, max(1, K // 8))))
+        print(json.dumps({"K": K, "speedup": sweep[-1]["speedup_mixed_vs_fp64"],
+                          "err": sweep[-1]["mixed"]["max_rel_err"]}), file=sys.stderr, flush=True)
+    kinds = []
+    for kind in (1, 2, 3):
+        kinds.append(batch_measure(ctx, abi, kind, lmax, pool[:32], args.steps, args.warmup,
+                                   check=range(0, 32, 4)))
+    cov = cov_batch_measure(ctx, abi, pool, args.steps, args.warmup)
+    ctx.close()
+    head = sweep[1]
+    return _line("MCM elements/s at lmax %d, batches of K TT blocks: FP64 vs mixed" % lmax,
+                 head["mixed"]["elements_per_s"], "elements/s", args, head["mixed"]["ms_per_step"],
+                 {"workload": f"mcm_batch_lmax{lmax}: K TT blocks from K mask cross-spectra "
+                              "(twelve seeded nside-1024 masks, 200 holes), K = 8, 16, 32, 64; "
+                              "TE, ++, -- at K = 32; covariance TTTT 8 blocks at lmax 2508",
+                  "lmax": lmax},
+                 data="seeded masks: twelve nside-1024 cut-sky masks, distinct cross-spectra",
+                 sweep=sweep, kinds=kinds, cov=cov)
+
+
+def cov_batch_measure(ctx, abi, pool, steps, warmup, lmax=2508, nblk=8):
+    """nblk TTTT covariance blocks at lmax (8 W spectra each from the pool,
+    signal-like spectra and noise ratios), FP64 (one psx_cov launch per
+    block) against the matrix-core passes."""
+    L = ctx.lib
+    VP = ctypes.c_void_p
+    n = LW = lmax + 1
+    rng = np.random.default_rng(9)
+    ell = np.arange(LW, dtype=np.float64)
+    dW = [ctx.alloc(8 * LW) for _ in range(8 * nblk)]
+    for k, d in enumerate(dW):
+        ctx.h2d(d, np.ascontiguousarray(pool[k % len(pool)][:LW]))
+    ds = [ctx.alloc(8 * LW) for _ in range(4 * nblk)]
+    dr = [ctx.alloc(8 * LW) for _ in range(4 * nblk)]
+    for d in ds:
+        ctx.h2d(d, 1e3 * rng.uniform(0.5, 2) * (ell + 10.0) ** -2.5 * np.exp(-(ell / 2000.0) ** 2))
+    for d in dr:
+        ctx.h2d(d, np.sqrt(1.0 + rng.uniform(0, 2) * (ell / LW) ** 2))
+    dC = [ctx.alloc(8 * n * n) for _ in range(nblk)]
+    arr = lambda ptrs: (VP * len(ptrs))(*[VP(p) for p in ptrs])
+    S, R, W, Cp = arr(ds), arr(dr), arr(dW), arr(dC)
+    out, ref = {}, []
+    for prec, name in ((abi.FP64, "fp64"), (abi.MIXED, "mixed"), (abi.FP64_MFMA, "fp64_mfma")):
+        f = lambda: ctx.check(L.psx_cov_batch_d(ctx.handle, 0, 0, lmax, nblk, S, R, W, LW, prec,
+                                                Cp, n), "psx_cov_batch_d")
+        names = ["psx_cov_tttt"] if prec == abi.FP64 else BATCH_KERNELS[prec]
+        dt, _ = _timed(ctx, f, [], steps, warmup)
+        blocks = []
+        for d in dC:
+            C = np.empty((n, n), order="F")
+            ctx.d2h(C, d)
+            blocks.append(C)
+        r = {"elements_per_s": nblk * n * n * steps / dt, "ms_per_step": dt / steps * 1e3}
+        if prec == abi.FP64:
+            ref = blocks
+        else:
+            r["per_block_max_rel_err"] = [rel_err(b, a) for a, b in zip(ref, blocks)]
+            r["max_rel_err"] = max(r["per_block_max_rel_err"])
+            r["max_normwise_err"] = max(norm_err(b, a) for a, b in zip(ref, blocks))
+        out[name] = r
+    for d in dW + ds + dr + dC:
+        ctx.free(d)
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out.update({"kind": "TTTT", "lmax": lmax, "blocks": nblk, "W_spectra": 8 * nblk})
+    return out
 
 
 def run_mcm_blocks(args, pkg) -> dict:
@@ -737,7 +972,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
                     choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
-                             "sht_pol", "quickpol", "master", "binned"],
+                             "sht_pol", "quickpol", "master", "binned", "mixed"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -754,7 +989,7 @@ def main() -> int:
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
               "e2e": run_e2e, "serve": run_serve, "sht": run_sht, "sht_pol": run_sht_pol,
               "quickpol": run_quickpol, "master": run_master,
-              "binned": run_binned}[args.workload]
+              "binned": run_binned, "mixed": run_mixed}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]
@@ -857,7 +1092,7 @@ def main() -> int:
         "mixed": None,
     }
     if not args.no_cpu:
-        nthreads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        nthreads = args.cpu_threads or min(16, physical_cores(), len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(lmax, LW, Ws, args.cpu_stride, nthreads)
     else:
         out["cpu_baseline"] = None

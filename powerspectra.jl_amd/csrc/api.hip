@@ -134,6 +134,12 @@ ContractRequest cov_request(int kind, int lmin, int lmax, const double* const* d
   return rq;
 }
 
+// column groups of 16 spectra per matrix-core pass (register budget of the
+// int8 kernel: seven int32 accumulators per row group and column group)
+#ifndef PSX_MIX_NCG_MAX
+#define PSX_MIX_NCG_MAX 4
+#endif
+
 void check_precision(int precision) {
   PSX_REQUIRE(precision == PSX_FP64 || precision == PSX_MIXED || precision == PSX_FP64_MFMA,
               PSX_ERR_ARGUMENT, "precision must be PSX_FP64, PSX_MIXED or PSX_FP64_MFMA");
@@ -162,20 +168,25 @@ void mcm_batch_dev(Ctx* c, int kind, int lmin, int lmax, int K, const double* co
     }
     return;
   }
-  MixRequest mr;
-  mr.lmin = lmin;
-  mr.lmax = lmax;
-  mr.LW = LW;
-  mr.fam = fam;
-  mr.ncol = K;
-  for (int k = 0; k < K; ++k) {
-    mr.dW[k] = dW[k];
-    mr.dout[k] = dM[k];
+  // matrix-core passes of at most 16 PSX_MIX_NCG_MAX spectra each
+  const int per = 16 * PSX_MIX_NCG_MAX;
+  for (int k0 = 0; k0 < K; k0 += per) {
+    const int nk = std::min(per, K - k0);
+    MixRequest mr;
+    mr.lmin = lmin;
+    mr.lmax = lmax;
+    mr.LW = LW;
+    mr.fam = fam;
+    mr.ncol = nk;
+    for (int k = 0; k < nk; ++k) {
+      mr.dW[k] = dW[k0 + k];
+      mr.dout[k] = dM[k0 + k];
+    }
+    mr.epi = EPI_MCM;
+    mr.ldo = ldM;
+    mr.mode = precision;
+    run_mixed(c, mr);
   }
-  mr.epi = EPI_MCM;
-  mr.ldo = ldM;
-  mr.mode = precision;
-  run_mixed(c, mr);
 }
 
 // nblk covariance blocks of one kind on device buffers; the arrays hold the
@@ -198,7 +209,7 @@ void cov_batch_dev(Ctx* c, int kind, int lmin, int lmax, int nblk,
     return;
   }
   const int fam = mix_family_of_cov(kind);
-  const int per = std::min(16, 64 / nW);  // blocks per matrix-core pass
+  const int per = std::min(16, 16 * PSX_MIX_NCG_MAX / nW);  // blocks per matrix-core pass
   for (int b0 = 0; b0 < nblk; b0 += per) {
     const int nb = std::min(per, nblk - b0);
     MixRequest mr;

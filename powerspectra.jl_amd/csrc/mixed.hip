@@ -36,6 +36,16 @@
 // 1e-12 of its terms to B' quantised against the envelope; measured per
 // block against the FP64 kernels and the oracle by the tests).
 //
+// Schedule: one wave per 16-row tile of one anti-diagonal S; a workgroup holds
+// the same row chunk of S, S + 2, S + 4, S + 6.  Their B rows coincide (B
+// depends on l3 = 2h + p only), so the four waves load one B stream
+// cooperatively, straight into LDS (global_load_lds), one K-step (64 h)
+// ahead into the other of two buffers, with one workgroup barrier per step;
+// the lane-factor windows of the family product are loaded one step ahead
+// into registers and not touched before their step.  Two workgroups per CU
+// (register budget of two waves per SIMD) let one wave's VALU work (the family
+// product, its digit split) run beside the other's MFMAs.
+//
 // Precision PSX_FP64_MFMA runs the same tiles on v_mfma_f64_16x16x4_f64 with
 // A and B in FP64 (no splitting): exact FP64 products, a different
 // summation order than the VALU kernels.
@@ -70,6 +80,7 @@ enum MixMode : int { MIX_I8 = 0, MIX_F64 = 1 };
 
 struct MTile {
   int S, x0, lo, hi;  // anti-diagonal, first row of the row groups, owned rows
+  int hb0, nsteps;    // the workgroup's common K-steps: h = hb0 .. hb0 + 64 nsteps - 1
 };
 
 struct MixArgs {
@@ -224,32 +235,37 @@ __global__ void __launch_bounds__(256) mix_bprep_kernel(PrepArgs a, int mode) {
 // One wave: anti-diagonal S, rows x0 .. x0 + 16 RG - 1 (RG row groups of 16),
 // all NCG column groups of 16.  Lane (j, g) = (lane % 16, lane / 16).
 template <int MODE, int FAM, int RG, int NCG, int EPI, int COVK>
-__global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mix_kernel(MixArgs A) {
   constexpr int kOdd = FAM == FAM_PO ? 1 : 0;
   constexpr int kUw = 16 * (RG - 1) + 80;  // staged window of each lane factor
   // per wave: lane-factor windows u1, u2 and the row-independent factor v
   __shared__ double lds_u[4][2][kUw];
   __shared__ double lds_v[4][64];
   __shared__ double lds_x[4][16][17];
+  // B digits of one K-step, shared by the workgroup's four waves (their
+  // anti-diagonals have one parity, so the same h rows): [column group]
+  // [digit][lane], two buffers (the next step lands while this one is used)
+  constexpr int kBcg = MODE == MIX_I8 ? NCG : 1, kBt = MODE == MIX_I8 ? kNdB : 1;
+  __shared__ v4i lds_b0[kBcg][kBt][64];
+  __shared__ v4i lds_b1[kBcg][kBt][64];
 
   const int lane = threadIdx.x & 63;
   const int wave = ufl(threadIdx.x >> 6);
+  // tiles come in whole workgroups (ntiles is a multiple of 4): no wave
+  // leaves before the barriers of the K-loop
   const int tile = ufl(blockIdx.x * 4 + wave);
-  if (tile >= A.ntiles) return;
+  if (ufl(blockIdx.x * 4) >= A.ntiles) return;
   const int j = lane & 15, g = lane >> 4;
-  const int S = __builtin_amdgcn_readfirstlane(A.tiles[tile].S);
-  const int x0 = __builtin_amdgcn_readfirstlane(A.tiles[tile].x0);
-  const int own_lo = __builtin_amdgcn_readfirstlane(A.tiles[tile].lo);
-  const int own_hi = __builtin_amdgcn_readfirstlane(A.tiles[tile].hi);
-  if (own_hi < own_lo) return;
+  const int S = ufl(A.tiles[tile].S);
+  const int x0 = ufl(A.tiles[tile].x0);
+  const int own_lo = ufl(A.tiles[tile].lo);
+  const int own_hi = ufl(A.tiles[tile].hi);
+  const int hb0 = ufl(A.tiles[tile].hb0);
+  const int nsteps = ufl(A.tiles[tile].nsteps);
 
   // l3 = 2h + p = S - 2c - kOdd,  c = H - h
   const int p = (S - kOdd) & 1;
   const int H = (S - kOdd - p) >> 1;
-  const int hb0 = max(0, H - (x0 + 16 * RG - 1)) & ~15;
-  const int clo = max(0, (S - kOdd - (A.LW - 1) + 1) >> 1);
-  const int hend = H - clo;
-  const int nsteps = hend >= hb0 ? (hend - hb0 + 64) >> 6 : 0;
 
   const double* T = kOdd ? A.Z : A.G;           // lane factors (index - kOdd)
   const double* Tc = kOdd ? A.Z : A.G;          // gamma_c / z_c
@@ -268,21 +284,37 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
     kap[rg] = 1.0 + 2.0 * P1 * P2;
   }
 
-  // stage the factors of one K-step (h = hb .. hb + 63)
-  auto stage = [&](int hb) {
-    // u1[q] = T[x0 - H + hb + q - kOdd], u2[q] = T[S - x0 - 16 RG + 1 - H + hb + q - kOdd]
+  // The factors of one K-step (h = hb .. hb + 63): the lane-factor windows
+  // u1[q] = T[x0 - H + hb + q - kOdd], u2[q] = T[S - x0 - 16 RG + 1 - H + hb + q - kOdd]
+  // and the row-independent v[q] = Tc[c] Ts[S - c] env[l3] (h = hb + q).
+  // They are loaded one step ahead into registers, untouched until the step
+  // starts (so that no wait for them falls inside the current step), then
+  // multiplied and written to this wave's LDS window.
+  struct Fetch {
+    double u1a, u1b, u2a, u2b, tc, ts, ev;
+  };
+  auto fetch = [&](int hb) {
+    Fetch f;
     const int b1 = x0 - H + hb - kOdd + kTabOff;
     const int b2 = S - x0 - 16 * RG + 1 - H + hb - kOdd + kTabOff;
-    for (int q = lane; q < kUw; q += 64) {
-      su1[q] = T[b1 + q];
-      su2[q] = T[b2 + q];
+    f.u1a = T[b1 + lane];
+    f.u2a = T[b2 + lane];
+    f.u1b = lane + 64 < kUw ? T[b1 + lane + 64] : 0.0;
+    f.u2b = lane + 64 < kUw ? T[b2 + lane + 64] : 0.0;
+    const int h = hb + lane, c = H - h;
+    f.tc = Tc[kTabOff + c];
+    f.ts = Ts[kTabOff + S - c];
+    f.ev = A.env[2 * h + p];
+    return f;
+  };
+  auto commit = [&](const Fetch& f) {
+    su1[lane] = f.u1a;
+    su2[lane] = f.u2a;
+    if (lane + 64 < kUw) {
+      su1[lane + 64] = f.u1b;
+      su2[lane + 64] = f.u2b;
     }
-    {
-      const int h = hb + lane;
-      const int c = H - h;
-      const int l3 = 2 * h + p;
-      sv[lane] = (Tc[kTabOff + c] * Ts[kTabOff + S - c]) * A.env[l3];
-    }
+    sv[lane] = (f.tc * f.ts) * f.ev;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -291,6 +323,19 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // B digits of the K-step at hb into one LDS buffer, straight from global
+  // memory (lane l's 16 bytes land at buf[cg][t][l]: lane (j, g) carries
+  // h = hb + 16 g .. + 15 of column 16 cg + j); the workgroup's waves share
+  // the NCG kNdB loads of a step
+  auto issue_b = [&](int hb, v4i (*buf)[kBt][64]) {
+    for (int n = wave; n < NCG * kNdB; n += 4) {
+      const int cg = n / kNdB, t = n - cg * kNdB;
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(A.bdig +
+              (((size_t)t * 2 + p) * A.hblocks + (hb >> 4) + g) * A.kpad + 16 * cg + j),
+          (__attribute__((address_space(3))) void*)&buf[cg][t][0], 16, 0, 0);
+    }
   };
 
   // ---- pass 1: exact maximum of |A'| per row (MIX_I8) ----
@@ -301,13 +346,16 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
     rscale[rg] = 1.0;
     rexp[rg] = 0;
   }
+  Fetch nf{};
   if constexpr (MODE == MIX_I8) {
+    if (nsteps > 0) nf = fetch(hb0);
     double rmax[RG];
 #pragma unroll
     for (int rg = 0; rg < RG; ++rg) rmax[rg] = 0.0;
     for (int st = 0; st < nsteps; ++st) {
       const int hb = hb0 + 64 * st;
-      stage(hb);
+      commit(nf);
+      if (st + 1 < nsteps) nf = fetch(hb + 64);
 #pragma unroll
       for (int rg = 0; rg < RG; ++rg) {
 #pragma unroll
@@ -351,18 +399,21 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
       for (int cg = 0; cg < NCG; ++cg) dacc[rg][cg] = v4d{0.0, 0.0, 0.0, 0.0};
   }
 
-  for (int st = 0; st < nsteps; ++st) {
-    const int hb = hb0 + 64 * st;
-    stage(hb);
-    if constexpr (MODE == MIX_I8) {
-      // B digits of this K-step: lane (j, g) holds h = hb + 16 g .. + 15 of column 16 cg + j
-      v4i bd[NCG][kNdB];
-#pragma unroll
-      for (int cg = 0; cg < NCG; ++cg)
-#pragma unroll
-        for (int t = 0; t < kNdB; ++t)
-          bd[cg][t] = A.bdig[(((size_t)t * 2 + p) * A.hblocks + (hb >> 4) + g) * A.kpad +
-                             16 * cg + j];
+  if constexpr (MODE == MIX_I8) {
+    // One K-step: every load of step st was issued a whole step earlier.
+    auto step = [&](int st, v4i (*cur)[kBt][64], v4i (*nxt)[kBt][64]) {
+      const int hb = hb0 + 64 * st;
+      // this wave's loads for step st (its share of cur, its factors) have
+      // landed; after the barrier every wave's share of cur has, and no wave
+      // still reads nxt (its step st - 1 buffer)
+      __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+      __syncthreads();
+      commit(nf);
+      if (st + 1 < nsteps) {
+        issue_b(hb + 64, nxt);
+        nf = fetch(hb + 64);
+      }
+      v4i ad[RG][kNd];
 #pragma unroll
       for (int rg = 0; rg < RG; ++rg) {
         uint32_t w[4][kNd];
@@ -379,23 +430,42 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
           }
           transpose4(lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3], w[qq]);
         }
-        v4i ad[kNd];
 #pragma unroll
         for (int t = 0; t < kNd; ++t)
-          ad[t] = v4i{(int)w[0][t], (int)w[1][t], (int)w[2][t], (int)w[3][t]};
+          ad[rg][t] = v4i{(int)w[0][t], (int)w[1][t], (int)w[2][t], (int)w[3][t]};
+      }
 #pragma unroll
-        for (int cg = 0; cg < NCG; ++cg) {
+      for (int cg = 0; cg < NCG; ++cg) {
+        v4i bd[kNdB];
 #pragma unroll
-          for (int s = 0; s < kNd; ++s)
+        for (int t = 0; t < kNdB; ++t) bd[t] = cur[cg][t][lane];
+#pragma unroll
+        for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+          for (int s2 = 0; s2 < kNd; ++s2)
 #pragma unroll
             for (int t = 0; t < kNdB; ++t) {
-              if (s + t < kDmin) continue;
-              iacc[rg][cg][s + t - kDmin] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                  ad[s], bd[cg][t], iacc[rg][cg][s + t - kDmin], 0, 0, 0);
+              if (s2 + t < kDmin) continue;
+              iacc[rg][cg][s2 + t - kDmin] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  ad[rg][s2], bd[t], iacc[rg][cg][s2 + t - kDmin], 0, 0, 0);
             }
-        }
       }
-    } else {
+      unstage();
+    };
+    if (nsteps > 0) {
+      issue_b(hb0, lds_b0);
+      nf = fetch(hb0);
+    }
+    for (int st = 0; st < nsteps; st += 2) {
+      step(st, lds_b0, lds_b1);
+      if (st + 1 < nsteps) step(st + 1, lds_b1, lds_b0);
+    }
+  } else {
+    if (nsteps > 0) nf = fetch(hb0);
+    for (int st = 0; st < nsteps; ++st) {
+      const int hb = hb0 + 64 * st;
+      commit(nf);
+      if (st + 1 < nsteps) nf = fetch(hb + 64);
       // FP64: 16 MFMA steps of 4 c; lane (j, g) supplies h = hb + 4 m + g
 #pragma unroll 4
       for (int m = 0; m < 16; ++m) {
@@ -414,9 +484,10 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
             dacc[rg][cg] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[cg], dacc[rg][cg], 0, 0, 0);
         }
       }
+      unstage();
     }
-    unstage();
   }
+  if (own_hi < own_lo) return;
 
   // ---- epilogue: lane (j, g) holds four rows of column 16 cg + j: rows
   // 4 g + i in register i for the int8 MFMA, rows 4 i + g for the FP64 one ----
@@ -521,18 +592,22 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs A) {
 
 // ---- tiles -------------------------------------------------------------------
 // For each anti-diagonal S the owned rows l1 in [max(lmin, S - lmax), min(S/2, lmax)]
-// are cut into chunks of `rows` from the diagonal down; a workgroup takes the
-// same chunk of 4 consecutive anti-diagonals; workgroups longest first, laid
-// out in runs per XCD as the VALU kernels' tiles.
+// are cut into chunks of `rows` from the diagonal down.  A workgroup takes the
+// same chunk of 4 anti-diagonals of one parity (S, S + 2, S + 4, S + 6): their
+// B rows are the same for the same h, so the four waves share one B stream
+// over the union of their K-ranges (the family products vanish outside a
+// wave's own range: zero table padding below it, zero What above LW - 1).
+// A short group is filled with inactive copies (no owned rows) of its first
+// tile.  Workgroups longest first, laid out in runs per XCD as the VALU
+// kernels' tiles.
 std::vector<MTile> make_mix_tiles(int lmin, int lmax, int LW, int odd, int rows) {
-  struct TC {
-    MTile t;
-    int cost;
-  };
-  std::vector<std::vector<TC>> chunks;
+  std::vector<std::vector<MTile>> chunks[2];  // [parity][chunk] -> tiles by S
   for (int S = 2 * lmin; S <= 2 * lmax; ++S) {
     const int hi = std::min(S / 2, lmax);
     const int lo = std::max(lmin, S - lmax);
+    const int p = (S - odd) & 1, H = (S - odd - p) >> 1;
+    const int clo = std::max(0, (S - odd - (LW - 1) + 1) >> 1);
+    const int hend = H - clo;
     int k = 0;
     for (int top = hi; top >= lo; top -= rows, ++k) {
       MTile t;
@@ -540,30 +615,42 @@ std::vector<MTile> make_mix_tiles(int lmin, int lmax, int LW, int odd, int rows)
       t.x0 = std::max(top - rows + 1, 0);
       t.lo = std::max(lo, top - rows + 1);
       t.hi = top;
-      const int p = (S - odd) & 1, H = (S - odd - p) >> 1;
-      const int hb0 = std::max(0, H - (t.x0 + rows - 1)) & ~15;
-      const int clo = std::max(0, (S - odd - (LW - 1) + 1) >> 1);
-      const int hend = H - clo;
-      const int cost = hend >= hb0 ? (hend - hb0 + 64) >> 6 : 0;
-      if ((int)chunks.size() <= k) chunks.resize(k + 1);
-      chunks[k].push_back({t, cost});
+      // own K-range [hb0, hend] (nsteps = 0: nothing to contract)
+      t.hb0 = std::max(0, H - (t.x0 + rows - 1)) & ~15;
+      t.nsteps = hend;  // last h, replaced by the group's step count below
+      auto& ck = chunks[p];
+      if ((int)ck.size() <= k) ck.resize(k + 1);
+      ck[k].push_back(t);
     }
   }
   struct WG {
-    int first, n, cost;
+    MTile t[4];
+    int cost;
   };
-  std::vector<TC> flat;
   std::vector<WG> wgs;
-  for (auto& ck : chunks)
-    for (size_t i = 0; i < ck.size(); i += 4) {
-      WG g{(int)flat.size(), 0, 0};
-      for (size_t q = i; q < std::min(ck.size(), i + 4); ++q) {
-        flat.push_back(ck[q]);
-        g.n++;
-        g.cost = std::max(g.cost, ck[q].cost);
+  for (auto& par : chunks)
+    for (auto& ck : par)
+      for (size_t i = 0; i < ck.size(); i += 4) {
+        WG g;
+        const size_t n = std::min(ck.size(), i + 4) - i;
+        int hb0 = 1 << 30, hend = -1;
+        for (size_t q = 0; q < n; ++q) {
+          hb0 = std::min(hb0, ck[i + q].hb0);
+          hend = std::max(hend, ck[i + q].nsteps);
+        }
+        const int nsteps = hend >= hb0 ? (hend - hb0 + 64) >> 6 : 0;
+        for (size_t q = 0; q < 4; ++q) {
+          g.t[q] = ck[i + (q < n ? q : 0)];
+          if (q >= n) {
+            g.t[q].lo = 1;
+            g.t[q].hi = 0;
+          }
+          g.t[q].hb0 = nsteps > 0 ? hb0 : 0;
+          g.t[q].nsteps = nsteps;
+        }
+        g.cost = nsteps;
+        wgs.push_back(g);
       }
-      wgs.push_back(g);
-    }
   int maxc = 1;
   for (const WG& g : wgs) maxc = std::max(maxc, g.cost);
   const int bucket = std::max(1, maxc / 32);
@@ -577,29 +664,33 @@ std::vector<MTile> make_mix_tiles(int lmin, int lmax, int LW, int odd, int rows)
     const int x = b % kXcds, m = b / kXcds;
     const long w = (long)(m / kRun) * kXcds * kRun + (long)x * kRun + m % kRun;
     if (w >= nwg) continue;
-    const WG& g = wgs[w];
-    for (int q = 0; q < 4; ++q)
-      out.push_back(q < g.n ? flat[g.first + q].t : MTile{2 * lmin, 0, 0, -1});
+    for (int q = 0; q < 4; ++q) out.push_back(wgs[w].t[q]);
   }
   return out;
 }
 
-// Host check of every table index a tile set can touch (the kernel's access
-// list: staged lane factors T[l - c - odd] over the window, T[c], Ts[S - c],
-// env[l3], B rows h) against the buffer sizes.
-void check_mix_geometry(const std::vector<MTile>& tiles, int lmax, int LW, int odd, int RG,
-                        int ntab, int hblocks, int envlen) {
-  for (const MTile& t : tiles) {
-    if (t.hi < t.lo) continue;
-    PSX_REQUIRE(t.x0 >= 0 && t.lo >= t.x0 && t.hi <= t.x0 + 16 * RG - 1 && t.hi <= lmax,
-                PSX_ERR_ARGUMENT, "mixed tile geometry: bad row range");
+// Host check of every index a tile set can touch (the kernel's access list:
+// staged lane factors T[l - c - odd] over the window, T[c], Ts[S - c],
+// env[l3], B rows h over the workgroup's K-range) against the buffer sizes,
+// and of the workgroup invariants the shared B stream relies on.
+void check_mix_geometry(const std::vector<MTile>& tiles, int lmax, int odd, int RG, int ntab,
+                        int hblocks, int envlen) {
+  PSX_REQUIRE(tiles.size() % 4 == 0, PSX_ERR_ARGUMENT, "mixed tiles: not whole workgroups");
+  for (size_t i = 0; i < tiles.size(); ++i) {
+    const MTile& t = tiles[i];
+    const MTile& t0 = tiles[i & ~size_t(3)];
+    PSX_REQUIRE(t.hb0 == t0.hb0 && t.nsteps == t0.nsteps && ((t.S - t0.S) & 1) == 0 &&
+                    t.hb0 >= 0 && (t.hb0 & 15) == 0 && t.nsteps >= 0,
+                PSX_ERR_ARGUMENT, "mixed tile geometry: workgroup K-range");
+    PSX_REQUIRE(t.x0 >= 0 && t.x0 + 16 * RG - 1 <= lmax + 16 * RG && t.S >= 0 && t.S <= 2 * lmax,
+                PSX_ERR_ARGUMENT, "mixed tile geometry: bad tile");
+    if (t.hi >= t.lo)
+      PSX_REQUIRE(t.lo >= t.x0 && t.hi <= t.x0 + 16 * RG - 1 && t.hi <= lmax, PSX_ERR_ARGUMENT,
+                  "mixed tile geometry: bad row range");
+    if (t.nsteps == 0) continue;
     const int S = t.S, p = (S - odd) & 1, H = (S - odd - p) >> 1;
-    const int hb0 = std::max(0, H - (t.x0 + 16 * RG - 1)) & ~15;
-    const int clo = std::max(0, (S - odd - (LW - 1) + 1) >> 1);
-    const int hend = H - clo;
-    if (hend < hb0) continue;
-    const int nsteps = (hend - hb0 + 64) >> 6;
-    const int hlast = hb0 + 64 * nsteps - 1;
+    const int hb0 = t.hb0;
+    const int hlast = hb0 + 64 * t.nsteps - 1;
     const int kUw = 16 * (RG - 1) + 80;
     const long u1lo = (long)t.x0 - H + hb0 - odd, u1hi = (long)t.x0 - H + hlast - 63 - odd + kUw - 1;
     const long b2 = (long)S - t.x0 - 16 * RG + 1 - H - odd;
@@ -609,7 +700,7 @@ void check_mix_geometry(const std::vector<MTile>& tiles, int lmax, int LW, int o
     const long lo = std::min({u1lo, u2lo, clo_i, slo}), hi = std::max({u1hi, u2hi, chi_i, shi});
     PSX_REQUIRE(lo >= -kTabOff && hi + kTabOff < ntab, PSX_ERR_ARGUMENT,
                 "mixed tile geometry: table index out of range");
-    PSX_REQUIRE(2L * hlast + p < envlen && (hlast >> 4) < hblocks && hb0 >= 0, PSX_ERR_ARGUMENT,
+    PSX_REQUIRE(2L * hlast + p < envlen && (hlast >> 4) < hblocks, PSX_ERR_ARGUMENT,
                 "mixed tile geometry: column table out of range");
   }
 }
@@ -640,7 +731,7 @@ MixFn mix_fn(int ncg) {
 }
 
 #ifndef PSX_MIX_RG
-#define PSX_MIX_RG 2
+#define PSX_MIX_RG 1
 #endif
 
 template <int MODE>
@@ -726,8 +817,8 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
     auto it = g_mix_tiles.find(key);
     if (it == g_mix_tiles.end()) {
       std::vector<MTile> h = make_mix_tiles(rq.lmin, rq.lmax, LW, odd, 16 * RG);
-      check_mix_geometry(h, rq.lmax, LW, odd, RG, table_entries(LW + 2 * rq.lmax + 512),
-                         hblocks, envlen);
+      check_mix_geometry(h, rq.lmax, odd, RG, table_entries(LW + 2 * rq.lmax + 512), hblocks,
+                         envlen);
       MixTiles m;
       m.n = (int)h.size();
       m.d = (MTile*)ctx_alloc(c, sizeof(MTile) * std::max<size_t>(1, h.size()));
