@@ -126,8 +126,10 @@ contract_kernel(KArgs A) {
 
   const int lane = threadIdx.x & 63;
   const int wave = ufl(threadIdx.x >> 6);
-  const int tile = ufl(blockIdx.x * (blockDim.x >> 6) + wave);
-  if (tile >= A.ntiles) return;
+  // tile lists come in whole workgroups (padded with empty tiles): every wave
+  // of a launched workgroup has a tile and reaches the epilogue's barriers
+  const int tile = ufl(blockIdx.x * (TPB / 64) + wave);
+  if (ufl(blockIdx.x * (TPB / 64)) >= A.ntiles) return;
   const int x0 = sload(&A.tiles[tile].x0, 0);
   const int S0 = sload(&A.tiles[tile].S0, 0);
   const int own_lo = sload(&A.tiles[tile].lo, 0);
@@ -245,40 +247,111 @@ contract_kernel(KArgs A) {
   }
 
   // ---- epilogue ----
+  // The workgroup's tiles are one row chunk over WG R consecutive
+  // anti-diagonals: S0 of wave w = S0 of wave 0 + w R, empty tiles (hi < lo)
+  // last (checked on the host).  Each output matrix goes through LDS and
+  // leaves as column segments: column l1 takes rows l2 = S - l1 (the mirrored
+  // element) and column l2 rows l1 = S - l2 (the direct one) over the span of
+  // anti-diagonals, consecutive threads on consecutive rows, instead of one
+  // scattered 8-byte store per lane.
+  constexpr int WG = TPB / 64, SPAN = WG * R, CPP = TPB / SPAN;
+  static_assert(TPB % SPAN == 0, "anti-diagonal span must divide the workgroup");
+  __shared__ double xs[WG][R][64];
+  __shared__ int tinfo[WG][4];
   const int lmin = A.lmin;
+  if (lane == 0) {
+    tinfo[wave][0] = x0;
+    tinfo[wave][1] = S0;
+    tinfo[wave][2] = own_lo;
+    tinfo[wave][3] = own_hi;
+  }
   const double L1 = (double)l1;
   const double pi1 = (L1 - 1.0) * L1 * (L1 + 1.0) * (L1 + 2.0);
-  bool bad = false;
+  double k02r[kE ? R : 1], k22r[(NPE + NPO) > 0 ? R : 1];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int l2 = S0 + r - l1;
-    if (l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax) continue;
     const double L2 = (double)l2;
     const double pi2 = (L2 - 1.0) * L2 * (L2 + 1.0) * (L2 + 2.0);
     const bool spin2 = (l1 >= 2) && (l2 >= 2);
-    const double k22 = spin2 ? 1.0 / (4.0 * pi1 * pi2) : 0.0;
-    const double k02 = spin2 ? 1.0 / (2.0 * sqrt(pi1 * pi2)) : 0.0;
-    double xi[NV];
+    if constexpr (kE) k02r[r] = spin2 ? 1.0 / (2.0 * sqrt(pi1 * pi2)) : 0.0;
+    if constexpr ((NPE + NPO) > 0) k22r[r] = spin2 ? 1.0 / (4.0 * pi1 * pi2) : 0.0;
+  }
+  bool bad = false;
+  // the value of pair (l1, S - l1) if this workgroup computed and owns it
+  auto staged = [&](int S, int m1, double& v) -> bool {
+    const int i = S - tinfo[0][1];
+    if (i < 0 || i >= SPAN) return false;
+    const int w = i / R, r = i - w * R;
+    if (m1 < tinfo[w][2] || m1 > tinfo[w][3]) return false;
+    const int m2 = S - m1;
+    if (m1 < lmin || m1 > m2 || m2 > lmax) return false;
+    v = xs[w][r][m1 - tinfo[w][0]];
+    return true;
+  };
+  // column segments of one output matrix; MCM weights (2 l2 + 1) on the
+  // direct element, (2 l1 + 1) on the mirrored one, none for covariance
+  auto store = [&](double* o, bool weighted) {
+    __syncthreads();
+    int l1lo = lmax + 1, l1hi = -1;
+#pragma unroll
+    for (int w = 0; w < WG; ++w)
+      if (tinfo[w][3] >= tinfo[w][2]) {
+        l1lo = min(l1lo, tinfo[w][2]);
+        l1hi = max(l1hi, tinfo[w][3]);
+      }
+    const int s0 = tinfo[0][1];
+    const int i = threadIdx.x % SPAN, cs = threadIdx.x / SPAN;
+    const int S = s0 + i;
+    for (int m1 = l1lo + cs; m1 <= l1hi; m1 += CPP) {
+      double v;
+      if (staged(S, m1, v))
+        o[(int64_t)(S - m1 - lmin) + (int64_t)(m1 - lmin) * A.ldo] =
+            weighted ? (2.0 * (double)m1 + 1.0) * v : v;
+    }
+    for (int m2 = s0 - l1hi + cs; m2 <= s0 + SPAN - 1 - l1lo; m2 += CPP) {
+      double v;
+      if (staged(S, S - m2, v))
+        o[(int64_t)(S - m2 - lmin) + (int64_t)(m2 - lmin) * A.ldo] =
+            weighted ? (2.0 * (double)m2 + 1.0) * v : v;
+    }
+    __syncthreads();
+  };
+  if constexpr (EPI == EPI_MCM) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const double kap_k = k < N00 ? 1.0 : (k < N00 + N02 ? k02 : k22);
-      xi[k] = kap_k * acc[k][r];
-    }
-    const int64_t i12 = (int64_t)(l1 - lmin) + (int64_t)(l2 - lmin) * A.ldo;
-    const int64_t i21 = (int64_t)(l2 - lmin) + (int64_t)(l1 - lmin) * A.ldo;
-    if constexpr (EPI == EPI_MCM) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        A.out[k][i12] = (2.0 * L2 + 1.0) * xi[k];
-        A.out[k][i21] = (2.0 * L1 + 1.0) * xi[k];
+      for (int r = 0; r < R; ++r) {
+        double kap_k = 1.0;
+        if constexpr (N02 > 0)
+          if (k >= N00 && k < N00 + N02) kap_k = k02r[r];
+        if constexpr ((NPE + NPO) > 0)
+          if (k >= N00 + N02) kap_k = k22r[r];
+        xs[wave][r][lane] = kap_k * acc[k][r];
       }
-    } else {
-      const double v = cov_combine<COVK>(l1, l2, xi, A.spec, A.ratio, bad);
-      A.out[0][i12] = v;
-      A.out[0][i21] = v;
+      store(A.out[k], true);
     }
-  }
-  if constexpr (EPI == EPI_COV) {
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int l2 = S0 + r - l1;
+      double v = 0.0;
+      if (!(l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax)) {
+        double xi[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          double kap_k = 1.0;
+          if constexpr (N02 > 0)
+            if (k >= N00 && k < N00 + N02) kap_k = k02r[r];
+          if constexpr ((NPE + NPO) > 0)
+            if (k >= N00 + N02) kap_k = k22r[r];
+          xi[k] = kap_k * acc[k][r];
+        }
+        v = cov_combine<COVK>(l1, l2, xi, A.spec, A.ratio, bad);
+      }
+      xs[wave][r][lane] = v;
+    }
+    store(A.out[0], false);
     if (bad) atomicOr(A.err, 1);
   }
 }
@@ -483,10 +556,26 @@ struct Geometry {
 // the buffers before launch (the access list follows the kernel: lane reads
 // of G/Z at l1-c(-1) and t+q(-1), uniform reads of D/IG at S0+q, of G/Z at c,
 // of the l3 records at l3 = S0+r-2c).
-Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R,
+Geometry check_geometry(const std::vector<Tile>& tiles, int lmax, int LW, int R, int waves,
                         int tables_n) {
   Geometry g{};
   g.R = R;
+  // workgroup invariants of the epilogue: whole workgroups, non-empty tiles
+  // first, wave j on anti-diagonals S0 + j R of its first tile
+  PSX_REQUIRE(waves >= 1 && tiles.size() % waves == 0, PSX_ERR_ARGUMENT,
+              "tile geometry: not whole workgroups");
+  for (size_t w0 = 0; w0 < tiles.size(); w0 += waves) {
+    bool empty_seen = false;
+    for (int j = 0; j < waves; ++j) {
+      const Tile& t = tiles[w0 + j];
+      if (t.hi < t.lo) {
+        empty_seen = true;
+        continue;
+      }
+      PSX_REQUIRE(!empty_seen && t.S0 == tiles[w0].S0 + j * R, PSX_ERR_ARGUMENT,
+                  "tile geometry: workgroup anti-diagonals not consecutive");
+    }
+  }
   long lo = 0, hi = 0, l3lo = 0, l3hi = 0;
   bool first = true;
   for (const Tile& t : tiles) {
@@ -559,7 +648,7 @@ const CachedTiles& get_tiles(Ctx* c, int lmin, int lmax, int LW, int R, int wave
   CachedTiles ct;
   ct.host = make_tiles(lmin, lmax, LW, R, waves);
   ct.bands = band_tiles(ct.host, waves, lmin, lmax, nbands);
-  ct.geo = check_geometry(ct.host, lmax, LW, R, 0);
+  ct.geo = check_geometry(ct.host, lmax, LW, R, waves, 0);
   ct.n = (int)ct.host.size();
   ct.d = (Tile*)ctx_alloc(c, sizeof(Tile) * std::max(1, ct.n));
   PSX_HIP(hipMemcpy(ct.d, ct.host.data(), sizeof(Tile) * ct.n, hipMemcpyHostToDevice));
@@ -981,7 +1070,7 @@ void run_contraction_noalloc(Ctx* c, const ContractRequest& rq0) {
   if (it == g_noalloc_tiles.end()) {
     HostTiles ht;
     ht.tiles = make_tiles(rq.lmin, rq.lmax, (int)rq.LW, cf.R, cf.waves);
-    ht.geo = check_geometry(ht.tiles, rq.lmax, (int)rq.LW, cf.R, 0);
+    ht.geo = check_geometry(ht.tiles, rq.lmax, (int)rq.LW, cf.R, cf.waves, 0);
     it = g_noalloc_tiles.emplace(key, std::move(ht)).first;
   }
   const HostTiles& ht = it->second;
