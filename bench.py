@@ -340,6 +340,17 @@ def run_mixed(args, pkg) -> dict:
     lmax = args.lmax
     pool = mask_W_pool(ctx, 12, lmax, 64)
     sweep = []
+    if args.batch_k:
+        # one batch size only (profiling runs)
+        r = batch_measure(ctx, abi, 0, lmax, pool[:args.batch_k], args.steps, args.warmup,
+                          check=range(0, args.batch_k, max(1, args.batch_k // 8)))
+        ctx.close()
+        return _line("MCM elements/s at lmax %d, a batch of %d TT blocks: FP64 vs mixed"
+                     % (lmax, args.batch_k), r["mixed"]["elements_per_s"], "elements/s", args,
+                     r["mixed"]["ms_per_step"],
+                     {"workload": f"mcm_batch_lmax{lmax}_K{args.batch_k}", "lmax": lmax},
+                     data="seeded masks: twelve nside-1024 cut-sky masks, distinct cross-spectra",
+                     batch=r)
     for K in (8, 16, 32, 64):
         sweep.append(batch_measure(ctx, abi, 0, lmax, pool[:K], args.steps, args.warmup,
                                    check=range(0, K, max(1, K // 8))))
@@ -600,6 +611,8 @@ def run_solve(args, pkg) -> dict:
                               "(host buffers, PCIe included)", "lmax": lmax, "LW": LW,
                   "W": "two seeded nside-2048 masks, 500 holes" if args.data == "masks"
                   else "analytic stand-ins"},
+                 data=("seeded masks: two nside-2048 cut-sky masks, 500 holes" if args.data == "masks"
+                       else "synthetic: seeded analytic stand-ins for the mask spectra"),
                  kernels=res,
                  solve={"lu_tt_s": t1 - t0, "lu_eebb_s": t3 - t2, "lu_tt_device_s": dev_s[0],
                         "lu_eebb_device_s": dev_s[1], "lu_eebb_from_blocks_s": t5 - t4,
@@ -843,13 +856,33 @@ def run_serve(args, pkg) -> dict:
     Poisson stream at 80 % of it; latency of every request in the mix against
     the same request alone.  Results stay on the device (released after
     completion)."""
+    n_req = 2000 if args.requests == 0 else args.requests
+    sv = serve_measure(pkg, n_req, args.slots, SERVE_RUNS)
+    runs, sat, alone = sv["runs"], sv["saturation_req_per_s"], sv["alone"]
+    head = runs[1]
+    pct = lambda x, q: float(np.percentile(x, q))
+    return _line("serving req/s + p99 latency (MCM and covariance requests, lmax 511-4095)",
+                 head["achieved_req_per_s"], "req/s", args, 1e3 / head["achieved_req_per_s"],
+                 {"workload": f"serve_{n_req}: Poisson arrivals at 80 % of saturation; half MCM "
+                              "(TT, TE, EE, BB), half covariance (TTTT, EEEE, TETE, TTEE); lmax "
+                              "log-uniform in [511, 4095]; results kept on the device",
+                  "slots": args.slots, "requests": n_req},
+                 serving={"saturation_req_per_s": sat,
+                          "alone_p50_ms": pct(alone, 50) * 1e3, "alone_p99_ms": pct(alone, 99) * 1e3,
+                          "runs": runs,
+                          "queue_model": [queue_model(alone, sat, ld) for ld in (0.3, 0.5, 0.8)]})
+
+
+def serve_measure(pkg, n_req: int, slots: int, load_runs) -> dict:
+    """The serving measurement: every request alone, saturation (all submitted
+    at once), then one Poisson stream per (load, policy, slack) of
+    `load_runs`."""
     import threading
 
     abi = pkg._abi
     ctx = pkg.Context(0)
-    n_req = 2000 if args.requests == 0 else args.requests
     LMAX = 4095
-    srv = pkg.Server(ctx, slots=args.slots, arena_bytes=48 << 30, lmax_max=LMAX)
+    srv = pkg.Server(ctx, slots=slots, arena_bytes=48 << 30, lmax_max=LMAX)
     npool = 64
     ell = np.arange(LMAX + 1, dtype=np.float64)
     for k in range(npool):                                     # mask spectra pool
@@ -879,6 +912,7 @@ def run_serve(args, pkg) -> dict:
                     ratios=[2000 + int(x) for x in r.integers(16, size=nR)])
 
     reqs = [args_of(q) for q in serve_requests(n_req, seed=11)]
+    n_req = len(reqs)
     # warm the tile lists and kernels of every (kind, lmax) once, untimed
     warm = [srv.submit(**r) for r in reqs]
     for t in warm:
@@ -941,31 +975,25 @@ def run_serve(args, pkg) -> dict:
                 "slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99),
                 "p99_over_alone_p99": pct(lat, 99) / pct(alone, 99)}
 
-    runs = [poisson(load * sat, getattr(abi, policy), slack) for load, policy, slack in SERVE_RUNS]
+    runs = [poisson(load * sat, getattr(abi, policy), slack) for load, policy, slack in load_runs]
     srv.close()
     ctx.close()
-    head = runs[1]
-    pct = lambda x, q: float(np.percentile(x, q))
-    return _line("serving req/s + p99 latency (MCM and covariance requests, lmax 511-4095)",
-                 head["achieved_req_per_s"], "req/s", args, 1e3 / head["achieved_req_per_s"],
-                 {"workload": f"serve_{n_req}: Poisson arrivals at 80 % of saturation; half MCM "
-                              "(TT, TE, EE, BB), half covariance (TTTT, EEEE, TETE, TTEE); lmax "
-                              "log-uniform in [511, 4095]; results kept on the device",
-                  "slots": args.slots, "requests": n_req},
-                 serving={"saturation_req_per_s": sat,
-                          "alone_p50_ms": pct(alone, 50) * 1e3, "alone_p99_ms": pct(alone, 99) * 1e3,
-                          "runs": runs})
+    return {"runs": runs, "saturation_req_per_s": sat, "alone": alone}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--lmax", type=int, default=3071)
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP64 vs mixed batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-k", type=int, default=0,
+                    help="mixed workload: time one batch of this many TT blocks only")
     ap.add_argument("--data", default="masks", choices=["masks", "synthetic"],
                     help="headline W spectra: seeded cut-sky masks (default) or analytic stand-ins")
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
@@ -1074,11 +1102,11 @@ def main() -> int:
         },
         "kernel_ms": {"psx_mcm_set": k_avg, "psx_records": v_ms / max(v_n, 1)},
         "roofline": {
-            # compute-bound on FP64: the kernel issues v_fma_f64 / v_mul_f64 on
-            # the vector ALUs; MI355X's FP64 vector and FP64 matrix peaks are
-            # the same 78.6 TFLOP/s
+            # compute-bound on FP64 (the roof named "mfma" is the compute roof):
+            # the kernel issues v_fma_f64 / v_mul_f64 on the vector ALUs, whose
+            # FP64 peak on MI355X equals the FP64 matrix peak, 78.6 TFLOP/s
             "bound": "mfma",
-            "pipe": "fp64 vector (no MFMA)",
+            "pipe": "compute roof, FP64 on the vector ALUs (FP64 vector peak = FP64 matrix peak)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -1090,7 +1118,10 @@ def main() -> int:
             "unit_def": "(l1<=l2, c) steps with even l3 <= LW-1; 20 FP64 flops each",
         },
         "mixed": None,
+        "serving": None,
     }
+    if not args.no_mixed:
+        out["mixed"] = mixed_leg(ctx, abi, lmax)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, physical_cores(), len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(lmax, LW, Ws, args.cpu_stride, nthreads)
@@ -1099,8 +1130,104 @@ def main() -> int:
     for d in dW + dM:
         ctx.free(d)
     ctx.close()
+    if not args.no_serve:
+        out["serving"] = serving_leg(pkg)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def mixed_leg(ctx, abi, lmax: int) -> dict:
+    """FP64 against the matrix-core contractions (int8 digit split, FP64 MFMA)
+    on batches of K = 16 and 64 TT blocks at lmax from distinct mask
+    cross-spectra; errors per checked block against the FP64 blocks."""
+    pool = mask_W_pool(ctx, 12, lmax, 64)
+    res = {}
+    for K in (16, 64):
+        r = batch_measure(ctx, abi, 0, lmax, pool[:K], 5, 1, check=range(0, K, max(1, K // 16)))
+        res[f"K{K}"] = {
+            "fp64_ms": r["fp64"]["ms_per_step"], "mixed_ms": r["mixed"]["ms_per_step"],
+            "fp64_mfma_ms": r["fp64_mfma"]["ms_per_step"],
+            "fp64_elements_per_s": r["fp64"]["elements_per_s"],
+            "mixed_elements_per_s": r["mixed"]["elements_per_s"],
+            "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
+            "speedup_fp64_mfma_vs_fp64": r["speedup_fp64_mfma_vs_fp64"],
+            "mixed_max_rel_err": r["mixed"]["max_rel_err"],
+            "mixed_max_normwise_err": r["mixed"]["max_normwise_err"],
+            "mixed_per_block_max_rel_err": r["mixed"]["per_block_max_rel_err"],
+            "fp64_mfma_max_rel_err": r["fp64_mfma"]["max_rel_err"],
+            "checked_blocks": r["checked_blocks"],
+        }
+    return {"workload": f"K TT blocks at lmax {lmax} from K distinct cross-spectra of twelve "
+                        "seeded nside-1024 cut-sky masks, W length lmax+1",
+            "precisions": {"fp64": "FP64 VALU kernels, groups of 8 spectra per launch",
+                           "mixed": "int8 MFMA, 6 x 7 balanced base-256 digits, 27 products",
+                           "fp64_mfma": "v_mfma_f64_16x16x4_f64"},
+            "error": "elementwise relative over |M| >= 1e-6 max|M| and normwise, against FP64",
+            **res}
+
+
+def queue_model(alone, sat: float, load: float, n: int = 200000, seed: int = 5) -> dict:
+    """What scheduling alone could give at this load: one ideal server with
+    the measured service-time distribution (the requests' alone times, scaled
+    so the server's capacity is the measured saturation rate), Poisson
+    arrivals at `load` of capacity, under FIFO and under preemptive
+    shortest-remaining-work-first (the policy that minimises time in system).
+    Returns p99 latency / p99 alone for both, and the moments behind the
+    Pollaczek-Khinchine mean wait lambda E[S^2] / (2 (1 - rho))."""
+    import heapq
+
+    rng = np.random.default_rng(seed)
+    a = np.asarray(alone, dtype=np.float64)
+    x = rng.choice(a, size=n) * ((1.0 / sat) / a.mean())
+    lam = load * sat
+    t = np.cumsum(rng.exponential(1.0 / lam, size=n))
+    # FIFO: Lindley recursion
+    w = np.zeros(n)
+    for i in range(1, n):
+        w[i] = max(0.0, w[i - 1] + x[i - 1] - (t[i] - t[i - 1]))
+    fifo = w + x
+    # preemptive SRPT: event simulation over arrivals and completions
+    srpt = np.zeros(n)
+    heap = []
+    now = 0.0
+    i = 0
+    while i < n or heap:
+        nxt = t[i] if i < n else np.inf
+        if heap:
+            rem, k = heap[0]
+            if now + rem <= nxt:
+                heapq.heappop(heap)
+                now += rem
+                srpt[k] = now - t[k]
+                continue
+            heapq.heapreplace(heap, (rem - (nxt - now), k))
+        now = nxt
+        heapq.heappush(heap, (x[i], i))
+        i += 1
+    p99a = float(np.percentile(a * ((1.0 / sat) / a.mean()), 99))
+    es, es2 = float(x.mean()), float((x * x).mean())
+    return {"model": "M/G/1, measured service-time distribution, capacity = saturation",
+            "load": load, "E_S_ms": es * 1e3, "E_S2_over_E_S_ms": es2 / es * 1e3,
+            "pk_mean_wait_ms": lam * es2 / (2.0 * (1.0 - load)) * 1e3,
+            "fifo_p99_over_alone_p99": float(np.percentile(fifo, 99)) / p99a,
+            "srpt_p99_over_alone_p99": float(np.percentile(srpt, 99)) / p99a}
+
+
+def serving_leg(pkg) -> dict:
+    """A reduced serving stream (400 requests, lmax 511-4095): saturation, then
+    Poisson arrivals at 80 % of it under deadline admission; latency against
+    the same request alone."""
+    sv = serve_measure(pkg, 400, 4, [(0.8, "SCHED_DEADLINE", 2.0)])
+    run = sv["runs"][0]
+    alone = sv["alone"]
+    return {"requests": 400, "load": 0.8, "policy": "deadline, slack 2",
+            "req_per_s": run["achieved_req_per_s"],
+            "saturation_req_per_s": sv["saturation_req_per_s"],
+            "latency_p50_ms": run["latency_p50_ms"], "latency_p99_ms": run["latency_p99_ms"],
+            "alone_p99_ms": float(np.percentile(alone, 99)) * 1e3,
+            "p99_over_alone_p99": run["p99_over_alone_p99"],
+            "slowdown_p99": run["slowdown_p99"],
+            "queue_model": queue_model(alone, sv["saturation_req_per_s"], 0.8)}
 
 
 if __name__ == "__main__":
