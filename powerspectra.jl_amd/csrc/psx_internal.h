@@ -152,8 +152,9 @@ void release_sht_plans(Ctx* c);
 void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
                  double* dalms);
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps);
+// (spin0 = false: the spin-2 part only, the I maps' alms left as they are)
 void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
-                     double* dteb);
+                     double* dteb, bool spin0 = true);
 void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb, double* diqu);
 
 // Column-major matrix copies between host memory (any pageable buffer,

@@ -360,14 +360,32 @@ __device__ __forceinline__ void rescale_pair(double& a, double& b, int& s, doubl
   }
 }
 
-// Analysis partial sums: part[chunk][idx(l, m)] = sum over the chunk's pairs of
-// lambda_lm(z_p) F+-_p(m).  Each thread carries kPairsPerThread independent
-// recurrences; per round of kLChunk l values the 256 partial sums of every
-// (l, re/im) are reduced through LDS.
-__global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double2* __restrict__ Fp,
-                                                         const double2* __restrict__ Fm,
-                                                         double2* __restrict__ part) {
-  __shared__ double red[kLChunk][2][256];
+// Up to kMaxGroup maps share one pass of the recurrences: lambda_lm(z) is the
+// same for every map, only the accumulations are per map.
+constexpr int kMaxGroup = 4;
+
+template <int NM>
+struct MapPtrs {
+  const double2* in[NM];   // analysis: F+ of each map; synthesis: a_lm
+  const double2* in2[NM];  // analysis: F- of each map
+  double2* out[NM];        // analysis: partial sums; synthesis: G
+};
+
+// l values per LDS reduction round of the analysis: 2 NM values per l and
+// thread go through the round's 16 rows of 256 doubles (32 KB)
+__host__ __device__ constexpr int an_lchunk(int nm) { return nm == 1 ? 8 : nm == 2 ? 4 : 2; }
+
+// Analysis partial sums: part_q[chunk][idx(l, m)] = sum over the chunk's pairs
+// of lambda_lm(z_p) F+-_p(m) for each map q of the group.  Each thread carries
+// kPairsPerThread independent recurrences; per round of LC l values the 256
+// partial sums of every (l, map, re/im) are reduced through LDS, in the same
+// order whatever the group size (one map alone gives the same sums).
+template <int NM>
+__global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, MapPtrs<NM> M) {
+  constexpr int LC = an_lchunk(NM);
+  constexpr int NO = 2 * LC * NM;  // outputs per round
+  static_assert(NO <= 16, "one reduction pass per round");
+  __shared__ double red[NO][256];
   const int m = blockIdx.y;
   const int chunk = blockIdx.x;
   const int lmax = A.R.lmax;
@@ -375,7 +393,7 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
   const double2* ABm = A.AB + alm_idx(lmax, m, m);   // ABm[l - m]
   double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread], wgt[kPairsPerThread];
   int sc[kPairsPerThread];
-  double2 fp[kPairsPerThread], fm[kPairsPerThread];
+  double2 fp[NM][kPairsPerThread], fm[NM][kPairsPerThread];
 #pragma unroll
   for (int r = 0; r < kPairsPerThread; ++r) {
     const int p = (chunk * kPairsPerThread + r) * 256 + tid;
@@ -386,17 +404,22 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
     wgt[r] = scale_weight(sc[r]);
     l2[r] = 0.0;
     const int64_t o = (int64_t)m * A.R.npairs + pp;
-    fp[r] = live ? Fp[o] : make_double2(0.0, 0.0);
-    fm[r] = live ? Fm[o] : make_double2(0.0, 0.0);
-  }
-  for (int l0 = m; l0 <= lmax; l0 += kLChunk) {
 #pragma unroll
-    for (int j = 0; j < kLChunk; ++j) {
+    for (int q = 0; q < NM; ++q) {
+      fp[q][r] = live ? M.in[q][o] : make_double2(0.0, 0.0);
+      fm[q][r] = live ? M.in2[q][o] : make_double2(0.0, 0.0);
+    }
+  }
+  for (int l0 = m; l0 <= lmax; l0 += LC) {
+#pragma unroll
+    for (int j = 0; j < LC; ++j) {
       const int l = l0 + j;
-      double accr = 0.0, acci = 0.0;
+      double accr[NM], acci[NM];
+#pragma unroll
+      for (int q = 0; q < NM; ++q) accr[q] = acci[q] = 0.0;
       if (l <= lmax) {
         const double2 ab = sload(ABm + (l - m));   // (0, 0) at l = m
-        const bool odd = j & 1;                    // l0 - m is a multiple of kLChunk
+        const bool odd = j & 1;                    // l0 - m is a multiple of LC
 #pragma unroll
         for (int r = 0; r < kPairsPerThread; ++r) {
           double lam;
@@ -408,34 +431,42 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, const double
             l1[r] = lam;
           }
           const double lt = lam * wgt[r];
-          const double2 f = odd ? fm[r] : fp[r];
-          accr = fma(lt, f.x, accr);
-          acci = fma(lt, f.y, acci);
+#pragma unroll
+          for (int q = 0; q < NM; ++q) {
+            const double2 f = odd ? fm[q][r] : fp[q][r];
+            accr[q] = fma(lt, f.x, accr[q]);
+            acci[q] = fma(lt, f.y, acci[q]);
+          }
         }
       }
-      red[j][0][tid] = accr;
-      red[j][1][tid] = acci;
+#pragma unroll
+      for (int q = 0; q < NM; ++q) {
+        red[(j * NM + q) * 2][tid] = accr[q];
+        red[(j * NM + q) * 2 + 1][tid] = acci[q];
+      }
     }
 #pragma unroll
     for (int r = 0; r < kPairsPerThread; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
     __syncthreads();
-    // 2 kLChunk = 16 outputs, 16 threads each: 16 values per thread, then a
+    // NO <= 16 outputs, 16 threads each: 16 values per thread, then a
     // reduction over the 16 lanes of the group
     {
       const int o = tid >> 4, g = tid & 15;
-      const int j = o >> 1, c = o & 1;
-      double s = 0.0;
+      if (o < NO) {
+        const int j = o / (2 * NM), q = (o >> 1) % NM, c = o & 1;
+        double s = 0.0;
 #pragma unroll
-      for (int t = g * 16; t < g * 16 + 16; ++t) s += red[j][c][t];
-      s += __shfl_xor(s, 1);
-      s += __shfl_xor(s, 2);
-      s += __shfl_xor(s, 4);
-      s += __shfl_xor(s, 8);
-      const int l = l0 + j;
-      if (g == 0 && l <= lmax) {
-        double* dst = (double*)(part + (int64_t)chunk * alm_idx(lmax, lmax + 1, lmax + 1) +
-                                alm_idx(lmax, l, m));
-        dst[c] = s;
+        for (int t = g * 16; t < g * 16 + 16; ++t) s += red[o][t];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        s += __shfl_xor(s, 8);
+        const int l = l0 + j;
+        if (g == 0 && l <= lmax) {
+          double* dst = (double*)(M.out[q] + (int64_t)chunk * alm_idx(lmax, lmax + 1, lmax + 1) +
+                                  alm_idx(lmax, l, m));
+          dst[c] = s;
+        }
       }
     }
     __syncthreads();
@@ -460,38 +491,46 @@ __global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ 
   alm[i] = a;
 }
 
-// Synthesis: G[m][k] = sum_l a_lm lambda_lm(z_k) for kSynPairs ring pairs per
-// thread (north: even + odd l+m parts, south: even - odd); a_lm and the
-// recurrence coefficients are uniform across the workgroup and read with
-// scalar loads, kSynChunk at a time ahead of the steps that use them (a
-// rescale check per such round: growth over 4 steps stays below 2^26).
-__global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const double2* __restrict__ alm,
-                                                          double2* __restrict__ G) {
+// ring pairs per synthesis thread: fewer for larger groups (per pair and
+// map, four accumulators)
+__host__ __device__ constexpr int syn_pairs(int nm) { return nm <= 2 ? kSynPairs : 2; }
+
+// Synthesis: G_q[m][k] = sum_l a_q,lm lambda_lm(z_k) for SP ring pairs per
+// thread and each map q of the group (north: even + odd l+m parts, south:
+// even - odd); a_lm and the recurrence coefficients are uniform across the
+// workgroup and read with scalar loads, kSynChunk at a time ahead of the
+// steps that use them (a rescale check per such round: growth over 4 steps
+// stays below 2^26).
+template <int NM>
+__global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
+  constexpr int SP = syn_pairs(NM);
   const int m = blockIdx.y;
   const int lmax = A.R.lmax;
   const int64_t i0 = alm_idx(lmax, m, m);
-  const double2* am = alm + i0;
   const double2* ABm = A.AB + i0;
-  double z[kSynPairs], l1[kSynPairs], l2[kSynPairs], wgt[kSynPairs];
-  double er[kSynPairs], ei[kSynPairs], orr[kSynPairs], oi[kSynPairs];
-  int sc[kSynPairs], pr[kSynPairs];
+  double z[SP], l1[SP], l2[SP], wgt[SP];
+  double er[NM][SP], ei[NM][SP], orr[NM][SP], oi[NM][SP];
+  int sc[SP], pr[SP];
 #pragma unroll
-  for (int r = 0; r < kSynPairs; ++r) {
-    const int p = (blockIdx.x * kSynPairs + r) * blockDim.x + threadIdx.x;
+  for (int r = 0; r < SP; ++r) {
+    const int p = (blockIdx.x * SP + r) * blockDim.x + threadIdx.x;
     pr[r] = p;
     const int pp = p < A.R.npairs ? p : 0;
     z[r] = A.R.z[pp];
     lambda_start(A, pp, m, l1[r], sc[r]);
     wgt[r] = scale_weight(sc[r]);
     l2[r] = 0.0;
-    er[r] = ei[r] = orr[r] = oi[r] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NM; ++q) er[q][r] = ei[q][r] = orr[q][r] = oi[q][r] = 0.0;
   }
   for (int l0 = m; l0 <= lmax; l0 += kSynChunk) {
-    double2 a[kSynChunk], ab[kSynChunk];
+    double2 a[NM][kSynChunk], ab[kSynChunk];
 #pragma unroll
     for (int j = 0; j < kSynChunk; ++j) {
       const bool in = l0 + j <= lmax;
-      a[j] = in ? sload(am + (l0 + j - m)) : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < NM; ++q)
+        a[q][j] = in ? sload(M.in[q] + i0 + (l0 + j - m)) : make_double2(0.0, 0.0);
       ab[j] = in ? sload(ABm + (l0 + j - m)) : make_double2(0.0, 0.0);
     }
 #pragma unroll
@@ -500,7 +539,7 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const doubl
       if (l > lmax) break;
       const bool odd = j & 1;                      // l0 - m is a multiple of kSynChunk
 #pragma unroll
-      for (int r = 0; r < kSynPairs; ++r) {
+      for (int r = 0; r < SP; ++r) {
         double lam;
         if (l == m) {
           lam = l1[r];
@@ -511,25 +550,33 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, const doubl
           l1[r] = lam;
         }
         const double lt = lam * wgt[r];
-        if (odd) {
-          orr[r] = fma(lt, a[j].x, orr[r]);
-          oi[r] = fma(lt, a[j].y, oi[r]);
-        } else {
-          er[r] = fma(lt, a[j].x, er[r]);
-          ei[r] = fma(lt, a[j].y, ei[r]);
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+          if (odd) {
+            orr[q][r] = fma(lt, a[q][j].x, orr[q][r]);
+            oi[q][r] = fma(lt, a[q][j].y, oi[q][r]);
+          } else {
+            er[q][r] = fma(lt, a[q][j].x, er[q][r]);
+            ei[q][r] = fma(lt, a[q][j].y, ei[q][r]);
+          }
         }
       }
     }
 #pragma unroll
-    for (int r = 0; r < kSynPairs; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
+    for (int r = 0; r < SP; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
   }
 #pragma unroll
-  for (int r = 0; r < kSynPairs; ++r) {
+  for (int r = 0; r < SP; ++r) {
     const int p = pr[r];
     if (p >= A.R.npairs) continue;
     const int kn = p, ks = A.R.nr - 1 - p;
-    G[(int64_t)m * A.R.nr + kn] = make_double2(er[r] + orr[r], ei[r] + oi[r]);
-    if (ks != kn) G[(int64_t)m * A.R.nr + ks] = make_double2(er[r] - orr[r], ei[r] - oi[r]);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+      double2* G = M.out[q];
+      G[(int64_t)m * A.R.nr + kn] = make_double2(er[q][r] + orr[q][r], ei[q][r] + oi[q][r]);
+      if (ks != kn)
+        G[(int64_t)m * A.R.nr + ks] = make_double2(er[q][r] - orr[q][r], ei[q][r] - oi[q][r]);
+    }
   }
 }
 
@@ -1198,9 +1245,8 @@ ShtBufs carve(const ShtPlan& P, void* mem) {
 
 LegArgs leg(const ShtPlan& P) { return LegArgs{P.ra(), P.dAB, P.dlogc, P.nchunk}; }
 
-// alm (+)= A(map)
-void analysis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap, double2* dalm,
-              bool accumulate) {
+// Ring Fourier coefficients of one map, folded into B.Fp, B.Fm.
+void fold_in(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap) {
   const Rings& R = P.R;
   const int ns = R.nside;
   hipStream_t s = c->stream;
@@ -1216,36 +1262,91 @@ void analysis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap, double2*
   hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)),
                      dim3(256), 0, s, P.ra(), B.X, B.Fp, B.Fm);
   PSX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(legendre_analysis, dim3((unsigned)P.nchunk, (unsigned)(P.lmax + 1)),
-                     dim3(256), 0, s, leg(P), B.Fp, B.Fm, B.part);
-  PSX_HIP(hipGetLastError());
-  const double w = 4.0 * kPi / (double)R.npix;
-  hipLaunchKernelGGL(sum_partials, dim3((unsigned)((P.nalm + 255) / 256)), dim3(256), 0, s,
-                     B.part, P.nchunk, P.nalm, w, accumulate ? 1 : 0, dalm);
+}
+
+template <int NM>
+void launch_analysis(ShtPlan& P, hipStream_t s, const ShtBufs* B) {
+  MapPtrs<NM> M{};
+  for (int q = 0; q < NM; ++q) {
+    M.in[q] = B[q].Fp;
+    M.in2[q] = B[q].Fm;
+    M.out[q] = B[q].part;
+  }
+  hipLaunchKernelGGL(legendre_analysis<NM>, dim3((unsigned)P.nchunk, (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, leg(P), M);
   PSX_HIP(hipGetLastError());
 }
 
-// map = S(alm)
-void synthesis(Ctx* c, ShtPlan& P, const ShtBufs& B, const double2* dalm, double* dmap) {
+template <int NM>
+void launch_synthesis(ShtPlan& P, hipStream_t s, const ShtBufs* B, const double2* const* alms) {
+  MapPtrs<NM> M{};
+  for (int q = 0; q < NM; ++q) {
+    M.in[q] = alms[q];
+    M.out[q] = B[q].G;
+  }
+  constexpr int SP = syn_pairs(NM);
+  hipLaunchKernelGGL(legendre_synthesis<NM>,
+                     dim3((unsigned)((P.R.npairs + 256 * SP - 1) / (256 * SP)),
+                          (unsigned)(P.lmax + 1)),
+                     dim3(256), 0, s, leg(P), M);
+  PSX_HIP(hipGetLastError());
+}
+
+// maps per analysis pass (the analysis holds each map's F+- of its ring pairs
+// in registers and reduces 2 values per map and l through LDS)
+#ifndef PSX_AN_GROUP
+#define PSX_AN_GROUP 2
+#endif
+
+// alms[q] (+)= A(maps[q]) for a group of nm <= kMaxGroup maps: Legendre passes
+// of up to PSX_AN_GROUP maps (the recurrence shared), ring transforms per map.
+void analysis_group(Ctx* c, ShtPlan& P, const ShtBufs* B, int nm, const double* const* maps,
+                    double2* const* alms, bool accumulate) {
+  PSX_REQUIRE(nm >= 1 && nm <= kMaxGroup, PSX_ERR_ARGUMENT, "1..4 maps per transform group");
+  hipStream_t s = c->stream;
+  for (int q = 0; q < nm; ++q) fold_in(c, P, B[q], maps[q]);
+  for (int q0 = 0; q0 < nm; q0 += PSX_AN_GROUP) {
+    switch (std::min(PSX_AN_GROUP, nm - q0)) {
+      case 1: launch_analysis<1>(P, s, B + q0); break;
+      case 2: launch_analysis<2>(P, s, B + q0); break;
+      case 3: launch_analysis<3>(P, s, B + q0); break;
+      default: launch_analysis<4>(P, s, B + q0); break;
+    }
+  }
+  const double w = 4.0 * kPi / (double)P.R.npix;
+  for (int q = 0; q < nm; ++q) {
+    hipLaunchKernelGGL(sum_partials, dim3((unsigned)((P.nalm + 255) / 256)), dim3(256), 0, s,
+                       B[q].part, P.nchunk, P.nalm, w, accumulate ? 1 : 0, alms[q]);
+    PSX_HIP(hipGetLastError());
+  }
+}
+
+// maps[q] = S(alms[q]) for a group of nm <= kMaxGroup coefficient sets
+void synthesis_group(Ctx* c, ShtPlan& P, const ShtBufs* B, int nm, const double2* const* alms,
+                     double* const* maps) {
+  PSX_REQUIRE(nm >= 1 && nm <= kMaxGroup, PSX_ERR_ARGUMENT, "1..4 maps per transform group");
   const Rings& R = P.R;
   const int ns = R.nside;
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL(legendre_synthesis,
-                     dim3((unsigned)((R.npairs + 256 * kSynPairs - 1) / (256 * kSynPairs)),
-                          (unsigned)(P.lmax + 1)),
-                     dim3(256), 0, s, leg(P), dalm, B.G);
-  PSX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
-                     dim3(256), 0, s, P.ra(), B.G, B.X);
-  PSX_HIP(hipGetLastError());
-  {
-    void* in[1] = {(void*)(B.X + R.xoff[ns - 1])};
-    void* out[1] = {(void*)(dmap + R.first[ns - 1])};
-    PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
+  switch (nm) {
+    case 1: launch_synthesis<1>(P, s, B, alms); break;
+    case 2: launch_synthesis<2>(P, s, B, alms); break;
+    case 3: launch_synthesis<3>(P, s, B, alms); break;
+    default: launch_synthesis<4>(P, s, B, alms); break;
   }
-  if (ns > 1) {
-    launch_polar_idft(ns, s, P.ra(), B.X, dmap);
+  for (int q = 0; q < nm; ++q) {
+    hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
+                       dim3(256), 0, s, P.ra(), B[q].G, B[q].X);
     PSX_HIP(hipGetLastError());
+    {
+      void* in[1] = {(void*)(B[q].X + R.xoff[ns - 1])};
+      void* out[1] = {(void*)(maps[q] + R.first[ns - 1])};
+      PSX_FFT(rocfft_execute(P.inv, in, out, P.info));
+    }
+    if (ns > 1) {
+      launch_polar_idft(ns, s, P.ra(), B[q].X, maps[q]);
+      PSX_HIP(hipGetLastError());
+    }
   }
 }
 
@@ -1374,37 +1475,81 @@ void check_sht_args(int nside, int lmax) {
 
 }  // namespace
 
-// map2alm of nmaps device maps into device alms (complex interleaved), with
-// niter refinements, on the context stream.
-void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
-                 double* dalms) {
+// map2alm of n device maps (any addresses) into device alms (complex
+// interleaved), with niter refinements, on the context stream.  Maps go in
+// groups of up to kMaxGroup that share their Legendre passes and iterate in
+// lockstep; every result equals the one-map transform bit for bit.
+void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* const* maps,
+                  double* const* alms) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
-  WorkBuf wk(c, sht_work_bytes(P));
-  const ShtBufs B = carve(P, wk.base());
+  const size_t per = (sht_work_bytes(P) + 255) & ~size_t(255);
+  const int gmax = std::min(n, kMaxGroup);
+  WorkBuf wk(c, per * (size_t)std::max(gmax, 1));
+  ShtBufs B[kMaxGroup];
+  for (int q = 0; q < gmax; ++q) B[q] = carve(P, (char*)wk.base() + q * per);
   const int64_t npix = P.R.npix;
-  for (int i = 0; i < nmaps; ++i) {
-    const double* map = dmaps + i * npix;
-    double2* alm = (double2*)dalms + i * P.nalm;
-    analysis(c, P, B, map, alm, false);
+  for (int g0 = 0; g0 < n; g0 += kMaxGroup) {
+    const int nm = std::min(kMaxGroup, n - g0);
+    const double* in[kMaxGroup];
+    double2* alm[kMaxGroup];
+    const double2* calm[kMaxGroup];
+    const double* res[kMaxGroup];
+    double* syn[kMaxGroup];
+    for (int q = 0; q < nm; ++q) {
+      in[q] = maps[g0 + q];
+      alm[q] = (double2*)alms[g0 + q];
+      calm[q] = alm[q];
+      res[q] = B[q].res;
+      syn[q] = B[q].syn;
+    }
+    analysis_group(c, P, B, nm, in, alm, false);
     for (int it = 0; it < niter; ++it) {
-      synthesis(c, P, B, alm, B.syn);
-      hipLaunchKernelGGL(residual, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
-                         c->stream, map, B.syn, npix, B.res);
-      PSX_HIP(hipGetLastError());
-      analysis(c, P, B, B.res, alm, true);
+      synthesis_group(c, P, B, nm, calm, syn);
+      for (int q = 0; q < nm; ++q) {
+        hipLaunchKernelGGL(residual, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+                           c->stream, in[q], (const double*)B[q].syn, npix, B[q].res);
+        PSX_HIP(hipGetLastError());
+      }
+      analysis_group(c, P, B, nm, res, alm, true);
     }
   }
   PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
+// map2alm of nmaps consecutive device maps into consecutive alm sets
+void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
+                 double* dalms) {
+  check_sht_args(nside, lmax);
+  const int64_t npix = 12LL * nside * nside;
+  const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+  std::vector<const double*> m(nmaps);
+  std::vector<double*> a(nmaps);
+  for (int i = 0; i < nmaps; ++i) {
+    m[i] = dmaps + i * npix;
+    a[i] = dalms + 2 * i * nalm;
+  }
+  map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data());
+}
+
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
-  WorkBuf wk(c, sht_work_bytes(P));
-  const ShtBufs B = carve(P, wk.base());
-  for (int i = 0; i < nmaps; ++i)
-    synthesis(c, P, B, (const double2*)dalms + i * P.nalm, dmaps + i * P.R.npix);
+  const size_t per = (sht_work_bytes(P) + 255) & ~size_t(255);
+  const int gmax = std::min(nmaps, kMaxGroup);
+  WorkBuf wk(c, per * (size_t)std::max(gmax, 1));
+  ShtBufs B[kMaxGroup];
+  for (int q = 0; q < gmax; ++q) B[q] = carve(P, (char*)wk.base() + q * per);
+  for (int g0 = 0; g0 < nmaps; g0 += kMaxGroup) {
+    const int nm = std::min(kMaxGroup, nmaps - g0);
+    const double2* alm[kMaxGroup];
+    double* map[kMaxGroup];
+    for (int q = 0; q < nm; ++q) {
+      alm[q] = (const double2*)dalms + (g0 + q) * P.nalm;
+      map[q] = dmaps + (g0 + q) * P.R.npix;
+    }
+    synthesis_group(c, P, B, nm, alm, map);
+  }
   PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -1412,12 +1557,19 @@ void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, do
 // alms, nalm complex each; niter refinements of both the spin-0 and the
 // spin-2 parts.
 void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
-                     double* dteb) {
+                     double* dteb, bool spin0) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax, true);
   const int64_t npix = P.R.npix, nalm = P.nalm;
-  for (int i = 0; i < nmaps; ++i)
-    map2alm_dev(c, nside, lmax, niter, 1, diqu + 3 * i * npix, dteb + 6 * i * nalm);
+  if (spin0) {
+    std::vector<const double*> m(nmaps);
+    std::vector<double*> a(nmaps);
+    for (int i = 0; i < nmaps; ++i) {
+      m[i] = diqu + 3 * i * npix;
+      a[i] = dteb + 6 * i * nalm;
+    }
+    map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data());
+  }
   WorkBuf wk(c, pol_work_bytes(P));
   const PolBufs B = carve_pol(P, wk.base());
   for (int i = 0; i < nmaps; ++i) {
@@ -1620,8 +1772,22 @@ int psx_master_spectra(psx_ctx* h, int nside, int lmax, int niter, const double*
         PSX_HIP(hipGetLastError());
       }
     }
-    map2alm_pol_dev(c, nside, lmax, niter, 2, dm, da);
-    map2alm_dev(c, nside, lmax, niter, nu, dw, dwa);
+    // spin 0: the two I maps and the distinct masks, in transform groups
+    {
+      const double* m0[6];
+      double* a0[6];
+      int n0 = 0;
+      for (int i = 0; i < 2; ++i, ++n0) {
+        m0[n0] = dm + 3 * i * npix;
+        a0[n0] = da + 6 * i * nalm;
+      }
+      for (int k = 0; k < nu; ++k, ++n0) {
+        m0[n0] = dw + k * npix;
+        a0[n0] = dwa + 2 * k * nalm;
+      }
+      map2alm_ptrs(c, nside, lmax, niter, n0, m0, a0);
+    }
+    map2alm_pol_dev(c, nside, lmax, niter, 2, dm, da, false);
     cross_spectra_dev(c, lmax, 6, da, dc);
     cross_spectra_dev(c, lmax, nu, dwa, dwc);
     std::vector<double> hc(36 * L), hw((size_t)nu * nu * L);

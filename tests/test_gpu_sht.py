@@ -58,12 +58,20 @@ def test_alm2map_vs_host(pkg, ctx, nside, lmax):
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
-def test_batch_of_maps(pkg, ctx):
-    nside, lmax = 16, 47
-    maps = np.stack([_smooth_map(nside, s) for s in range(3)])
-    A = ctx.map2alm(maps, nside, lmax, 3)
-    for i in range(3):
-        assert np.array_equal(A[i], ctx.map2alm(maps[i], nside, lmax, 3)[0])
+@pytest.mark.parametrize("nside,lmax,niter,nmaps", [(16, 47, 3, 3), (16, 47, 3, 6),
+                                                     (1024, 3071, 1, 5)])
+def test_batch_of_maps(pkg, ctx, nside, lmax, niter, nmaps):
+    """Maps go through the Legendre passes in groups of up to four that share
+    the recurrence (6 = 4 + 2, 5 = 4 + 1); each result equals the one-map
+    transform bit for bit, in both directions (nside 1024: two ring-pair
+    chunks per m)."""
+    maps = np.stack([_smooth_map(nside, s) for s in range(nmaps)])
+    A = ctx.map2alm(maps, nside, lmax, niter)
+    for i in range(nmaps):
+        assert np.array_equal(A[i], ctx.map2alm(maps[i], nside, lmax, niter)[0]), i
+    B = ctx.alm2map(A, nside, lmax)
+    for i in range(nmaps):
+        assert np.array_equal(B[i], ctx.alm2map(A[i], nside, lmax)[0]), i
 
 
 def test_roundtrip_nside1024(pkg, ctx):
