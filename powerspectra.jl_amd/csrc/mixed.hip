@@ -104,6 +104,11 @@ struct MixArgs {
   const double* const* spec;   // device table [blk][4]
   const double* const* ratio;  // device table [blk][4]
   int* err;
+  // EPI_MCM: the values kappa Xi go to an anti-diagonal-major stage,
+  // dmat[(doff[S - 2 lmin] + l1 - lo(S)) kpad + k], lo(S) = max(lmin, S - lmax),
+  // and mix_scatter_kernel writes the K outputs from it
+  double* dmat;
+  const int64_t* doff;
 };
 
 __device__ __forceinline__ int ufl(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -523,12 +528,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
         xv[i] = x;
       }
       if constexpr (EPI == EPI_MCM) {
+        // kappa Xi into the stage: for one i, the 4 g-rows of 16 consecutive
+        // columns are 4 whole 128-byte lines
+        const int64_t drow = A.doff[S - 2 * lmin] - max(lmin, S - lmax);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int l1 = x0 + 16 * rg + ri[i];
           const int l2 = S - l1;
-          if (k >= A.ncol || l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax)
-            continue;
+          if (l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax) continue;
           const double L1 = (double)l1, L2 = (double)l2;
           double kk = 1.0;
           if constexpr (FAM != FAM_00) {
@@ -538,9 +545,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
             if constexpr (FAM == FAM_02) kk = spin2 ? 1.0 / (2.0 * sqrt(pi1 * pi2)) : 0.0;
             else kk = spin2 ? 1.0 / (4.0 * pi1 * pi2) : 0.0;
           }
-          double* o = A.out[k];
-          o[(int64_t)(l1 - lmin) + (int64_t)(l2 - lmin) * A.ldo] = (2.0 * L2 + 1.0) * kk * xv[i];
-          o[(int64_t)(l2 - lmin) + (int64_t)(l1 - lmin) * A.ldo] = (2.0 * L1 + 1.0) * kk * xv[i];
+          A.dmat[(drow + l1) * A.kpad + k] = kk * xv[i];
         }
       } else {
         // covariance: rows x columns of this column group through LDS, then
@@ -588,6 +593,87 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
   if constexpr (EPI == EPI_COV) {
     if (bad) atomicOr(A.err, 1);
   }
+}
+
+// ---- stage -> outputs ---------------------------------------------------------
+// One workgroup per (16 x 16 tile of (l1 <= l2) rows and columns, 16
+// spectra): the tile's values (16 consecutive doubles per pair in the stage)
+// go through LDS, then both triangles of each output are written as column
+// segments of 16 rows: M_k[l1, l2] = (2 l2 + 1) v, M_k[l2, l1] = (2 l1 + 1) v.
+// Tiles come in 4 x 4 groups (from a host list), so that neighbouring
+// workgroups write neighbouring segments of both triangles.
+struct ScatterArgs {
+  const double* dmat;
+  const int64_t* doff;
+  const int2* tiles;   // (ta, tb), ta <= tb, in 16-row units
+  double* const* out;
+  int64_t ldo;
+  int lmin, lmax, kpad, ncol;
+};
+
+__global__ void __launch_bounds__(256) mix_scatter_kernel(ScatterArgs A) {
+  __shared__ double buf[16][16][17];   // [k][a][b]
+  __shared__ int64_t srow[31];         // stage row of (l1 = 0) on each anti-diagonal
+  const int2 tt = A.tiles[blockIdx.x];
+  const int kc = (int)blockIdx.y;
+  const int a0 = A.lmin + 16 * tt.x, b0 = A.lmin + 16 * tt.y;
+  const int tid = (int)threadIdx.x;
+  if (tid < 31) {
+    const int S = a0 + b0 + tid;
+    srow[tid] = S <= 2 * A.lmax ? A.doff[S - 2 * A.lmin] - max(A.lmin, S - A.lmax) : 0;
+  }
+  __syncthreads();
+  // load: 8 threads per (a, b) pair, 16 bytes each
+#pragma unroll
+  for (int pass = 0; pass < 8; ++pass) {
+    const int pr = (tid >> 3) + 32 * pass, part = tid & 7;
+    const int al = pr >> 4, bl = pr & 15;
+    const int a = a0 + al, b = b0 + bl;
+    double2 v = make_double2(0.0, 0.0);
+    if (a <= b && b <= A.lmax)
+      v = *(const double2*)(A.dmat + (srow[al + bl] + a) * A.kpad + 16 * kc + 2 * part);
+    buf[2 * part][al][bl] = v.x;
+    buf[2 * part + 1][al][bl] = v.y;
+  }
+  __syncthreads();
+  // direct elements, rows a consecutive; then the mirror, rows b consecutive
+#pragma unroll 4
+  for (int pass = 0; pass < 16; ++pass) {
+    const int idx = tid + 256 * pass;
+    const int kl = idx >> 8, cl = (idx >> 4) & 15, rl = idx & 15;
+    const int k = 16 * kc + kl;
+    if (k >= A.ncol) continue;
+    double* o = A.out[k];
+    {
+      const int a = a0 + rl, b = b0 + cl;
+      if (a <= b && b <= A.lmax)
+        o[(int64_t)(a - A.lmin) + (int64_t)(b - A.lmin) * A.ldo] =
+            (2.0 * (double)b + 1.0) * buf[kl][rl][cl];
+    }
+    {
+      const int a = a0 + cl, b = b0 + rl;
+      if (a <= b && b <= A.lmax)
+        o[(int64_t)(b - A.lmin) + (int64_t)(a - A.lmin) * A.ldo] =
+            (2.0 * (double)a + 1.0) * buf[kl][cl][rl];
+    }
+  }
+}
+
+// The scatter's tile list: (ta <= tb) tiles of an nt x nt grid, in 4 x 4
+// groups along the upper triangle.
+#ifndef PSX_SCATTER_GROUP
+#define PSX_SCATTER_GROUP 4
+#endif
+std::vector<int2> make_scatter_tiles(int nt) {
+  constexpr int G = PSX_SCATTER_GROUP;
+  std::vector<int2> out;
+  out.reserve((size_t)nt * (nt + 1) / 2);
+  for (int sb = 0; sb < nt; sb += G)
+    for (int sa = 0; sa <= sb; sa += G)
+      for (int tb = sb; tb < std::min(nt, sb + G); ++tb)
+        for (int ta = sa; ta < std::min(nt, sa + G); ++ta)
+          if (ta <= tb) out.push_back(make_int2(ta, tb));
+  return out;
 }
 
 // ---- tiles -------------------------------------------------------------------
@@ -714,6 +800,12 @@ struct MixTileKey {
 struct MixTiles {
   MTile* d = nullptr;
   int n = 0;
+  // the output stage's row offsets: doff[S - 2 lmin] = pairs (l1 <= l2) on
+  // the anti-diagonals before S; npairs in all
+  int64_t* doff = nullptr;
+  int64_t npairs = 0;
+  int2* stiles = nullptr;   // the scatter's tile list
+  int nstiles = 0;
 };
 std::map<std::pair<void*, MixTileKey>, MixTiles> g_mix_tiles;
 std::mutex g_mix_mu;
@@ -823,6 +915,19 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
       m.n = (int)h.size();
       m.d = (MTile*)ctx_alloc(c, sizeof(MTile) * std::max<size_t>(1, h.size()));
       PSX_HIP(hipMemcpy(m.d, h.data(), sizeof(MTile) * h.size(), hipMemcpyHostToDevice));
+      std::vector<int64_t> off(2 * (size_t)(rq.lmax - rq.lmin) + 1);
+      int64_t acc = 0;
+      for (int S = 2 * rq.lmin; S <= 2 * rq.lmax; ++S) {
+        off[S - 2 * rq.lmin] = acc;
+        acc += std::max(0, std::min(S / 2, rq.lmax) - std::max(rq.lmin, S - rq.lmax) + 1);
+      }
+      m.npairs = acc;
+      m.doff = (int64_t*)ctx_alloc(c, sizeof(int64_t) * off.size());
+      PSX_HIP(hipMemcpy(m.doff, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+      const std::vector<int2> st = make_scatter_tiles((rq.lmax - rq.lmin + 1 + 15) / 16);
+      m.nstiles = (int)st.size();
+      m.stiles = (int2*)ctx_alloc(c, sizeof(int2) * st.size());
+      PSX_HIP(hipMemcpy(m.stiles, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice));
       it = g_mix_tiles.emplace(key, m).first;
     }
     mt = &it->second;
@@ -835,13 +940,16 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
   const size_t b_tab = mode == MIX_I8 ? al256((size_t)kNdB * 2 * hblocks * kpad * 16)
                                       : al256(sizeof(double) * 2 * (size_t)hblocks * 16 * kpad);
   const size_t b_ptr = al256(sizeof(void*) * (kMaxCols + 2 * 4 * kMaxBlk + kMaxCols));
-  const size_t need = b_cs + b_env + b_tab + b_ptr + 256;
+  // EPI_MCM: the anti-diagonal-major output stage, npairs x kpad doubles
+  const size_t b_stage = rq.epi == EPI_MCM ? al256(sizeof(double) * (size_t)mt->npairs * kpad) : 0;
+  const size_t need = b_cs + b_env + b_tab + b_ptr + 256 + b_stage;
   char* base = (char*)ctx_scratch(c, need);
   double* colscale = (double*)base;
   double* env = (double*)(base + b_cs);
   char* tab = base + b_cs + b_env;
   void** ptrs = (void**)(base + b_cs + b_env + b_tab);
   int* err = (int*)(base + b_cs + b_env + b_tab + b_ptr);
+  double* stage = rq.epi == EPI_MCM ? (double*)(base + b_cs + b_env + b_tab + b_ptr + 256) : nullptr;
 
   // pointer tables: W sources, outputs, spectra, ratios (one copy)
   std::vector<const void*> hp(kMaxCols + 2 * 4 * kMaxBlk + kMaxCols, nullptr);
@@ -898,11 +1006,28 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
   ma.spec = (const double* const*)(ptrs + 2 * kMaxCols);
   ma.ratio = (const double* const*)(ptrs + 2 * kMaxCols + 4 * kMaxBlk);
   ma.err = err;
+  ma.dmat = stage;
+  ma.doff = mt->doff;
   const MixFn fn = mode == MIX_I8 ? pick_mix<MIX_I8>(rq.fam, rq.epi, rq.cov_kind, ncg)
                                   : pick_mix<MIX_F64>(rq.fam, rq.epi, rq.cov_kind, ncg);
   {
     TimedLaunch tl(c, mode == MIX_I8 ? "psx_mix_i8" : "psx_mix_f64");
     hipLaunchKernelGGL(fn, dim3((unsigned)((mt->n + 3) / 4)), dim3(256), 0, c->stream, ma);
+  }
+  if (rq.epi == EPI_MCM) {
+    ScatterArgs sa{};
+    sa.dmat = stage;
+    sa.doff = mt->doff;
+    sa.out = (double* const*)(ptrs + kMaxCols);
+    sa.ldo = rq.ldo;
+    sa.lmin = rq.lmin;
+    sa.lmax = rq.lmax;
+    sa.kpad = kpad;
+    sa.ncol = K;
+    sa.tiles = mt->stiles;
+    TimedLaunch tl(c, "psx_mix_scatter");
+    hipLaunchKernelGGL(mix_scatter_kernel, dim3((unsigned)mt->nstiles, (unsigned)ncg), dim3(256),
+                       0, c->stream, sa);
   }
   if (rq.epi == EPI_COV) {
     int h = 0;
