@@ -240,7 +240,8 @@ def norm_err(A, R) -> float:
 
 # kernels each precision of a batch launches (HIP-event timers of the library)
 BATCH_KERNELS = {0: ["psx_mcm_group", "psx_mcm_00", "psx_mcm_02", "psx_mcm_pp", "psx_mcm_mm"],
-                 1: ["psx_mix_prep", "psx_mix_i8"], 2: ["psx_mix_prep", "psx_mix_f64"]}
+                 1: ["psx_mix_prep", "psx_rowmax", "psx_mix_i8", "psx_mix_scatter"],
+                 2: ["psx_mix_prep", "psx_mix_f64", "psx_mix_scatter"]}
 INT8_PEAK_TOPS = 5000.0      # MI355X dense int8 MFMA
 MIX_PRODUCTS = 27            # int8 digit products per (step, column): s + t >= 5 of 6 x 7 digits
 

@@ -82,6 +82,7 @@ struct KArgs {
   const double* spec[4];
   const double* ratio[4];
   int* err;
+  const int64_t* doff;   // EPI_RMAX
 };
 
 template <typename T>
@@ -109,7 +110,14 @@ __host__ __device__ constexpr int rec_stride(int nf) {
   return nf <= 1 ? 1 : nf <= 2 ? 2 : nf <= 4 ? 4 : nf <= 8 ? 8 : 16;
 }
 
-// EPI: EPI_MCM or EPI_COV; COVK: covariance kind for EPI_COV; WPE: minimum
+// One contracted term: acc += w t, or for EPI_RMAX the running max of |w t|.
+template <int EPI>
+__device__ __forceinline__ double accum(double w, double t, double acc) {
+  if constexpr (EPI == EPI_RMAX) return fmax(acc, fabs(w * t));
+  else return fma(w, t, acc);
+}
+
+// EPI: EPI_MCM, EPI_COV or EPI_RMAX; COVK: covariance kind for EPI_COV; WPE: minimum
 // waves per SIMD the register allocation must allow; TPB: threads per
 // workgroup (64 per tile).
 template <int R, int N00, int N02, int NPE, int NPO, int EPI, int COVK, int WPE, int TPB>
@@ -207,26 +215,27 @@ contract_kernel(KArgs A) {
           Q = fma(E, E, -kap[r]);
         }
 #pragma unroll
-        for (int k = 0; k < N00; ++k) acc[k][r] = fma(sload(rr, k), p, acc[k][r]);
+        for (int k = 0; k < N00; ++k) acc[k][r] = accum<EPI>(sload(rr, k), p, acc[k][r]);
         double pq = 0.0;
         if constexpr (N02 > 0) {
           pq = p * Q;
 #pragma unroll
           for (int k = 0; k < N02; ++k)
-            acc[N00 + k][r] = fma(sload(rr, N00 + k), pq, acc[N00 + k][r]);
+            acc[N00 + k][r] = accum<EPI>(sload(rr, N00 + k), pq, acc[N00 + k][r]);
         }
         if constexpr (NPE > 0) {
           const double pqq = (N02 > 0 ? pq : p * Q) * Q;
 #pragma unroll
           for (int k = 0; k < NPE; ++k)
-            acc[N00 + N02 + k][r] = fma(sload(rr, N00 + N02 + k), pqq, acc[N00 + N02 + k][r]);
+            acc[N00 + N02 + k][r] =
+                accum<EPI>(sload(rr, N00 + N02 + k), pqq, acc[N00 + N02 + k][r]);
         }
         if constexpr (NPO > 0) {
           const double D1 = sig[r] - sload(rr, FT1);
           const double pd = (z1 * zw[wi]) * (D1 * D1);
 #pragma unroll
           for (int k = 0; k < NPO; ++k)
-            acc[NE + k][r] = fma(sload(rr, NE + k), pd, acc[NE + k][r]);
+            acc[NE + k][r] = accum<EPI>(sload(rr, NE + k), pd, acc[NE + k][r]);
         }
       }
       // advance the windows to step c + 1: the slot that held r = R-1 now
@@ -247,6 +256,17 @@ contract_kernel(KArgs A) {
   }
 
   // ---- epilogue ----
+  if constexpr (EPI == EPI_RMAX) {
+    // the row maxima, one per owned pair, anti-diagonal-major
+    const int lmin = A.lmin;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int S = S0 + r, l2 = S - l1;
+      if (l1 < own_lo || l1 > own_hi || l1 < lmin || l1 > l2 || l2 > lmax) continue;
+      A.out[0][A.doff[S - 2 * lmin] + (l1 - max(lmin, S - lmax))] = acc[0][r];
+    }
+    return;
+  }
   // The workgroup's tiles are one row chunk over WG R consecutive
   // anti-diagonals: S0 of wave w = S0 of wave 0 + w R, empty tiles (hi < lo)
   // last (checked on the host).  Each output matrix goes through LDS and
@@ -366,10 +386,12 @@ struct RecArgs {
   int nv, ne, rs;
   int ft, ft1;  // field of T / T1, or -1
   int* err;     // cleared here (covariance calls), or null
+  int raw;      // V = W itself (EPI_RMAX), not (2 l3 + 1) W / (4 pi)
 };
 
-__device__ __forceinline__ double what_at(const double* W, int l3, int LW) {
-  return (l3 >= 0 && l3 < LW) ? (2.0 * l3 + 1.0) * W[l3] * kInv4Pi : 0.0;
+__device__ __forceinline__ double what_at(const double* W, int l3, int LW, int raw) {
+  if (l3 < 0 || l3 >= LW) return 0.0;
+  return raw ? W[l3] : (2.0 * l3 + 1.0) * W[l3] * kInv4Pi;
 }
 
 __global__ void __launch_bounds__(256) record_kernel(RecArgs a) {
@@ -381,9 +403,9 @@ __global__ void __launch_bounds__(256) record_kernel(RecArgs a) {
   for (int f = 0; f < a.rs; ++f) {
     double v = 0.0;
     if (f < a.ne) {
-      v = what_at(a.W[f], l3, a.LW);
+      v = what_at(a.W[f], l3, a.LW, a.raw);
     } else if (f < a.nv) {
-      v = what_at(a.W[f], l3 - 1, a.LW);
+      v = what_at(a.W[f], l3 - 1, a.LW, a.raw);
     } else if (f == a.ft) {
       v = (double)l3 * (double)(l3 + 1);
     } else if (f == a.ft1) {
@@ -716,6 +738,19 @@ Config cfg(const char* name) {
 
 Config pick_config(const ContractRequest& rq) {
   const int* f = rq.nfam;
+  if (rq.epi == EPI_RMAX) {
+    // row maxima of one family against one raw table (the matrix-core
+    // contraction's row exponents)
+    if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0)
+      return cfg<8, 1, 0, 0, 0, EPI_RMAX, 0>("psx_rowmax");
+    if (f[0] == 0 && f[1] == 1 && f[2] == 0 && f[3] == 0)
+      return cfg<8, 0, 1, 0, 0, EPI_RMAX, 0>("psx_rowmax");
+    if (f[0] == 0 && f[1] == 0 && f[2] == 1 && f[3] == 0)
+      return cfg<8, 0, 0, 1, 0, EPI_RMAX, 0>("psx_rowmax");
+    if (f[0] == 0 && f[1] == 0 && f[2] == 0 && f[3] == 1)
+      return cfg<8, 0, 0, 0, 1, EPI_RMAX, 0>("psx_rowmax");
+    throw Error{PSX_ERR_ARGUMENT, "row maxima of one family only"};
+  }
   if (rq.epi == EPI_MCM) {
     if (f[0] == 1 && f[1] == 0 && f[2] == 0 && f[3] == 0) {
       // TT: 8 anti-diagonals per tile up to 4096 rows (5 % faster at lmax 3071),
@@ -855,6 +890,7 @@ static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ra.ft = rl.ft;
   ra.ft1 = rl.ft1;
   ra.err = rq.epi == EPI_COV ? err : nullptr;
+  ra.raw = rq.epi == EPI_RMAX ? 1 : 0;
   {
     dim3 grid((unsigned)((g.wh_len + 255) / 256));
     if (s == c->stream) {
@@ -878,7 +914,8 @@ static int* run_contraction_checked(Ctx* c, const ContractRequest& rq, hipStream
   ka.IG = c->tables.dIG;
   ka.rec = rec;
   ka.ldo = rq.ldo;
-  if (rq.epi == EPI_MCM) {
+  ka.doff = rq.doff;
+  if (rq.epi == EPI_MCM || rq.epi == EPI_RMAX) {
     for (int k = 0; k < nv; ++k) ka.out[k] = rq.dout[k];
   } else {
     ka.out[0] = rq.dout[0];

@@ -22,8 +22,10 @@
 //     scale 2^54 for all l3 (7 digits: a spectrum far below the envelope
 //     at some l3 keeps its relative precision there);
 //   * A' takes one exponent per row over its whole family: the exact maximum
-//     of |A'| over the row (a first pass over c), scaled to < 2^46 (6
-//     digits, split on the fly);
+//     of |A'| over the row, scaled to < 2^46 (6 digits, split on the fly).
+//     The maxima come from the FP64 contraction kernel run as a row-maximum
+//     pass (contract.hip, EPI_RMAX: the same family product times env, few
+//     ulp apart from this kernel's; the split tolerates |x| < 2^47);
 //   * the 27 digit products with s + t >= 5 accumulate in int32 (exact:
 //     below 2^30 per diagonal at lmax 8191) over the whole family, seven
 //     diagonals per output;
@@ -109,6 +111,9 @@ struct MixArgs {
   // and mix_scatter_kernel writes the K outputs from it
   double* dmat;
   const int64_t* doff;
+  // MIX_I8: max over its family of |A'| per pair (l1 <= l2), anti-diagonal-major
+  // like the stage (the row-maximum pass of the FP64 contraction kernel)
+  const double* rowmax;
 };
 
 __device__ __forceinline__ int ufl(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -343,41 +348,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
     }
   };
 
-  // ---- pass 1: exact maximum of |A'| per row (MIX_I8) ----
+  // ---- row scales (MIX_I8): the exact maximum of |A'| over each row's
+  // family, from the row-maximum pass; rows outside l1 <= l2 in [lmin, lmax]
+  // are computed but never written, their scale does not matter ----
   double rscale[RG];
   int rexp[RG];
+  Fetch nf{};
 #pragma unroll
   for (int rg = 0; rg < RG; ++rg) {
     rscale[rg] = 1.0;
     rexp[rg] = 0;
-  }
-  Fetch nf{};
-  if constexpr (MODE == MIX_I8) {
-    if (nsteps > 0) nf = fetch(hb0);
-    double rmax[RG];
-#pragma unroll
-    for (int rg = 0; rg < RG; ++rg) rmax[rg] = 0.0;
-    for (int st = 0; st < nsteps; ++st) {
-      const int hb = hb0 + 64 * st;
-      commit(nf);
-      if (st + 1 < nsteps) nf = fetch(hb + 64);
-#pragma unroll
-      for (int rg = 0; rg < RG; ++rg) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int q = 16 * g + i;
-          const double a = fam_value<FAM>(su1[16 * rg + j + q], su2[16 * (RG - 1 - rg) + 15 - j + q],
-                                          sv[q], sig[rg], kap[rg], 2 * (hb + q) + p);
-          rmax[rg] = fmax(rmax[rg], fabs(a));
-        }
-      }
-      unstage();
-    }
-#pragma unroll
-    for (int rg = 0; rg < RG; ++rg) {
-      double m = rmax[rg];
-      m = fmax(m, __shfl_xor(m, 16));
-      m = fmax(m, __shfl_xor(m, 32));
+    if constexpr (MODE == MIX_I8) {
+      const int l1 = x0 + 16 * rg + j, l2 = S - l1;
+      double m = 0.0;
+      if (l1 >= A.lmin && l1 <= l2 && l2 <= A.lmax)
+        m = A.rowmax[A.doff[S - 2 * A.lmin] + (l1 - max(A.lmin, S - A.lmax))];
       int e = 0;
       (void)frexp(m, &e);
       rexp[rg] = m > 0.0 ? e : 0;
@@ -942,14 +927,28 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
   const size_t b_ptr = al256(sizeof(void*) * (kMaxCols + 2 * 4 * kMaxBlk + kMaxCols));
   // EPI_MCM: the anti-diagonal-major output stage, npairs x kpad doubles
   const size_t b_stage = rq.epi == EPI_MCM ? al256(sizeof(double) * (size_t)mt->npairs * kpad) : 0;
-  const size_t need = b_cs + b_env + b_tab + b_ptr + 256 + b_stage;
+  // MIX_I8: the row maxima (npairs doubles) and the row-maximum pass's records
+  ContractRequest rr;
+  rr.lmin = rq.lmin;
+  rr.lmax = rq.lmax;
+  rr.LW = LW;
+  rr.nfam[rq.fam] = 1;
+  rr.epi = EPI_RMAX;
+  rr.ldo = n;
+  rr.doff = mt->doff;
+  const size_t b_rmax = mode == MIX_I8 ? al256(sizeof(double) * (size_t)mt->npairs) : 0;
+  const size_t b_rrec = mode == MIX_I8 ? al256(contraction_scratch_bytes(rr)) : 0;
+  const size_t need = b_cs + b_env + b_tab + b_ptr + 256 + b_stage + b_rmax + b_rrec;
   char* base = (char*)ctx_scratch(c, need);
   double* colscale = (double*)base;
   double* env = (double*)(base + b_cs);
   char* tab = base + b_cs + b_env;
   void** ptrs = (void**)(base + b_cs + b_env + b_tab);
   int* err = (int*)(base + b_cs + b_env + b_tab + b_ptr);
-  double* stage = rq.epi == EPI_MCM ? (double*)(base + b_cs + b_env + b_tab + b_ptr + 256) : nullptr;
+  char* tail = base + b_cs + b_env + b_tab + b_ptr + 256;
+  double* stage = rq.epi == EPI_MCM ? (double*)tail : nullptr;
+  double* rowmax = mode == MIX_I8 ? (double*)(tail + b_stage) : nullptr;
+  void* rrec = mode == MIX_I8 ? (void*)(tail + b_stage + b_rmax) : nullptr;
 
   // pointer tables: W sources, outputs, spectra, ratios (one copy)
   std::vector<const void*> hp(kMaxCols + 2 * 4 * kMaxBlk + kMaxCols, nullptr);
@@ -983,6 +982,13 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
                        mode);
   }
 
+  if (mode == MIX_I8) {
+    // row maxima of |A'| = |family product| env(l3) over l3 <= LW - 1
+    rr.dW[0] = env;
+    rr.dout[0] = rowmax;
+    (void)run_contraction_on(c, rr, c->stream, rrec, b_rrec);
+  }
+
   MixArgs ma{};
   ma.tiles = mt->d;
   ma.ntiles = mt->n;
@@ -1008,6 +1014,7 @@ void run_mixed(Ctx* c, const MixRequest& rq) {
   ma.err = err;
   ma.dmat = stage;
   ma.doff = mt->doff;
+  ma.rowmax = rowmax;
   const MixFn fn = mode == MIX_I8 ? pick_mix<MIX_I8>(rq.fam, rq.epi, rq.cov_kind, ncg)
                                   : pick_mix<MIX_F64>(rq.fam, rq.epi, rq.cov_kind, ncg);
   {

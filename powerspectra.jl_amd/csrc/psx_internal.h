@@ -207,7 +207,9 @@ enum Family : int {
 
 enum EpiMode : int {
   EPI_MCM = 0,   // out_k[l1,l2] = (2l2+1) kappa Ξ_k, out_k[l2,l1] = (2l1+1) kappa Ξ_k
-  EPI_COV = 1    // out[l1,l2] = out[l2,l1] = sum_k coef_k(l1,l2) Ξ_k
+  EPI_COV = 1,   // out[l1,l2] = out[l2,l1] = sum_k coef_k(l1,l2) Ξ_k
+  EPI_RMAX = 2   // one V table taken raw (no (2 l3 + 1) / 4 pi): out[doff[S - 2 lmin] + l1 -
+                 // max(lmin, S - lmax)] = max over the family of |term x V(l3)|
 };
 
 struct ContractRequest {
@@ -225,6 +227,8 @@ struct ContractRequest {
   const double* dspec[4];
   const double* dratio[4];
   int precision = PSX_FP64;
+  // EPI_RMAX: the anti-diagonal row offsets of the output table
+  const int64_t* doff = nullptr;
 };
 
 void run_contraction(Ctx* c, const ContractRequest& rq);
