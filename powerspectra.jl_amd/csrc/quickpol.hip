@@ -68,6 +68,15 @@ __device__ __forceinline__ void qp_adv(QpState& s, double au, double rj, double 
   s.A = au * rv;
 }
 
+// The same step with Y = kje - tj c given through its uniform part kje = kj e.
+__device__ __forceinline__ void qp_step(QpState& s, double au, double rj, double rv, double r,
+                                        double jp, double Y) {
+  const double fn = -fma(Y, s.f, jp * s.A * s.p) * (rj * r);
+  s.p = s.f;
+  s.f = fn;
+  s.A = au * rv;
+}
+
 // The step out of j = 0 (nmin = 0: l == l'', M1 == 0), where the recurrence
 // divided by j gives A(1) f(1) + (M3 - M2) f(0) = 0.
 __device__ __forceinline__ void qp_adv0(QpState& s, double au, double rv, double e) {
@@ -93,7 +102,9 @@ __device__ __forceinline__ int wave_min(int v) {
 }
 
 __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
-  __shared__ double tab[5][kQpSteps];   // au_a, rj_a, au_b, rj_b, W per step
+  // per step: au_a, rj_a, au_b, rj_b, W, then the other uniform factors
+  // j + 1, 2j + 1, kj e_a, kj e_b, (j + 1)^2
+  __shared__ double tab[10][kQpSteps];
   const int d = (int)blockIdx.x - A.bl;                                   // l - l''
   const int lpp0 = A.lmax - (int)blockIdx.y * 64;                         // lane 0's l''
   const int lane = (int)threadIdx.x;
@@ -176,43 +187,47 @@ __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
         rb = 1.0 / ((double)j * ab);
       }
       const double w = j <= jend && j < A.LW ? A.W[j] : 0.0;
+      const double jp = (double)j + 1.0, tj = 2.0 * (double)j + 1.0;
+      const double kj = tj * (double)j * jp;
       __syncthreads();
       tab[0][lane] = aa;
       tab[1][lane] = ra;
       tab[2][lane] = ab;
       tab[3][lane] = rb;
       tab[4][lane] = w;
+      tab[5][lane] = jp;
+      tab[6][lane] = tj;
+      tab[7][lane] = kj * ea;
+      tab[8][lane] = kj * eb;
+      tab[9][lane] = jp2;
       __syncthreads();
     }
     const int tend = min(kQpSteps, jend - jb + 1);
     const int tm = max(0, min(tend, nmin_end - jb));   // steps with j < nmin_end
     for (int t = 0; t < tm; ++t) {
-      const int j = jb + t;
-      const double jp = j + 1.0, tj = 2.0 * j + 1.0;
+      const double jp = tab[5][t], tj = tab[6][t];
       fa.S = fma(tj * fa.f, fa.f, fa.S);
       fb.S = fma(tj * fb.f, fb.f, fb.S);
       D = fma(tab[4][t] * fa.f, fb.f, D);      // W is zero beyond LW
-      const double v = s2 - jp * jp;            // (l+l''+1)^2 - (j+1)^2 > 0
+      const double v = s2 - tab[9][t];          // (l+l''+1)^2 - (j+1)^2 > 0
       const double r = rsqrt(v);
       const double rv = v * r;
-      const double kj = tj * (double)j * jp;
-      qp_adv(fa, tab[0][t], tab[1][t], rv, r, jp, tj, kj, ca, ea);
-      qp_adv(fb, tab[2][t], tab[3][t], rv, r, jp, tj, kj, cb, eb);
+      qp_step(fa, tab[0][t], tab[1][t], rv, r, jp, fma(-tj, ca, tab[7][t]));
+      qp_step(fb, tab[2][t], tab[3][t], rv, r, jp, fma(-tj, cb, tab[8][t]));
     }
     for (int t = tm; t < tend; ++t) {          // the last steps, lane by lane
       const int j = jb + t;
       if (j > nmax) break;
-      const double jp = j + 1.0, tj = 2.0 * j + 1.0;
+      const double jp = tab[5][t], tj = tab[6][t];
       fa.S = fma(tj * fa.f, fa.f, fa.S);
       fb.S = fma(tj * fb.f, fb.f, fb.S);
       D = fma(tab[4][t] * fa.f, fb.f, D);
       if (j == nmax) break;
-      const double v = s2 - jp * jp;
+      const double v = s2 - tab[9][t];
       const double r = rsqrt(v);
       const double rv = v * r;
-      const double kj = tj * (double)j * jp;
-      qp_adv(fa, tab[0][t], tab[1][t], rv, r, jp, tj, kj, ca, ea);
-      qp_adv(fb, tab[2][t], tab[3][t], rv, r, jp, tj, kj, cb, eb);
+      qp_step(fa, tab[0][t], tab[1][t], rv, r, jp, fma(-tj, ca, tab[7][t]));
+      qp_step(fb, tab[2][t], tab[3][t], rv, r, jp, fma(-tj, cb, tab[8][t]));
     }
     qp_rescale(fa, D);
     qp_rescale(fb, D);
