@@ -140,124 +140,189 @@ __device__ __forceinline__ void trig_at(const double* ctab, int n, int t, double
   s = ctab[ts];
 }
 
+// Polar rings have n = 4i pixels, so the forward DFT splits in radix 4: the
+// four real sub-sequences x_r[j] = x[4j + r] (j < i) have length-i DFTs
+//   Y_r[p] = x_r[0] + (-1)^p x_r[i/2] (i even)
+//            + sum_{j=1}^{ceil(i/2)-1} [s_rj cos(2 pi p j/i) - i d_rj sin(2 pi p j/i)]
+// (s_rj, d_rj = x_r[j] +- x_r[i-j]); with Z_r = e^{-2 pi i r p/n} Y_r[p] and
+// Y_r[i-p] = conj(Y_r[p]) (real input), one p gives four outputs:
+//   X[p] = sum_r Z_r,                 X[i+p] = sum_r (-i)^r Z_r,
+//   X[i-p] = sum_r (-i)^r conj(Z_r),  X[2i-p] = sum_r (-1)^r conj(Z_r),
+// which cover X[0 .. 2i] for p = 0 .. floor(i/2) (each written once).  One
+// thread per p carries the four sub-sums with one (cos, sin) rotation, reset
+// from the LDS cosine table every kRotAnchor steps; the sub-sequence values
+// are the same for every thread at a step (LDS broadcast).
 template <int NMAX>
 __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
                                                  double2* __restrict__ X, int ns) {
   __shared__ double ctab[NMAX];
-  __shared__ double sd[NMAX];                       // s_j at [j], d_j at [NMAX/2 + j]
+  __shared__ double sv[4][NMAX / 8], dv[4][NMAX / 8];   // s_rj, d_rj at [r][j], j >= 1
   const int pr = blockIdx.x;                        // polar ring ordinal
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
-  const int n = R.n[k], h = n / 2;
+  const int n = R.n[k], i = n / 4;
+  const int H = (i + 1) / 2 - 1;                    // folded pairs j = 1 .. H
   const double* x = map + R.first[k];
   for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
-  for (int j = threadIdx.x; j < h; j += blockDim.x) {
-    const double a = j > 0 ? x[j] : 0.0, b = j > 0 ? x[n - j] : 0.0;
-    sd[j] = a + b;
-    sd[NMAX / 2 + j] = a - b;
+  for (int e = threadIdx.x; e < 4 * H; e += blockDim.x) {
+    const int r = e & 3, j = (e >> 2) + 1;
+    const double a = x[4 * j + r], b = x[4 * (i - j) + r];
+    sv[r][j] = a + b;
+    dv[r][j] = a - b;
   }
-  const double x0 = x[0], xh = x[h];
   __syncthreads();
-  // frequencies q and q + half2 per thread, half2 = ceil((h + 1) / 2)
-  const int half2 = (h + 2) / 2;
-  for (int qa = threadIdx.x; qa < half2; qa += blockDim.x) {
-    const int qb = qa + half2;                      // <= h + 1; used only if <= h
-    const bool hasb = qb <= h;
-    double ra = (qa & 1) ? x0 - xh : x0 + xh, ia = 0.0;
-    double rb = (qb & 1) ? x0 - xh : x0 + xh, ib = 0.0;
-    double cda, sda, cdb, sdb;                      // one-step rotations
-    trig_at(ctab, n, qa % n, cda, sda);
-    trig_at(ctab, n, hasb ? qb % n : 0, cdb, sdb);
-    double ca = 1.0, sa = 0.0, cb = 1.0, sb = 0.0;  // cos, sin at j = 0
-    int ta = 0, tb = 0;                             // q j mod n at the last anchor
-    for (int j0 = 1; j0 < h; j0 += kRotAnchor) {
-      // anchor at j0 from the table
-      ta += qa * (j0 == 1 ? 1 : kRotAnchor);
-      ta %= n;
-      tb += (hasb ? qb : 0) * (j0 == 1 ? 1 : kRotAnchor);
-      tb %= n;
-      trig_at(ctab, n, ta, ca, sa);
-      trig_at(ctab, n, tb, cb, sb);
-      const int jend = min(h, j0 + kRotAnchor);
+  const bool even = (i & 1) == 0;
+  double x0[4], xm[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    x0[r] = x[r];
+    xm[r] = even ? x[4 * (i / 2) + r] : 0.0;
+  }
+  double2* out = X + R.xoff[k];
+  const int P = i / 2;
+  for (int p = threadIdx.x; p <= P; p += blockDim.x) {
+    double ar[4], ai[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ar[r] = x0[r] + ((p & 1) ? -xm[r] : xm[r]);
+      ai[r] = 0.0;
+    }
+    double cd, sd;                                  // one-step rotation, angle 2 pi p / i
+    trig_at(ctab, n, (4 * p) % n, cd, sd);
+    double c = 1.0, sn = 0.0;
+    int ta = 0;                                     // 4 (p j mod i) at the last anchor
+    for (int j0 = 1; j0 <= H; j0 += kRotAnchor) {
+      ta = (int)((4LL * p * j0) % n);
+      trig_at(ctab, n, ta, c, sn);
+      const int jend = min(H + 1, j0 + kRotAnchor);
       for (int j = j0; j < jend; ++j) {
-        const double sj = sd[j], dj = sd[NMAX / 2 + j];
-        ra = fma(sj, ca, ra);
-        ia = fma(-dj, sa, ia);
-        rb = fma(sj, cb, rb);
-        ib = fma(-dj, sb, ib);
-        const double ca2 = fma(ca, cda, -sa * sda), sa2 = fma(sa, cda, ca * sda);
-        const double cb2 = fma(cb, cdb, -sb * sdb), sb2 = fma(sb, cdb, cb * sdb);
-        ca = ca2;
-        sa = sa2;
-        cb = cb2;
-        sb = sb2;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          ar[r] = fma(sv[r][j], c, ar[r]);
+          ai[r] = fma(-dv[r][j], sn, ai[r]);
+        }
+        const double c2 = fma(c, cd, -sn * sd), s2 = fma(sn, cd, c * sd);
+        c = c2;
+        sn = s2;
       }
     }
-    X[R.xoff[k] + qa] = make_double2(ra, ia);
-    if (hasb) X[R.xoff[k] + qb] = make_double2(rb, ib);
+    // Z_r = e^{-2 pi i r p / n} Y_r
+    double zx[4], zy[4];
+    zx[0] = ar[0];
+    zy[0] = ai[0];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+      double cr, sr;
+      trig_at(ctab, n, (r * p) % n, cr, sr);
+      zx[r] = fma(cr, ar[r], sr * ai[r]);
+      zy[r] = fma(cr, ai[r], -sr * ar[r]);
+    }
+    const double2 Xp = make_double2(zx[0] + zx[1] + zx[2] + zx[3], zy[0] + zy[1] + zy[2] + zy[3]);
+    const double2 Xip = make_double2(zx[0] + zy[1] - zx[2] - zy[3], zy[0] - zx[1] - zy[2] + zx[3]);
+    const double2 Xim = make_double2(zx[0] - zy[1] - zx[2] + zy[3], -zy[0] - zx[1] + zy[2] + zx[3]);
+    const double2 X2m = make_double2(zx[0] - zx[1] + zx[2] - zx[3], -zy[0] + zy[1] - zy[2] + zy[3]);
+    if (p == 0) {
+      out[0] = Xp;
+      out[i] = Xip;
+      out[2 * i] = X2m;
+    } else if (2 * p < i) {
+      out[p] = Xp;
+      out[i - p] = Xim;
+      out[i + p] = Xip;
+      out[2 * i - p] = X2m;
+    } else {                                        // p = i / 2
+      out[p] = Xp;
+      out[i + p] = Xip;
+    }
   }
 }
 
-// Inverse DFT of one polar ring from its Hermitian half Y[0 .. n/2]:
-// f_j = Y_0 + (-1)^j Y_{n/2} + 2 sum_{0<q<n/2} Re(Y_q e^{2 pi i q j/n}).
-// Outputs j and n - j share the cosine sum A and see the sine sum B with
-// opposite signs (f_j = base + A - B, f_{n-j} = base + A + B), so each step
-// serves two pixels; each thread takes two such j, with (cos, sin) rotated
-// along q as in the forward transform.
+// Inverse DFT of one polar ring (n = 4i pixels) from its Hermitian half
+// Y[0 .. n/2] (Y_0, Y_{n/2} taken as real): f_j = sum_{q<n} Yh_q e^{2 pi i q j/n}
+// with Yh_q = Y_q (q <= n/2), conj(Y_{n-q}) above.  Radix 4 on the output,
+// j = 4m + r:
+//   W_r[p] = e^{2 pi i p r/n} sum_{k<4} i^{kr} Yh_{p+ki},
+//   f_{4m+r} = sum_{p<i} W_r[p] e^{2 pi i p m/i}
+//            = W_r[0] + (-1)^m W_r[i/2] (i even)
+//              + 2 sum_{p=1}^{ceil(i/2)-1} Re(W_r[p] e^{2 pi i p m/i})
+// (W_r is Hermitian because f is real).  One thread per m carries the four
+// sums with one rotation and writes pixels 4m + r and 4(i - m) + r, whose
+// cosine sums agree and sine sums are opposite.
 template <int NMAX>
 __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
                                                   double* __restrict__ map, int ns) {
   __shared__ double ctab[NMAX];
-  __shared__ double2 ys[NMAX / 2 + 1];              // 2 Y_q
+  __shared__ double2 wv[4][NMAX / 8 + 1];           // 2 W_r[p], p = 0 .. i/2
   const int pr = blockIdx.x;
   const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
-  const int n = R.n[k], h = n / 2;
+  const int n = R.n[k], h = n / 2, i = n / 4;
   for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  __syncthreads();
   const double2* Y = X + R.xoff[k];
-  for (int q = threadIdx.x; q <= h; q += blockDim.x) {
-    const double2 y = Y[q];
-    ys[q] = make_double2(2.0 * y.x, 2.0 * y.y);
+  const int P = i / 2;
+  for (int p = threadIdx.x; p <= P; p += blockDim.x) {
+    double2 y[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int q = p + kk * i;
+      if (q <= h) {
+        y[kk] = Y[q];
+        if (q == 0 || q == h) y[kk].y = 0.0;
+      } else {
+        y[kk] = Y[n - q];
+        y[kk].y = -y[kk].y;
+      }
+    }
+    // sum_k i^{kr} y_k for r = 0 .. 3 (i (a + ib) = -b + ia)
+    double bx[4], by[4];
+    bx[0] = y[0].x + y[1].x + y[2].x + y[3].x;
+    by[0] = y[0].y + y[1].y + y[2].y + y[3].y;
+    bx[1] = y[0].x - y[1].y - y[2].x + y[3].y;
+    by[1] = y[0].y + y[1].x - y[2].y - y[3].x;
+    bx[2] = y[0].x - y[1].x + y[2].x - y[3].x;
+    by[2] = y[0].y - y[1].y + y[2].y - y[3].y;
+    bx[3] = y[0].x + y[1].y - y[2].x - y[3].y;
+    by[3] = y[0].y - y[1].x - y[2].y + y[3].x;
+    wv[0][p] = make_double2(2.0 * bx[0], 2.0 * by[0]);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+      double cr, sr;                                // e^{+2 pi i p r / n}
+      trig_at(ctab, n, (r * p) % n, cr, sr);
+      wv[r][p] = make_double2(2.0 * fma(cr, bx[r], -sr * by[r]), 2.0 * fma(cr, by[r], sr * bx[r]));
+    }
   }
   __syncthreads();
   double* f = map + R.first[k];
-  const int half2 = (h + 2) / 2;
-  for (int ja = threadIdx.x; ja < half2; ja += blockDim.x) {
-    const int jb = ja + half2;
-    const bool hasb = jb <= h;
-    double Aa = 0.0, Ba = 0.0, Ab = 0.0, Bb = 0.0;
-    double cda, sda, cdb, sdb;
-    trig_at(ctab, n, ja % n, cda, sda);
-    trig_at(ctab, n, hasb ? jb % n : 0, cdb, sdb);
-    double ca = 1.0, sa = 0.0, cb = 1.0, sb = 0.0;
-    int ta = 0, tb = 0;
-    for (int q0 = 1; q0 < h; q0 += kRotAnchor) {
-      ta += ja * (q0 == 1 ? 1 : kRotAnchor);
-      ta %= n;
-      tb += (hasb ? jb : 0) * (q0 == 1 ? 1 : kRotAnchor);
-      tb %= n;
-      trig_at(ctab, n, ta, ca, sa);
-      trig_at(ctab, n, tb, cb, sb);
-      const int qend = min(h, q0 + kRotAnchor);
-      for (int q = q0; q < qend; ++q) {
-        const double2 y = ys[q];
-        Aa = fma(y.x, ca, Aa);
-        Ba = fma(y.y, sa, Ba);
-        Ab = fma(y.x, cb, Ab);
-        Bb = fma(y.y, sb, Bb);
-        const double ca2 = fma(ca, cda, -sa * sda), sa2 = fma(sa, cda, ca * sda);
-        const double cb2 = fma(cb, cdb, -sb * sdb), sb2 = fma(sb, cdb, cb * sdb);
-        ca = ca2;
-        sa = sa2;
-        cb = cb2;
-        sb = sb2;
+  const bool even = (i & 1) == 0;
+  const int H = (i + 1) / 2 - 1;                    // p = 1 .. H
+  for (int m = threadIdx.x; m <= P; m += blockDim.x) {
+    double A[4], B[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      A[r] = 0.5 * (wv[r][0].x + (even ? ((m & 1) ? -wv[r][P].x : wv[r][P].x) : 0.0));
+      B[r] = 0.0;
+    }
+    double cd, sd;                                  // one-step rotation, angle 2 pi m / i
+    trig_at(ctab, n, (4 * m) % n, cd, sd);
+    double c = 1.0, sn = 0.0;
+    for (int p0 = 1; p0 <= H; p0 += kRotAnchor) {
+      trig_at(ctab, n, (int)((4LL * m * p0) % n), c, sn);
+      const int pend = min(H + 1, p0 + kRotAnchor);
+      for (int p = p0; p < pend; ++p) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double2 w = wv[r][p];
+          A[r] = fma(w.x, c, A[r]);
+          B[r] = fma(w.y, sn, B[r]);
+        }
+        const double c2 = fma(c, cd, -sn * sd), s2 = fma(sn, cd, c * sd);
+        c = c2;
+        sn = s2;
       }
     }
-    const double ya = 0.5 * (ys[0].x + ((ja & 1) ? -ys[h].x : ys[h].x));
-    f[ja] = ya + Aa - Ba;
-    if (ja > 0 && ja < h) f[n - ja] = ya + Aa + Ba;
-    if (hasb) {
-      const double yb = 0.5 * (ys[0].x + ((jb & 1) ? -ys[h].x : ys[h].x));
-      f[jb] = yb + Ab - Bb;
-      if (jb > 0 && jb < h) f[n - jb] = yb + Ab + Bb;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      f[4 * m + r] = A[r] - B[r];
+      if (m > 0 && 2 * m < i) f[4 * (i - m) + r] = A[r] + B[r];
     }
   }
 }
