@@ -645,37 +645,65 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM>
   }
 }
 
-// Hermitian half spectrum of each ring from G: Y[q] collects G_m e^{i m phi0}
-// for m = q mod n and conj(G_m e^{i m phi0}) for m = -q mod n (m >= 1); m = 0
-// contributes Re(G_0).
-__global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, const double2* __restrict__ G,
+// Hermitian half spectrum of each ring from G (m-major: G[m * nr + k]).
+// Y[q] collects G_m e^{i m phi0} for m = q mod n and conj(G_m e^{i m phi0})
+// for m = -q mod n (m >= 1); m = 0 contributes Re(G_0).  Every HEALPix ring
+// has phi0 = pi/n (first pixel shifted by half a pixel) or phi0 = 0, so
+// e^{i n phi0} = s = -1 or +1 exactly and
+//   Y[q] = e^{i q phi0} (A + conj(B)),  A = sum_{j>=0} s^j G_{q+jn},
+//                                      B = sum_{j>=1} s^j G_{jn-q}:
+// one small-angle sincos per output.  A workgroup covers kFoldRings adjacent
+// rings (lane & 7) and kFoldQ values of q; where the rings share n, a wave
+// reads 8 rows of 8 adjacent rings (128 contiguous bytes each).  Workgroup b
+// runs on XCD b mod 8: the 1-D grid gives all q-blocks of one ring group the
+// same XCD, so the rows of G the group reads are fetched into one L2.
+constexpr int kFoldRings = 8, kFoldQ = 32;
+
+__global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, int nqb,
+                                                      const double2* __restrict__ G,
                                                       double2* __restrict__ X) {
-  const int k = blockIdx.y;
+  const int b = blockIdx.x;
+  const int u = b >> 3;
+  const int grp = (b & 7) + 8 * (u / nqb);
+  const int k = grp * kFoldRings + (threadIdx.x & (kFoldRings - 1));
+  const int q = (u % nqb) * kFoldQ + (threadIdx.x / kFoldRings);
+  if (k >= R.nr) return;
   const int n = R.n[k];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q > n / 2) return;
   const double phi0 = R.phi0[k];
-  double yr = 0.0, yi = 0.0;
-  for (int m = q; m <= R.lmax; m += n) {
-    const double2 g = G[(int64_t)m * R.nr + k];
-    if (m == 0) {
-      yr += g.x;
-      continue;
-    }
-    double s, c;
-    sincos((double)m * phi0, &s, &c);
-    yr += g.x * c - g.y * s;
-    yi += g.x * s + g.y * c;
+  const double s = phi0 != 0.0 ? -1.0 : 1.0;
+  const int64_t nr = R.nr;
+  double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, sg = 1.0;
+  int m = q;
+  if (q == 0) {
+    ar = G[k].x;
+    m = n;
+    sg = s;
   }
-  for (int m = (q == 0 ? n : n - q); m <= R.lmax; m += n) {
-    const double2 g = G[(int64_t)m * R.nr + k];
-    double s, c;
-    sincos((double)m * phi0, &s, &c);
-    // conj(g e^{i m phi0})
-    yr += g.x * c - g.y * s;
-    yi -= g.x * s + g.y * c;
+  for (; m <= R.lmax; m += n, sg *= s) {
+    const double2 g = G[(int64_t)m * nr + k];
+    ar = fma(sg, g.x, ar);
+    ai = fma(sg, g.y, ai);
   }
-  X[R.xoff[k] + q] = make_double2(yr, yi);
+  sg = s;
+  for (m = n - q; m <= R.lmax; m += n, sg *= s) {
+    const double2 g = G[(int64_t)m * nr + k];
+    br = fma(sg, g.x, br);
+    bi = fma(sg, g.y, bi);
+  }
+  double sn, c;
+  sincos((double)q * phi0, &sn, &c);
+  const double tr = ar + br, ti = ai - bi;
+  X[R.xoff[k] + q] = make_double2(c * tr - sn * ti, sn * tr + c * ti);
+}
+
+void launch_fold_synthesis(const Rings& R, hipStream_t s, RingArgs ra, const double2* G,
+                           double2* X) {
+  const int nqb = (2 * R.nside + 1 + kFoldQ - 1) / kFoldQ;     // n/2 <= 2 nside
+  const int ngrp = (R.nr + kFoldRings - 1) / kFoldRings;
+  const int64_t nb = (int64_t)8 * nqb * ((ngrp + 7) / 8);
+  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)nb), dim3(256), 0, s, ra, nqb, G, X);
+  PSX_HIP(hipGetLastError());
 }
 
 // ---- spin 2 ---------------------------------------------------------------
@@ -1400,9 +1428,7 @@ void synthesis_group(Ctx* c, ShtPlan& P, const ShtBufs* B, int nm, const double2
     default: launch_synthesis<4>(P, s, B, alms); break;
   }
   for (int q = 0; q < nm; ++q) {
-    hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
-                       dim3(256), 0, s, P.ra(), B[q].G, B[q].X);
-    PSX_HIP(hipGetLastError());
+    launch_fold_synthesis(R, s, P.ra(), B[q].G, B[q].X);
     {
       void* in[1] = {(void*)(B[q].X + R.xoff[ns - 1])};
       void* out[1] = {(void*)(maps[q] + R.first[ns - 1])};
@@ -1479,9 +1505,7 @@ void ring_unfold(Ctx* c, ShtPlan& P, const double2* G, double2* X, double* dmap)
   const Rings& R = P.R;
   const int ns = R.nside;
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)((2 * ns + 1 + 255) / 256), (unsigned)R.nr),
-                     dim3(256), 0, s, P.ra(), G, X);
-  PSX_HIP(hipGetLastError());
+  launch_fold_synthesis(R, s, P.ra(), G, X);
   {
     void* in[1] = {(void*)(X + R.xoff[ns - 1])};
     void* out[1] = {(void*)(dmap + R.first[ns - 1])};
