@@ -440,28 +440,53 @@ struct MapPtrs {
 // thread go through the round's 16 rows of 256 doubles (32 KB)
 __host__ __device__ constexpr int an_lchunk(int nm) { return nm == 1 ? 8 : nm == 2 ? 4 : 2; }
 
+// DPP move of a double inside a row of 16 lanes (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 16 lanes of a row: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_ror 4, row_ror 8.  Every lane ends with the row's sum (lanes of quads
+// 0, 2 and of quads 1, 3 may differ in the last bit: the caller takes lane 0).
+__device__ __forceinline__ double row16_sum(double s) {
+  s += dpp_f64<0xB1>(s);
+  s += dpp_f64<0x4E>(s);
+  s += dpp_f64<0x124>(s);
+  s += dpp_f64<0x128>(s);
+  return s;
+}
+
 // Analysis partial sums: part_q[chunk][idx(l, m)] = sum over the chunk's pairs
 // of lambda_lm(z_p) F+-_p(m) for each map q of the group.  Each thread carries
 // kPairsPerThread independent recurrences; per round of LC l values the 256
-// partial sums of every (l, map, re/im) are reduced through LDS, in the same
-// order whatever the group size (one map alone gives the same sums).
+// partial sums of every (l, map, re/im) are reduced through LDS (16 lanes per
+// output, each summing a stride-16 column of its row so that a wave's reads
+// are consecutive doubles, then a DPP sum over the 16 lanes), in the same
+// order whatever the group size (one map alone gives the same sums).  The
+// round holding l = m, m + 1 and the last, partial round take the checked
+// path; the rounds between carry no per-step branch.
 template <int NM>
 __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, MapPtrs<NM> M) {
   constexpr int LC = an_lchunk(NM);
   constexpr int NO = 2 * LC * NM;  // outputs per round
-  static_assert(NO <= 16, "one reduction pass per round");
+  static_assert(NO <= 16, "one output per row of 16 lanes");
+  constexpr int P = kPairsPerThread;
   __shared__ double red[NO][256];
   const int m = blockIdx.y;
   const int chunk = blockIdx.x;
   const int lmax = A.R.lmax;
   const int tid = threadIdx.x;
   const double2* ABm = A.AB + alm_idx(lmax, m, m);   // ABm[l - m]
-  double z[kPairsPerThread], l1[kPairsPerThread], l2[kPairsPerThread], wgt[kPairsPerThread];
-  int sc[kPairsPerThread];
-  double2 fp[NM][kPairsPerThread], fm[NM][kPairsPerThread];
+  double z[P], l1[P], l2[P], wgt[P];
+  int sc[P];
+  double2 fp[NM][P], fm[NM][P];
 #pragma unroll
-  for (int r = 0; r < kPairsPerThread; ++r) {
-    const int p = (chunk * kPairsPerThread + r) * 256 + tid;
+  for (int r = 0; r < P; ++r) {
+    const int p = (chunk * P + r) * 256 + tid;
     const bool live = p < A.R.npairs;
     const int pp = live ? p : 0;
     z[r] = A.R.z[pp];
@@ -475,23 +500,32 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, MapPtrs<NM> 
       fm[q][r] = live ? M.in2[q][o] : make_double2(0.0, 0.0);
     }
   }
-  for (int l0 = m; l0 <= lmax; l0 += LC) {
+  const int o = tid >> 4, g = tid & 15;
+  const int oj = o / (2 * NM), oq = (o >> 1) % NM, oc = o & 1;
+  const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
+  double2* outq = M.out[0];
+#pragma unroll
+  for (int q = 1; q < NM; ++q)
+    if (oq == q) outq = M.out[q];
+  outq += (int64_t)chunk * nalm;
+  auto round = [&](const int l0, const bool edge) {
 #pragma unroll
     for (int j = 0; j < LC; ++j) {
       const int l = l0 + j;
       double accr[NM], acci[NM];
 #pragma unroll
       for (int q = 0; q < NM; ++q) accr[q] = acci[q] = 0.0;
-      if (l <= lmax) {
+      if (!edge || l <= lmax) {
         const double2 ab = sload(ABm + (l - m));   // (0, 0) at l = m
         const bool odd = j & 1;                    // l0 - m is a multiple of LC
 #pragma unroll
-        for (int r = 0; r < kPairsPerThread; ++r) {
+        for (int r = 0; r < P; ++r) {
           double lam;
-          if (l == m) {
+          if (edge && l == m) {
             lam = l1[r];
           } else {
-            lam = l == m + 1 ? ab.x * z[r] * l1[r] : ab.x * fma(z[r], l1[r], -ab.y * l2[r]);
+            lam = (edge && l == m + 1) ? ab.x * z[r] * l1[r]
+                                       : ab.x * fma(z[r], l1[r], -ab.y * l2[r]);
             l2[r] = l1[r];
             l1[r] = lam;
           }
@@ -511,31 +545,25 @@ __global__ void __launch_bounds__(256) legendre_analysis(LegArgs A, MapPtrs<NM> 
       }
     }
 #pragma unroll
-    for (int r = 0; r < kPairsPerThread; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
+    for (int r = 0; r < P; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
     __syncthreads();
-    // NO <= 16 outputs, 16 threads each: 16 values per thread, then a
-    // reduction over the 16 lanes of the group
-    {
-      const int o = tid >> 4, g = tid & 15;
-      if (o < NO) {
-        const int j = o / (2 * NM), q = (o >> 1) % NM, c = o & 1;
-        double s = 0.0;
+    if (o < NO) {                  // whole rows of 16 lanes
+      double h[4];
 #pragma unroll
-        for (int t = g * 16; t < g * 16 + 16; ++t) s += red[o][t];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        s += __shfl_xor(s, 8);
-        const int l = l0 + j;
-        if (g == 0 && l <= lmax) {
-          double* dst = (double*)(M.out[q] + (int64_t)chunk * alm_idx(lmax, lmax + 1, lmax + 1) +
-                                  alm_idx(lmax, l, m));
-          dst[c] = s;
-        }
+      for (int k = 0; k < 4; ++k) {
+        const double* row = &red[o][64 * k + g];
+        h[k] = (row[0] + row[16]) + (row[32] + row[48]);
       }
+      const double s = row16_sum((h[0] + h[1]) + (h[2] + h[3]));
+      const int l = l0 + oj;
+      if (g == 0 && l <= lmax) ((double*)(outq + alm_idx(lmax, l, m)))[oc] = s;
     }
     __syncthreads();
-  }
+  };
+  int l0 = m;
+  round(l0, true);
+  for (l0 += LC; l0 + LC - 1 <= lmax; l0 += LC) round(l0, false);
+  if (l0 <= lmax) round(l0, true);
 }
 
 // alm (+)= w sum_chunk part[chunk]
@@ -873,14 +901,19 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
     um[r].x *= keep;
     um[r].y *= keep;
   }
-  for (int l0 = m; l0 <= lmax; l0 += kLChunk2) {
+  const int o = tid >> 4, g = tid & 15;
+  const int oj = o >> 2, oc = o & 3;
+  double* const outp = (double*)(part + ((int64_t)(oc >> 1) * A.nchunk + chunk) * nalm);
+  // the first round holds l = ls, the last may pass lmax: both take the
+  // checked path, the rounds between carry no per-step branch
+  auto round = [&](const int l0, const bool edge) {
 #pragma unroll
     for (int j = 0; j < kLChunk2; ++j) {
       const int l = l0 + j;
       double er = 0.0, ei = 0.0, br = 0.0, bi = 0.0;
-      if (l <= lmax && l >= ls) {
+      if (!edge || (l <= lmax && l >= ls)) {
         double a = 0.0, b = 0.0, c = 0.0;
-        if (l > ls) {
+        if (!edge || l > ls) {
           const double* cf = Cm + 3 * (l - m);
           a = sload(cf);
           b = sload(cf + 1);
@@ -890,7 +923,7 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
 #pragma unroll
         for (int r = 0; r < kPairsPerThread; ++r) {
           double tp, tm;
-          pol_adv(l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
+          pol_adv(edge && l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
           // lambda+ pairs with the (-1)^{l+m} fold, lambda- with the other
           const double2 qa = odd ? qm[r] : qp[r], qb = odd ? qp[r] : qm[r];
           const double2 ua = odd ? um[r] : up[r], ub = odd ? up[r] : um[r];
@@ -907,27 +940,26 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
     }
 #pragma unroll
     for (int r = 0; r < kPairsPerThread; ++r)
-      if (l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
+      if (!edge || l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
     __syncthreads();
     {
-      const int o = tid >> 4, g = tid & 15;
-      const int j = o >> 2, cc = o & 3;
-      double s = 0.0;
+      // 16 lanes per output, each summing a stride-16 column of the row
+      double h[4];
 #pragma unroll
-      for (int t = g * 16; t < g * 16 + 16; ++t) s += red[j][cc][t];
-      s += __shfl_xor(s, 1);
-      s += __shfl_xor(s, 2);
-      s += __shfl_xor(s, 4);
-      s += __shfl_xor(s, 8);
-      const int l = l0 + j;
-      if (g == 0 && l <= lmax) {
-        double* dst = (double*)(part + ((int64_t)(cc >> 1) * A.nchunk + chunk) * nalm +
-                                alm_idx(lmax, l, m));
-        dst[cc & 1] = s;
+      for (int k = 0; k < 4; ++k) {
+        const double* row = &red[oj][oc][64 * k + g];
+        h[k] = (row[0] + row[16]) + (row[32] + row[48]);
       }
+      const double sum = row16_sum((h[0] + h[1]) + (h[2] + h[3]));
+      const int l = l0 + oj;
+      if (g == 0 && l <= lmax) outp[2 * alm_idx(lmax, l, m) + (oc & 1)] = sum;
     }
     __syncthreads();
-  }
+  };
+  int l0 = m;
+  round(l0, true);
+  for (l0 += kLChunk2; l0 + kLChunk2 - 1 <= lmax; l0 += kLChunk2) round(l0, false);
+  if (l0 <= lmax) round(l0, true);
 }
 
 // Spin-2 synthesis: GQ[m][k], GU[m][k] = Q_m, U_m of every ring, kSynPairs
