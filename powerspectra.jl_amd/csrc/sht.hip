@@ -616,11 +616,13 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM>
 #pragma unroll
     for (int q = 0; q < NM; ++q) er[q][r] = ei[q][r] = orr[q][r] = oi[q][r] = 0.0;
   }
-  for (int l0 = m; l0 <= lmax; l0 += kSynChunk) {
+  // the first round holds l = m, m + 1, the last may pass lmax: both take
+  // the checked path, the rounds between load and step unconditionally
+  auto round = [&](const int l0, const bool edge) {
     double2 a[NM][kSynChunk], ab[kSynChunk];
 #pragma unroll
     for (int j = 0; j < kSynChunk; ++j) {
-      const bool in = l0 + j <= lmax;
+      const bool in = !edge || l0 + j <= lmax;
 #pragma unroll
       for (int q = 0; q < NM; ++q)
         a[q][j] = in ? sload(M.in[q] + i0 + (l0 + j - m)) : make_double2(0.0, 0.0);
@@ -629,16 +631,16 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM>
 #pragma unroll
     for (int j = 0; j < kSynChunk; ++j) {
       const int l = l0 + j;
-      if (l > lmax) break;
+      if (edge && l > lmax) break;
       const bool odd = j & 1;                      // l0 - m is a multiple of kSynChunk
 #pragma unroll
       for (int r = 0; r < SP; ++r) {
         double lam;
-        if (l == m) {
+        if (edge && l == m) {
           lam = l1[r];
         } else {
-          lam = l == m + 1 ? ab[j].x * z[r] * l1[r]
-                           : ab[j].x * fma(z[r], l1[r], -ab[j].y * l2[r]);
+          lam = (edge && l == m + 1) ? ab[j].x * z[r] * l1[r]
+                                     : ab[j].x * fma(z[r], l1[r], -ab[j].y * l2[r]);
           l2[r] = l1[r];
           l1[r] = lam;
         }
@@ -657,7 +659,11 @@ __global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM>
     }
 #pragma unroll
     for (int r = 0; r < SP; ++r) rescale_pair(l1[r], l2[r], sc[r], wgt[r]);
-  }
+  };
+  int l0 = m;
+  round(l0, true);
+  for (l0 += kSynChunk; l0 + kSynChunk - 1 <= lmax; l0 += kSynChunk) round(l0, false);
+  if (l0 <= lmax) round(l0, true);
 #pragma unroll
   for (int r = 0; r < SP; ++r) {
     const int p = pr[r];
@@ -995,30 +1001,34 @@ legendre_synthesis_s2(PolArgs A,
     x2[r] = y2[r] = 0.0;
     qe[r] = qo[r] = ue[r] = uo[r] = make_double2(0.0, 0.0);
   }
-  for (int l0 = m; l0 <= lmax; l0 += kLChunk2) {
+  // the first round holds l = ls, the last may pass lmax: both take the
+  // checked path, the rounds between load and step unconditionally
+  auto round = [&](const int l0, const bool edge) {
     double2 e[kLChunk2], b[kLChunk2];
     double ca[kLChunk2], cb[kLChunk2], cc[kLChunk2];
 #pragma unroll
     for (int j = 0; j < kLChunk2; ++j) {
       const int l = l0 + j;
-      const bool in = l >= ls && l <= lmax;
+      const bool in = !edge || (l >= ls && l <= lmax);
       e[j] = in ? sload(em + (l - m)) : make_double2(0.0, 0.0);
       b[j] = in ? sload(bm + (l - m)) : make_double2(0.0, 0.0);
       const double* cf = Cm + 3 * (l - m);
-      ca[j] = in && l > ls ? sload(cf) : 0.0;
-      cb[j] = in && l > ls ? sload(cf + 1) : 0.0;
-      cc[j] = in && l > ls ? sload(cf + 2) : 0.0;
+      const bool rec = !edge || (in && l > ls);
+      ca[j] = rec ? sload(cf) : 0.0;
+      cb[j] = rec ? sload(cf + 1) : 0.0;
+      cc[j] = rec ? sload(cf + 2) : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < kLChunk2; ++j) {
       const int l = l0 + j;
-      if (l > lmax) break;
-      if (l < ls) continue;
+      if (edge && l > lmax) break;
+      if (edge && l < ls) continue;
       const bool odd = j & 1;                      // l0 - m is a multiple of kLChunk2
 #pragma unroll
       for (int r = 0; r < kSynPairs; ++r) {
         double tp, tm;
-        pol_adv(l == ls, ca[j], cb[j], cc[j], z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
+        pol_adv(edge && l == ls, ca[j], cb[j], cc[j], z[r], x1[r], x2[r], y1[r], y2[r], wgt[r],
+                tp, tm);
         // lambda+ a_E and i lambda- a_B (Q); lambda+ a_B and -i lambda- a_E (U)
         const double2 pq = make_double2(tp * e[j].x, tp * e[j].y);
         const double2 mq = make_double2(-tm * b[j].y, tm * b[j].x);
@@ -1038,8 +1048,12 @@ legendre_synthesis_s2(PolArgs A,
     }
 #pragma unroll
     for (int r = 0; r < kSynPairs; ++r)
-      if (l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
-  }
+      if (!edge || l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
+  };
+  int l0 = m;
+  round(l0, true);
+  for (l0 += kLChunk2; l0 + kLChunk2 - 1 <= lmax; l0 += kLChunk2) round(l0, false);
+  if (l0 <= lmax) round(l0, true);
 #pragma unroll
   for (int r = 0; r < kSynPairs; ++r) {
     const int p = pr[r];
