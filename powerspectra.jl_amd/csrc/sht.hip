@@ -46,7 +46,28 @@ constexpr int kLChunk = 8;                  // l values per LDS reduction round
 #define PSX_SHT_PAIRS 4
 #endif
 constexpr int kPairsPerThread = PSX_SHT_PAIRS;   // ring pairs per analysis thread
-constexpr int kSynPairs = 4;                // ring pairs per synthesis thread
+#ifndef PSX_SYN_PAIRS
+#define PSX_SYN_PAIRS 4
+#endif
+constexpr int kSynPairs = PSX_SYN_PAIRS;     // ring pairs per synthesis thread
+// The spin-2 synthesis runs 2 ring pairs per thread at 5 waves per SIMD
+// (96 VGPRs, 28 bytes spilled outside the l loop): more waves cover its four
+// dependent recurrence chains better than more pairs per thread (4 pairs at
+// 3 waves took 1.8x as long at nside 1024, lmax 3071).
+#ifndef PSX_SYN2_PAIRS
+#define PSX_SYN2_PAIRS 2
+#endif
+#ifndef PSX_SYN2_WAVES
+#define PSX_SYN2_WAVES 5
+#endif
+constexpr int kSyn2Pairs = PSX_SYN2_PAIRS;   // ring pairs per spin-2 synthesis thread
+#ifndef PSX_AN2_PAIRS
+#define PSX_AN2_PAIRS 4
+#endif
+#ifndef PSX_AN2_WAVES
+#define PSX_AN2_WAVES 3
+#endif
+constexpr int kAn2Pairs = PSX_AN2_PAIRS;     // ring pairs per spin-2 analysis thread
 constexpr int kSynChunk = 4;                // l values read ahead per synthesis round
 constexpr double kBig = 0x1p600;            // rescaling threshold
 constexpr double kInvBig = 0x1p-600;
@@ -586,7 +607,13 @@ __global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ 
 
 // ring pairs per synthesis thread: fewer for larger groups (per pair and
 // map, four accumulators)
-__host__ __device__ constexpr int syn_pairs(int nm) { return nm <= 2 ? kSynPairs : 2; }
+#ifndef PSX_SYN_PAIRS4
+#define PSX_SYN_PAIRS4 2
+#endif
+#ifndef PSX_SYN_WAVES
+#define PSX_SYN_WAVES 1
+#endif
+__host__ __device__ constexpr int syn_pairs(int nm) { return nm <= 2 ? kSynPairs : PSX_SYN_PAIRS4; }
 
 // Synthesis: G_q[m][k] = sum_l a_q,lm lambda_lm(z_k) for SP ring pairs per
 // thread and each map q of the group (north: even + odd l+m parts, south:
@@ -595,7 +622,8 @@ __host__ __device__ constexpr int syn_pairs(int nm) { return nm <= 2 ? kSynPairs
 // steps that use them (a rescale check per such round: growth over 4 steps
 // stays below 2^26).
 template <int NM>
-__global__ void __launch_bounds__(256) legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PSX_SYN_WAVES)))
+legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
   constexpr int SP = syn_pairs(NM);
   const int m = blockIdx.y;
   const int lmax = A.R.lmax;
@@ -863,12 +891,10 @@ __device__ __forceinline__ void pol_rescale(double& x1, double& x2, double& y1, 
 // Spin-2 analysis partial sums: part[comp][chunk][idx(l, m)], comp 0 = E, 1 = B,
 // of sum over the chunk's pairs of [2 lambda+ Q + 2i lambda- U] (E) and
 // [2 lambda+ U - 2i lambda- Q] (B); F holds Q+, Q-, U+, U- at stride fs.
-// Both spin-2 Legendre kernels ask for 3 waves per SIMD (168 VGPRs): at their
-// natural 175 VGPRs only 2 waves fit, too few to cover the latency of the
-// dependent FP64 chains of the four recurrences.  The price is a few spilled
-// values: 8 scratch accesses per round of kLChunk2 steps in the analysis, none
-// inside the synthesis loop.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// The spin-2 analysis asks for PSX_AN2_WAVES waves per SIMD with
+// kAn2Pairs ring pairs per thread (152 VGPRs at 4 pairs and 3 waves, no
+// spills).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PSX_AN2_WAVES)))
 legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
                                                             int64_t fs, double2* __restrict__ part) {
   __shared__ double red[kLChunk2][4][256];
@@ -879,13 +905,13 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
   const int ls = m > 2 ? m : 2;
   const double* Cm = A.abc + 3 * alm_idx(lmax, m, m);   // Cm[3 (l - m) + 0..2]
   const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
-  double z[kPairsPerThread], x1[kPairsPerThread], x2[kPairsPerThread];
-  double y1[kPairsPerThread], y2[kPairsPerThread], wgt[kPairsPerThread];
-  int sc[kPairsPerThread];
-  double2 qp[kPairsPerThread], qm[kPairsPerThread], up[kPairsPerThread], um[kPairsPerThread];
+  double z[kAn2Pairs], x1[kAn2Pairs], x2[kAn2Pairs];
+  double y1[kAn2Pairs], y2[kAn2Pairs], wgt[kAn2Pairs];
+  int sc[kAn2Pairs];
+  double2 qp[kAn2Pairs], qm[kAn2Pairs], up[kAn2Pairs], um[kAn2Pairs];
 #pragma unroll
-  for (int r = 0; r < kPairsPerThread; ++r) {
-    const int p = (chunk * kPairsPerThread + r) * 256 + tid;
+  for (int r = 0; r < kAn2Pairs; ++r) {
+    const int p = (chunk * kAn2Pairs + r) * 256 + tid;
     const bool live = p < A.R.npairs;
     const int pp = live ? p : 0;
     z[r] = A.R.z[pp];
@@ -927,7 +953,7 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
         }
         const bool odd = j & 1;                    // l0 - m is a multiple of kLChunk2
 #pragma unroll
-        for (int r = 0; r < kPairsPerThread; ++r) {
+        for (int r = 0; r < kAn2Pairs; ++r) {
           double tp, tm;
           pol_adv(edge && l == ls, a, b, c, z[r], x1[r], x2[r], y1[r], y2[r], wgt[r], tp, tm);
           // lambda+ pairs with the (-1)^{l+m} fold, lambda- with the other
@@ -945,7 +971,7 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
       red[j][3][tid] = bi;
     }
 #pragma unroll
-    for (int r = 0; r < kPairsPerThread; ++r)
+    for (int r = 0; r < kAn2Pairs; ++r)
       if (!edge || l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
     __syncthreads();
     {
@@ -968,12 +994,12 @@ legendre_analysis_s2(PolArgs A, const double2* __restrict__ F,
   if (l0 <= lmax) round(l0, true);
 }
 
-// Spin-2 synthesis: GQ[m][k], GU[m][k] = Q_m, U_m of every ring, kSynPairs
+// Spin-2 synthesis: GQ[m][k], GU[m][k] = Q_m, U_m of every ring, kSyn2Pairs
 // ring pairs per thread; lambda+ terms go to the part of the (-1)^{l+m}
 // parity, lambda- terms to the other (north = even + odd, south = even - odd).
 // a_E, a_B and the coefficients of kLChunk2 steps are read ahead with scalar
 // loads.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PSX_SYN2_WAVES)))
 legendre_synthesis_s2(PolArgs A,
                                                              const double2* __restrict__ aE,
                                                              const double2* __restrict__ aB,
@@ -986,13 +1012,13 @@ legendre_synthesis_s2(PolArgs A,
   const double2* em = aE + i0;
   const double2* bm = aB + i0;
   const double* Cm = A.abc + 3 * i0;
-  double z[kSynPairs], x1[kSynPairs], x2[kSynPairs], y1[kSynPairs], y2[kSynPairs];
-  double wgt[kSynPairs];
-  double2 qe[kSynPairs], qo[kSynPairs], ue[kSynPairs], uo[kSynPairs];
-  int sc[kSynPairs], pr[kSynPairs];
+  double z[kSyn2Pairs], x1[kSyn2Pairs], x2[kSyn2Pairs], y1[kSyn2Pairs], y2[kSyn2Pairs];
+  double wgt[kSyn2Pairs];
+  double2 qe[kSyn2Pairs], qo[kSyn2Pairs], ue[kSyn2Pairs], uo[kSyn2Pairs];
+  int sc[kSyn2Pairs], pr[kSyn2Pairs];
 #pragma unroll
-  for (int r = 0; r < kSynPairs; ++r) {
-    const int p = (blockIdx.x * kSynPairs + r) * blockDim.x + threadIdx.x;
+  for (int r = 0; r < kSyn2Pairs; ++r) {
+    const int p = (blockIdx.x * kSyn2Pairs + r) * blockDim.x + threadIdx.x;
     pr[r] = p;
     const int pp = p < A.R.npairs ? p : 0;
     z[r] = A.R.z[pp];
@@ -1025,7 +1051,7 @@ legendre_synthesis_s2(PolArgs A,
       if (edge && l < ls) continue;
       const bool odd = j & 1;                      // l0 - m is a multiple of kLChunk2
 #pragma unroll
-      for (int r = 0; r < kSynPairs; ++r) {
+      for (int r = 0; r < kSyn2Pairs; ++r) {
         double tp, tm;
         pol_adv(edge && l == ls, ca[j], cb[j], cc[j], z[r], x1[r], x2[r], y1[r], y2[r], wgt[r],
                 tp, tm);
@@ -1047,7 +1073,7 @@ legendre_synthesis_s2(PolArgs A,
       }
     }
 #pragma unroll
-    for (int r = 0; r < kSynPairs; ++r)
+    for (int r = 0; r < kSyn2Pairs; ++r)
       if (!edge || l0 + kLChunk2 > ls) pol_rescale(x1[r], x2[r], y1[r], y2[r], sc[r], wgt[r]);
   };
   int l0 = m;
@@ -1055,7 +1081,7 @@ legendre_synthesis_s2(PolArgs A,
   for (l0 += kLChunk2; l0 + kLChunk2 - 1 <= lmax; l0 += kLChunk2) round(l0, false);
   if (l0 <= lmax) round(l0, true);
 #pragma unroll
-  for (int r = 0; r < kSynPairs; ++r) {
+  for (int r = 0; r < kSyn2Pairs; ++r) {
     const int p = pr[r];
     if (p >= A.R.npairs) continue;
     const int kn = p, ks = A.R.nr - 1 - p;
@@ -1177,7 +1203,8 @@ struct ShtPlan {
   Rings R;
   int lmax = 0;
   int64_t nalm = 0;
-  int nchunk = 0;
+  int nchunk = 0;          // pair chunks of the spin-0 analysis
+  int nchunk2 = 0;         // pair chunks of the spin-2 analysis
   // device
   int* dn = nullptr;
   int64_t* dfirst = nullptr;
@@ -1252,6 +1279,7 @@ ShtPlan* get_plan(Ctx* c, int nside, int lmax, bool pol = false) {
   P->lmax = lmax;
   P->nalm = alm_idx(lmax, lmax + 1, lmax + 1);
   P->nchunk = (P->R.npairs + 256 * kPairsPerThread - 1) / (256 * kPairsPerThread);
+  P->nchunk2 = (P->R.npairs + 256 * kAn2Pairs - 1) / (256 * kAn2Pairs);
   const Rings& R = P->R;
   P->dn = (int*)ctx_alloc(c, sizeof(int) * R.nr);
   P->dfirst = (int64_t*)ctx_alloc(c, sizeof(int64_t) * R.nr);
@@ -1498,7 +1526,7 @@ struct PolBufs {
 size_t pol_work_bytes(const ShtPlan& P) {
   const Rings& R = P.R;
   const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
-  return sizeof(double2) * (2 * (size_t)R.xlen + 4 * fl + 2 * (size_t)P.nchunk * P.nalm +
+  return sizeof(double2) * (2 * (size_t)R.xlen + 4 * fl + 2 * (size_t)P.nchunk2 * P.nalm +
                             2 * (size_t)(P.lmax + 1) * R.nr) +
          4 * sizeof(double) * (size_t)R.npix + 8 * 256;  // 7 buffers, 256-aligned
 }
@@ -1517,7 +1545,7 @@ PolBufs carve_pol(const ShtPlan& P, void* mem) {
   B.X[1] = (double2*)take(sizeof(double2) * R.xlen);
   B.F = (double2*)take(4 * sizeof(double2) * fl);
   B.fs = (int64_t)fl;
-  B.part = (double2*)take(2 * sizeof(double2) * P.nchunk * P.nalm);
+  B.part = (double2*)take(2 * sizeof(double2) * P.nchunk2 * P.nalm);
   B.G[0] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
   B.G[1] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
   B.syn = (double*)take(2 * sizeof(double) * R.npix);
@@ -1525,7 +1553,7 @@ PolBufs carve_pol(const ShtPlan& P, void* mem) {
   return B;
 }
 
-PolArgs pleg(const ShtPlan& P) { return PolArgs{P.ra(), P.domz, P.dabc, P.dlogc2, P.nchunk}; }
+PolArgs pleg(const ShtPlan& P) { return PolArgs{P.ra(), P.domz, P.dabc, P.dlogc2, P.nchunk2}; }
 
 // Ring Fourier coefficients of one real map, folded into F+ and F-.
 void ring_fold(Ctx* c, ShtPlan& P, const double* dmap, double2* X, double2* Fp, double2* Fm) {
@@ -1570,17 +1598,17 @@ void analysis_pol(Ctx* c, ShtPlan& P, const PolBufs& B, const double* dQ, const 
   hipStream_t s = c->stream;
   ring_fold(c, P, dQ, B.X[0], B.F, B.F + B.fs);
   ring_fold(c, P, dU, B.X[1], B.F + 2 * B.fs, B.F + 3 * B.fs);
-  hipLaunchKernelGGL(legendre_analysis_s2, dim3((unsigned)P.nchunk, (unsigned)(P.lmax + 1)),
+  hipLaunchKernelGGL(legendre_analysis_s2, dim3((unsigned)P.nchunk2, (unsigned)(P.lmax + 1)),
                      dim3(256), 0, s, pleg(P), (const double2*)B.F, B.fs, B.part);
   PSX_HIP(hipGetLastError());
   // the kernel sums 2 lambda+- terms; a_E, a_B carry -w
   const double w = -0.5 * 4.0 * kPi / (double)R.npix;
   const unsigned nb = (unsigned)((P.nalm + 255) / 256);
-  hipLaunchKernelGGL(sum_partials, dim3(nb), dim3(256), 0, s, (const double2*)B.part, P.nchunk,
+  hipLaunchKernelGGL(sum_partials, dim3(nb), dim3(256), 0, s, (const double2*)B.part, P.nchunk2,
                      P.nalm, w, accumulate ? 1 : 0, aE);
   PSX_HIP(hipGetLastError());
   hipLaunchKernelGGL(sum_partials, dim3(nb), dim3(256), 0, s,
-                     (const double2*)(B.part + (int64_t)P.nchunk * P.nalm), P.nchunk, P.nalm, w,
+                     (const double2*)(B.part + (int64_t)P.nchunk2 * P.nalm), P.nchunk2, P.nalm, w,
                      accumulate ? 1 : 0, aB);
   PSX_HIP(hipGetLastError());
 }
@@ -1590,7 +1618,7 @@ void synthesis_pol(Ctx* c, ShtPlan& P, const PolBufs& B, const double2* aE, cons
                    double* dQ, double* dU) {
   const Rings& R = P.R;
   hipLaunchKernelGGL(legendre_synthesis_s2,
-                     dim3((unsigned)((R.npairs + 256 * kSynPairs - 1) / (256 * kSynPairs)),
+                     dim3((unsigned)((R.npairs + 256 * kSyn2Pairs - 1) / (256 * kSyn2Pairs)),
                           (unsigned)(P.lmax + 1)),
                      dim3(256), 0, c->stream, pleg(P), aE, aB, B.G[0], B.G[1]);
   PSX_HIP(hipGetLastError());
