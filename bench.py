@@ -520,6 +520,138 @@ def run_cov(args, pkg) -> dict:
                  decouple=dec)
 
 
+def _cpu_pass(jobs, call, nthreads: int, runs: int):
+    """Median wall time of `runs` passes of call(job) over jobs, after a
+    warm-up pass over an eighth of them, on nthreads host threads (inline for
+    one).  Returns (median, sorted pass times)."""
+    import concurrent.futures
+
+    def timed(js, pool):
+        t0 = time.perf_counter()
+        if pool is None:
+            for jb in js:
+                call(jb)
+        else:
+            list(pool.map(call, js))
+        return time.perf_counter() - t0
+
+    pool = concurrent.futures.ThreadPoolExecutor(nthreads) if nthreads > 1 else None
+    try:
+        timed(jobs[: max(1, len(jobs) // 8)], pool)
+        ts = sorted(timed(jobs, pool) for _ in range(runs))
+    finally:
+        if pool is not None:
+            pool.shutdown()
+    return ts[len(ts) // 2], ts
+
+
+def run_cpu_configs(args, pkg) -> dict:
+    """The reference-algorithm CPU restatement (the oracle, one call per l1
+    row) beside the GPU kernels on three more configurations: the TT block at
+    lmax 767, the TTTT, EEEE, TETE and TTEE covariance blocks at lmax 2508, and
+    the TT block at lmax 6143.  CPU: systematic row samples (l1 = 0 mod
+    stride, largest rows first) on all physical cores (at most 16 per GPU) and
+    on one core, the median of 3 passes after a warm-up; full-block time =
+    stride x sample.  GPU: HIP-event kernel averages of the device-resident
+    calls.  The sampled rows are compared with the GPU blocks."""
+    from oracle import oracle as O
+
+    ctx = pkg.Context(0)
+    L = ctx.lib
+    nthreads = args.cpu_threads or min(16, physical_cores(), len(os.sched_getaffinity(0)))
+    runs = 3
+    VP = ctypes.c_void_p
+    arr = lambda ptrs: (VP * len(ptrs))(*[VP(p) for p in ptrs])
+
+    def cpu_side(n, lmax, call, strides):
+        out = {}
+        for key, st, nt in (("cpu", strides[0], nthreads), ("cpu_one_core", strides[1], 1)):
+            rows = sorted(range(0, n, st), key=lambda r: -min(r, lmax - r))
+            med, ts = _cpu_pass(rows, call, nt, runs)
+            out[key] = {"elements_per_s": n * n / (med * st), "threads": nt,
+                        "sample": f"rows l1 = 0 mod {st} ({len(rows)} of {n})",
+                        "passes_s": [round(t, 3) for t in ts]}
+        return out
+
+    def row_err(G, C, stride, n):
+        rows = range(0, n, stride)
+        g = np.concatenate([G[r, r:] for r in rows])
+        c = np.concatenate([C[r, r:] for r in rows])
+        return rel_err(g, c)
+
+    def mcm_case(lmax, strides):
+        LW = n = lmax + 1
+        W = synthetic_W(LW, 1, False)
+        dW = ctx.alloc(8 * LW)
+        ctx.h2d(dW, W)
+        dM = ctx.alloc(8 * n * n)
+        f = lambda: ctx.check(L.psx_mcm_d(ctx.handle, 0, 0, lmax, dW, LW, dM, n), "psx_mcm_00")
+        _, ks = _timed(ctx, f, ["psx_mcm_00"], args.steps, args.warmup)
+        G = np.empty((n, n), order="F")
+        ctx.d2h(G, dM)
+        ctx.free(dW)
+        ctx.free(dM)
+        C = np.full((n, n), np.nan, order="F")
+        res = cpu_side(n, lmax, lambda r: O.mcm_rows(O.MCM_00, 0, lmax, r, r, W, nthreads=1,
+                                                     out=C), strides)
+        ms = ks["psx_mcm_00"]
+        res["gpu"] = {"elements_per_s": n * n / (ms * 1e-3), "kernel_ms": ms}
+        res["gpu_vs_cpu"] = res["gpu"]["elements_per_s"] / res["cpu"]["elements_per_s"]
+        res["max_rel_err_sampled_rows"] = row_err(G, C, strides[0], n)
+        return res
+
+    def cov_case(lmax, kinds, strides):
+        LW = n = lmax + 1
+        rng = np.random.default_rng(0)
+        ell = np.arange(LW, dtype=np.float64)
+        sp = [1e3 / (ell + 10.0) ** 2.5 * rng.uniform(1, 2) for _ in range(4)]
+        rt = [1.0 + 0.1 * rng.uniform(size=LW) for _ in range(4)]
+        Ws = [synthetic_W(LW, 10 + k, False) for k in range(8)]
+        dsp, drt, dW = ([ctx.alloc(8 * LW) for _ in v] for v in (sp, rt, Ws))
+        for d, h in zip(dsp + drt + dW, sp + rt + Ws):
+            ctx.h2d(d, h)
+        dC = ctx.alloc(8 * n * n)
+        G = np.empty((n, n), order="F")
+        C = np.full((n, n), np.nan, order="F")
+        out = {}
+        for kind, name in kinds:
+            nW, nS, nR = O.COV_ARITY[kind]
+            kname = "psx_cov_" + name
+            S, R, W = arr(dsp), arr(drt), arr(dW[:nW])
+            f = lambda: ctx.check(L.psx_cov_d(ctx.handle, kind, 0, lmax, S, R, W, LW, 0, dC, n),
+                                  kname)
+            _, ks = _timed(ctx, f, [kname], args.steps, args.warmup)
+            ctx.d2h(G, dC)
+            call = lambda r: O.cov(kind, 0, lmax, sp[:nS], rt[:nR], Ws[:nW], nthreads=1,
+                                   rows=(r, r), out=C)
+            res = cpu_side(n, lmax, call, strides)
+            ms = ks[kname]
+            res["gpu"] = {"elements_per_s": n * n / (ms * 1e-3), "kernel_ms": ms}
+            res["gpu_vs_cpu"] = res["gpu"]["elements_per_s"] / res["cpu"]["elements_per_s"]
+            res["max_rel_err_sampled_rows"] = row_err(G, C, strides[0], n)
+            out[name] = res
+        for d in dsp + drt + dW + [dC]:
+            ctx.free(d)
+        return out
+
+    configs = {
+        "cfg0_TT_lmax767": mcm_case(767, (1, 2)),
+        "cfg2_cov_lmax2508": cov_case(2508, [(0, "tttt"), (1, "eeee"), (3, "tete"), (6, "ttee")],
+                                      (16, 128)),
+        "cfg3_TT_lmax6143": mcm_case(6143, (32, 256)),
+    }
+    ctx.close()
+    phys = physical_cores()
+    c0 = configs["cfg0_TT_lmax767"]
+    return _line("reference-algorithm CPU restatement vs GPU kernels, TT lmax 767, covariance "
+                 "lmax 2508, TT lmax 6143", c0["cpu"]["elements_per_s"], "elements/s", args,
+                 c0["gpu"]["kernel_ms"],
+                 {"workload": "cpu_configs: TT at lmax 767; TTTT, EEEE, TETE, TTEE at lmax 2508 "
+                              "(8 / 8 / 5 / 2 W spectra); TT at lmax 6143", "lmax": 767},
+                 value_note="value = CPU elements/s of the lmax-767 TT block on all cores",
+                 cpu_threads=nthreads, physical_cores=phys, configs=configs)
+
+
 def run_solve(args, pkg) -> dict:
     """lmax 6143: the TT and fused ++/-- kernels, then the LU solves the
     reference's maskedalm2spectra does (M \\ pCl for TT, [M++ M--; M-- M++]
@@ -1001,7 +1133,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent requests")
     ap.add_argument("--workload", default="mcm_set",
                     choices=["mcm_set", "mcm_blocks", "cov", "solve", "e2e", "serve", "sht",
-                             "sht_pol", "quickpol", "master", "binned", "mixed"],
+                             "sht_pol", "quickpol", "master", "binned", "mixed", "cpu_configs"],
                     help="mcm_set (default) is the headline; the others are side measurements")
     args = ap.parse_args()
 
@@ -1018,7 +1150,8 @@ def main() -> int:
         fn = {"mcm_blocks": run_mcm_blocks, "cov": run_cov, "solve": run_solve,
               "e2e": run_e2e, "serve": run_serve, "sht": run_sht, "sht_pol": run_sht_pol,
               "quickpol": run_quickpol, "master": run_master,
-              "binned": run_binned, "mixed": run_mixed}[args.workload]
+              "binned": run_binned, "mixed": run_mixed,
+              "cpu_configs": run_cpu_configs}[args.workload]
         print(json.dumps(fn(args, pkg)), flush=True)
         return 0
     abi = sys.modules[pkg.__name__ + "._abi"]

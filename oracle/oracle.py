@@ -100,7 +100,7 @@ def mcm_rows(kind: int, lmin: int, lmax: int, l1_lo: int, l1_hi: int, W: np.ndar
 
 
 def cov(kind: int, lmin: int, lmax: int, spectra, ratios, W, nthreads: int = 0,
-        rows: tuple | None = None) -> np.ndarray:
+        rows: tuple | None = None, out: np.ndarray | None = None) -> np.ndarray:
     nW, nS, nR = COV_ARITY[kind]
     spectra = [np.ascontiguousarray(s, dtype=np.float64) for s in spectra]
     ratios = [np.ascontiguousarray(r, dtype=np.float64) for r in ratios]
@@ -113,7 +113,7 @@ def cov(kind: int, lmin: int, lmax: int, spectra, ratios, W, nthreads: int = 0,
     wp = (_dp * nW)(*[_ptr(w) for w in W])
     n = lmax - lmin + 1
     lo, hi = rows if rows is not None else (lmin, lmax)
-    C = np.full((n, n), np.nan, order="F")
+    C = np.full((n, n), np.nan, order="F") if out is None else out
     st = lib().psxo_cov_rows(kind, lmin, lmax, lo, hi, sp, rp, wp, LW, C.ctypes.data_as(_dp), n,
                              nthreads)
     if st != 0:
