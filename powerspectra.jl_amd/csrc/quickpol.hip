@@ -44,7 +44,27 @@ struct QpArgs {
   int lmax, bl, bu;
   QpFam a, b;
   double sgn;
+  int nd, ntask;    // diagonals, wavefront tasks (nd x row blocks)
 };
+
+// Launch order of the wavefront tasks.  Task t = (row block t / nd, diagonal
+// t % nd) runs families of about 2 (lmax - 64 (t / nd)) steps, longest first.
+// All wavefronts are resident at once (~2 per SIMD), and consecutive
+// workgroups go to different SIMDs, so the tasks are taken in layers of
+// kQpLayer with every odd layer reversed: the SIMD that holds one of the
+// longest families of the first layer also gets one of the shortest of the
+// second.
+#ifndef PSX_QP_LAYER
+#define PSX_QP_LAYER 1024
+#endif
+constexpr int kQpLayer = PSX_QP_LAYER;
+
+__device__ __forceinline__ int qp_task(int b, int ntask) {
+  const int L = b / kQpLayer, k = b - L * kQpLayer;
+  if (!(L & 1)) return b;
+  const int size = min(kQpLayer, ntask - L * kQpLayer);
+  return L * kQpLayer + (size - 1 - k);
+}
 
 constexpr double kQpBig = 1e150;
 constexpr double kQpTiny = 1e-150;
@@ -68,13 +88,24 @@ __device__ __forceinline__ void qp_adv(QpState& s, double au, double rj, double 
   s.A = au * rv;
 }
 
-// The same step with Y = kje - tj c given through its uniform part kje = kj e.
-__device__ __forceinline__ void qp_step(QpState& s, double au, double rj, double rv, double r,
-                                        double jp, double Y) {
-  const double fn = -fma(Y, s.f, jp * s.A * s.p) * (rj * r);
+// The main-loop step.  With rj(j) = 1/(j au(j)) folded in, the recurrence
+// reads f(j+1) = -[Yr f(j) + C(j) f(j-1)] r, Yr = Y(j) rj(j), and the carried
+// coefficient C(j) = (j+1) A(j) rj(j) (held in s.A) advances as
+// C(j+1) = g(j) rv with the uniform g(j) = (j+2) au(j) rj(j+1).
+__device__ __forceinline__ void qp_step(QpState& s, double Yr, double g, double rv, double r) {
+  const double fn = -fma(Yr, s.f, s.A * s.p) * r;
   s.p = s.f;
   s.f = fn;
-  s.A = au * rv;
+  s.A = g * rv;
+}
+
+// 1/sqrt(v) for finite v > 0: the hardware estimate r0 (about 23 correct
+// bits) and one Newton-Raphson step carried to second order,
+// h = 1 - v r0^2, r = r0 (1 + h/2 + 3h^2/8), which leaves about one ulp.
+__device__ __forceinline__ double qp_rsqrt(double v) {
+  const double r0 = __builtin_amdgcn_rsq(v);
+  const double h = fma(-v * r0, r0, 1.0);
+  return fma(r0 * h, fma(0.375, h, 0.5), r0);
 }
 
 // The step out of j = 0 (nmin = 0: l == l'', M1 == 0), where the recurrence
@@ -102,11 +133,12 @@ __device__ __forceinline__ int wave_min(int v) {
 }
 
 __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
-  // per step: au_a, rj_a, au_b, rj_b, W, then the other uniform factors
-  // j + 1, 2j + 1, kj e_a, kj e_b, (j + 1)^2
-  __shared__ double tab[10][kQpSteps];
-  const int d = (int)blockIdx.x - A.bl;                                   // l - l''
-  const int lpp0 = A.lmax - (int)blockIdx.y * 64;                         // lane 0's l''
+  // per step and family (a, b): tj rj, kj e rj, g (qp_step); then W, tj =
+  // 2j + 1 and (j + 1)^2
+  __shared__ double tab[9][kQpSteps];
+  const int task = qp_task((int)blockIdx.x, A.ntask);
+  const int d = task % A.nd - A.bl;                                       // l - l''
+  const int lpp0 = A.lmax - (task / A.nd) * 64;                           // lane 0's l''
   const int lane = (int)threadIdx.x;
   const int lpp = lpp0 - lane;                                            // l'', descending
   const int l = lpp + d;
@@ -171,63 +203,74 @@ __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
     qp_adv0(fb, sqrt((1.0 - d2) * (1.0 - mb2)), rv, eb);
     j1 = 1;
   }
+  // the main loop carries C(j1) = (j1+1) A(j1) rj(j1) in place of A(j1)
+  {
+    const double q = ((double)j1 + 1.0) * ((double)j1 + 1.0);
+    fa.A *= ((double)j1 + 1.0) / ((double)j1 * sqrt((q - d2) * (q - ma2)));
+    fb.A *= ((double)j1 + 1.0) / ((double)j1 * sqrt((q - d2) * (q - mb2)));
+  }
 
   // main part: both families step together, uniform j; steps below
   // nmin_end need no per-lane test
   for (int jb = j1; jb <= jend; jb += kQpSteps) {
     {
       const int j = jb + lane;
-      const double jp2 = ((double)j + 1.0) * ((double)j + 1.0);
-      const double t = jp2 - d2;
-      double aa = 0.0, ra = 0.0, ab = 0.0, rb = 0.0;
+      const double jp = (double)j + 1.0, jq = (double)j + 2.0;
+      const double jp2 = jp * jp, jq2 = jq * jq;
+      const double tj = 2.0 * (double)j + 1.0, kj = tj * (double)j * jp;
+      double yta = 0.0, yka = 0.0, ytb = 0.0, ykb = 0.0, ga = 0.0, gb = 0.0;
       if (j < jend) {
-        aa = sqrt(t * (jp2 - ma2));
-        ra = 1.0 / ((double)j * aa);
-        ab = sqrt(t * (jp2 - mb2));
-        rb = 1.0 / ((double)j * ab);
+        // au(j) = sqrt(((j+1)^2 - d^2)((j+1)^2 - M1^2)) > 0 for j >= j1
+        const double aa = sqrt((jp2 - d2) * (jp2 - ma2)), ab = sqrt((jp2 - d2) * (jp2 - mb2));
+        const double aa1 = sqrt((jq2 - d2) * (jq2 - ma2));
+        const double ab1 = sqrt((jq2 - d2) * (jq2 - mb2));
+        const double ra = 1.0 / ((double)j * aa), rb = 1.0 / ((double)j * ab);
+        yta = tj * ra;
+        yka = kj * ea * ra;
+        ytb = tj * rb;
+        ykb = kj * eb * rb;
+        ga = jq * aa / (jp * aa1);
+        gb = jq * ab / (jp * ab1);
       }
       const double w = j <= jend && j < A.LW ? A.W[j] : 0.0;
-      const double jp = (double)j + 1.0, tj = 2.0 * (double)j + 1.0;
-      const double kj = tj * (double)j * jp;
       __syncthreads();
-      tab[0][lane] = aa;
-      tab[1][lane] = ra;
-      tab[2][lane] = ab;
-      tab[3][lane] = rb;
-      tab[4][lane] = w;
-      tab[5][lane] = jp;
-      tab[6][lane] = tj;
-      tab[7][lane] = kj * ea;
-      tab[8][lane] = kj * eb;
-      tab[9][lane] = jp2;
+      tab[0][lane] = yta;
+      tab[1][lane] = yka;
+      tab[2][lane] = ga;
+      tab[3][lane] = ytb;
+      tab[4][lane] = ykb;
+      tab[5][lane] = gb;
+      tab[6][lane] = w;
+      tab[7][lane] = tj;
+      tab[8][lane] = jp2;
       __syncthreads();
     }
     const int tend = min(kQpSteps, jend - jb + 1);
     const int tm = max(0, min(tend, nmin_end - jb));   // steps with j < nmin_end
     for (int t = 0; t < tm; ++t) {
-      const double jp = tab[5][t], tj = tab[6][t];
+      const double tj = tab[7][t];
       fa.S = fma(tj * fa.f, fa.f, fa.S);
       fb.S = fma(tj * fb.f, fb.f, fb.S);
-      D = fma(tab[4][t] * fa.f, fb.f, D);      // W is zero beyond LW
-      const double v = s2 - tab[9][t];          // (l+l''+1)^2 - (j+1)^2 > 0
-      const double r = rsqrt(v);
+      D = fma(tab[6][t] * fa.f, fb.f, D);      // W is zero beyond LW
+      const double v = s2 - tab[8][t];          // (l+l''+1)^2 - (j+1)^2 > 0
+      const double r = qp_rsqrt(v);
       const double rv = v * r;
-      qp_step(fa, tab[0][t], tab[1][t], rv, r, jp, fma(-tj, ca, tab[7][t]));
-      qp_step(fb, tab[2][t], tab[3][t], rv, r, jp, fma(-tj, cb, tab[8][t]));
+      qp_step(fa, fma(-tab[0][t], ca, tab[1][t]), tab[2][t], rv, r);
+      qp_step(fb, fma(-tab[3][t], cb, tab[4][t]), tab[5][t], rv, r);
     }
     for (int t = tm; t < tend; ++t) {          // the last steps, lane by lane
       const int j = jb + t;
       if (j > nmax) break;
-      const double jp = tab[5][t], tj = tab[6][t];
+      const double tj = tab[7][t];
       fa.S = fma(tj * fa.f, fa.f, fa.S);
       fb.S = fma(tj * fb.f, fb.f, fb.S);
-      D = fma(tab[4][t] * fa.f, fb.f, D);
+      D = fma(tab[6][t] * fa.f, fb.f, D);
       if (j == nmax) break;
-      const double v = s2 - tab[9][t];
-      const double r = rsqrt(v);
+      const double v = s2 - tab[8][t];
+      const double r = qp_rsqrt(v);
       const double rv = v * r;
-      qp_step(fa, tab[0][t], tab[1][t], rv, r, jp, fma(-tj, ca, tab[7][t]));
-      qp_step(fb, tab[2][t], tab[3][t], rv, r, jp, fma(-tj, cb, tab[8][t]));
+      qp_step(fa, fma(-tab[0][t], ca, tab[1][t]), tab[2][t], rv, r);
+      qp_step(fb, fma(-tab[3][t], cb, tab[4][t]), tab[5][t], rv, r);
     }
     qp_rescale(fa, D);
     qp_rescale(fb, D);
@@ -266,7 +309,9 @@ void quickpol_xi_dev(Ctx* c, int nu1, int nu2, int s1, int s2, int lmax, int bl,
   A.a = QpFam{s1 + nu1, -s1, -nu1};
   A.b = QpFam{s2 + nu2, -s2, -nu2};
   A.sgn = ((s1 + s2 + nu1 + nu2) & 1) ? -1.0 : 1.0;
-  const dim3 grid((unsigned)(bl + bu + 1), (unsigned)((lmax + 1 + 63) / 64));
+  A.nd = bl + bu + 1;
+  A.ntask = A.nd * ((lmax + 1 + 63) / 64);
+  const dim3 grid((unsigned)A.ntask);
   TimedLaunch tl(c, "psx_quickpol_xi");
   hipLaunchKernelGGL(quickpol_xi_kernel, grid, dim3(64), 0, c->stream, A);
   PSX_HIP(hipGetLastError());
