@@ -605,6 +605,20 @@ __global__ void __launch_bounds__(256) sum_partials(const double2* __restrict__ 
   alm[i] = a;
 }
 
+// G, the Legendre synthesis's output (G[m][ring] = sum_l a_lm lambda_lm), is
+// stored in groups of kGRings rings: entry (m, k) at
+// ((k / kGRings) (lmax + 1) + m) kGRings + k % kGRings.  The syntheses write
+// 128-byte runs of 8 adjacent rings, and the fold, which takes 8 adjacent
+// rings per workgroup, reads whole rows of a group for consecutive m.
+constexpr int kGRings = 8;
+__host__ __device__ inline int64_t gidx(int lmax, int m, int k) {
+  return ((int64_t)(k / kGRings) * (lmax + 1) + m) * kGRings + (k % kGRings);
+}
+// entries of G: the ring count padded to whole groups
+inline int64_t g_elems(int lmax, int nr) {
+  return (int64_t)((nr + kGRings - 1) / kGRings) * kGRings * (lmax + 1);
+}
+
 // ring pairs per synthesis thread: fewer for larger groups (per pair and
 // map, four accumulators)
 #ifndef PSX_SYN_PAIRS4
@@ -700,14 +714,14 @@ legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
       double2* G = M.out[q];
-      G[(int64_t)m * A.R.nr + kn] = make_double2(er[q][r] + orr[q][r], ei[q][r] + oi[q][r]);
+      G[gidx(lmax, m, kn)] = make_double2(er[q][r] + orr[q][r], ei[q][r] + oi[q][r]);
       if (ks != kn)
-        G[(int64_t)m * A.R.nr + ks] = make_double2(er[q][r] - orr[q][r], ei[q][r] - oi[q][r]);
+        G[gidx(lmax, m, ks)] = make_double2(er[q][r] - orr[q][r], ei[q][r] - oi[q][r]);
     }
   }
 }
 
-// Hermitian half spectrum of each ring from G (m-major: G[m * nr + k]).
+// Hermitian half spectrum of each ring from G (layout of gidx).
 // Y[q] collects G_m e^{i m phi0} for m = q mod n and conj(G_m e^{i m phi0})
 // for m = -q mod n (m >= 1); m = 0 contributes Re(G_0).  Every HEALPix ring
 // has phi0 = pi/n (first pixel shifted by half a pixel) or phi0 = 0, so
@@ -716,10 +730,10 @@ legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
 //                                      B = sum_{j>=1} s^j G_{jn-q}:
 // one small-angle sincos per output.  A workgroup covers kFoldRings adjacent
 // rings (lane & 7) and kFoldQ values of q; where the rings share n, a wave
-// reads 8 rows of 8 adjacent rings (128 contiguous bytes each).  Workgroup b
+// reads 8 consecutive rows of one ring group (1 KB contiguous).  Workgroup b
 // runs on XCD b mod 8: the 1-D grid gives all q-blocks of one ring group the
 // same XCD, so the rows of G the group reads are fetched into one L2.
-constexpr int kFoldRings = 8, kFoldQ = 32;
+constexpr int kFoldRings = kGRings, kFoldQ = 32;
 
 __global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, int nqb,
                                                       const double2* __restrict__ G,
@@ -734,22 +748,21 @@ __global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, int nqb,
   if (q > n / 2) return;
   const double phi0 = R.phi0[k];
   const double s = phi0 != 0.0 ? -1.0 : 1.0;
-  const int64_t nr = R.nr;
   double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, sg = 1.0;
   int m = q;
   if (q == 0) {
-    ar = G[k].x;
+    ar = G[gidx(R.lmax, 0, k)].x;
     m = n;
     sg = s;
   }
   for (; m <= R.lmax; m += n, sg *= s) {
-    const double2 g = G[(int64_t)m * nr + k];
+    const double2 g = G[gidx(R.lmax, m, k)];
     ar = fma(sg, g.x, ar);
     ai = fma(sg, g.y, ai);
   }
   sg = s;
   for (m = n - q; m <= R.lmax; m += n, sg *= s) {
-    const double2 g = G[(int64_t)m * nr + k];
+    const double2 g = G[gidx(R.lmax, m, k)];
     br = fma(sg, g.x, br);
     bi = fma(sg, g.y, bi);
   }
@@ -1085,7 +1098,7 @@ legendre_synthesis_s2(PolArgs A,
     const int p = pr[r];
     if (p >= A.R.npairs) continue;
     const int kn = p, ks = A.R.nr - 1 - p;
-    const int64_t on = (int64_t)m * A.R.nr + kn, os = (int64_t)m * A.R.nr + ks;
+    const int64_t on = gidx(lmax, m, kn), os = gidx(lmax, m, ks);
     // the factor -1/2 turns 2 lambda+- sums into Q_m, U_m
     GQ[on] = make_double2(-0.5 * (qe[r].x + qo[r].x), -0.5 * (qe[r].y + qo[r].y));
     GU[on] = make_double2(-0.5 * (ue[r].x + uo[r].x), -0.5 * (ue[r].y + uo[r].y));
@@ -1386,7 +1399,7 @@ size_t sht_work_bytes(const ShtPlan& P) {
   const Rings& R = P.R;
   const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
   return sizeof(double2) * ((size_t)R.xlen + 2 * fl + (size_t)P.nchunk * P.nalm +
-                            (size_t)(P.lmax + 1) * R.nr) +
+                            (size_t)g_elems(P.lmax, R.nr)) +
          2 * sizeof(double) * (size_t)R.npix + 8 * 256;  // 7 buffers, 256-aligned
 }
 
@@ -1404,7 +1417,7 @@ ShtBufs carve(const ShtPlan& P, void* mem) {
   B.Fp = (double2*)take(sizeof(double2) * fl);
   B.Fm = (double2*)take(sizeof(double2) * fl);
   B.part = (double2*)take(sizeof(double2) * P.nchunk * P.nalm);
-  B.G = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
+  B.G = (double2*)take(sizeof(double2) * g_elems(P.lmax, R.nr));
   B.syn = (double*)take(sizeof(double) * R.npix);
   B.res = (double*)take(sizeof(double) * R.npix);
   return B;
@@ -1527,7 +1540,7 @@ size_t pol_work_bytes(const ShtPlan& P) {
   const Rings& R = P.R;
   const size_t fl = (size_t)(P.lmax + 1) * R.npairs;
   return sizeof(double2) * (2 * (size_t)R.xlen + 4 * fl + 2 * (size_t)P.nchunk2 * P.nalm +
-                            2 * (size_t)(P.lmax + 1) * R.nr) +
+                            2 * (size_t)g_elems(P.lmax, R.nr)) +
          4 * sizeof(double) * (size_t)R.npix + 8 * 256;  // 7 buffers, 256-aligned
 }
 
@@ -1546,8 +1559,8 @@ PolBufs carve_pol(const ShtPlan& P, void* mem) {
   B.F = (double2*)take(4 * sizeof(double2) * fl);
   B.fs = (int64_t)fl;
   B.part = (double2*)take(2 * sizeof(double2) * P.nchunk2 * P.nalm);
-  B.G[0] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
-  B.G[1] = (double2*)take(sizeof(double2) * (P.lmax + 1) * R.nr);
+  B.G[0] = (double2*)take(sizeof(double2) * g_elems(P.lmax, R.nr));
+  B.G[1] = (double2*)take(sizeof(double2) * g_elems(P.lmax, R.nr));
   B.syn = (double*)take(2 * sizeof(double) * R.npix);
   B.res = (double*)take(2 * sizeof(double) * R.npix);
   return B;
