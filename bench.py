@@ -57,7 +57,8 @@ def csteps(lmin: int, lmax: int, LW: int) -> tuple[int, int]:
     """Number of (l1 <= l2, c) steps with an even (resp. odd) l3 <= LW-1.
 
     Even: l3 = l1+l2-2c, 0 <= c <= min(l1, l2).  Odd: l3 = l1+l2-2c-1,
-    0 <= c <= min(l1, l2) - 1.  The even count is the survey's T_con(even)."""
+    0 <= c <= min(l1, l2) - 1.  The even count is T_con(even), the contracted
+    (l1 <= l2, l3) terms of the even families."""
     ev = od = 0
     for l1 in range(lmin, lmax + 1):
         l2 = np.arange(l1, lmax + 1, dtype=np.int64)

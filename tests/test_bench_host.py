@@ -1,5 +1,5 @@
 """Host-side pieces of bench.py (no GPU): the module imports, the work
-counts behind its roofline figures reproduce the survey's tables, brute force
+counts behind its roofline figures match their tabulated values, brute force
 agrees with the closed counts, and more than one GPU is refused with a
 message."""
 import os
@@ -17,7 +17,7 @@ def _bench():
     return bench
 
 
-def test_csteps_matches_survey_counts():
+def test_csteps_matches_tabulated_counts():
     """T_con(even) and T_con(odd) with l3 <= lmax (LW = lmax + 1) for the
     configurations' lmax values."""
     b = _bench()
