@@ -477,40 +477,54 @@ void d2h_matrices_overlapped(Ctx* c, const D2HMatrix* m, int nm) {
   d2h_matrices_on(c->xstream, c->stage, c->stage_ev, m, nm);
 }
 
-void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
-                int64_t rows, int64_t cols) {
+namespace {
+void h2d_matrix_on(Ctx* c, hipStream_t s, double* dev, int64_t ldd, const double* host,
+                   int64_t ldh, int64_t rows, int64_t cols) {
   const size_t colb = sizeof(double) * (size_t)rows;
   if (rows <= 0 || cols <= 0) return;
   if (colb > kStageBytes && (cols == 1 || (ldd == rows && ldh == rows))) {
     const int64_t total = rows * cols, full = total / kLineDoubles, rem = total % kLineDoubles;
-    if (full) h2d_matrix(c, dev, kLineDoubles, host, kLineDoubles, kLineDoubles, full);
+    if (full) h2d_matrix_on(c, s, dev, kLineDoubles, host, kLineDoubles, kLineDoubles, full);
     if (rem)
-      h2d_matrix(c, dev + full * kLineDoubles, rem, host + full * kLineDoubles, rem, rem, 1);
+      h2d_matrix_on(c, s, dev + full * kLineDoubles, rem, host + full * kLineDoubles, rem, rem,
+                    1);
     return;
   }
   if (colb * (size_t)cols <= kDirectBytes || colb > kStageBytes) {
     PSX_HIP(hipMemcpy2DAsync(dev, ldd * sizeof(double), host, ldh * sizeof(double), colb,
-                             (size_t)cols, hipMemcpyHostToDevice, c->stream));
-    PSX_HIP(hipStreamSynchronize(c->stream));
+                             (size_t)cols, hipMemcpyHostToDevice, s));
+    PSX_HIP(hipStreamSynchronize(s));
     return;
   }
-  ensure_stage(c);
   const int64_t cpc = (int64_t)(kStageBytes / colb);
   const int64_t nch = (cols + cpc - 1) / cpc;
   for (int64_t k = 0; k < nch; ++k) {
     const int b = (int)(k & 1);
     const int64_t j0 = k * cpc, nc = std::min(cpc, cols - j0);
-    if (k >= 2) PSX_HIP(hipEventSynchronize(c->stage_ev[b]));  // buffer b free again
+    PSX_HIP(hipEventSynchronize(c->stage_ev[b]));  // buffer b free again
     copy_cols((double*)c->stage[b], rows, host + j0 * ldh, ldh, rows, nc);
     if (ldd == rows)
       PSX_HIP(hipMemcpyAsync(dev + j0 * ldd, c->stage[b], colb * (size_t)nc,
-                             hipMemcpyHostToDevice, c->stream));
+                             hipMemcpyHostToDevice, s));
     else
       PSX_HIP(hipMemcpy2DAsync(dev + j0 * ldd, ldd * sizeof(double), c->stage[b], colb, colb,
-                               (size_t)nc, hipMemcpyHostToDevice, c->stream));
-    PSX_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+                               (size_t)nc, hipMemcpyHostToDevice, s));
+    PSX_HIP(hipEventRecord(c->stage_ev[b], s));
   }
-  PSX_HIP(hipStreamSynchronize(c->stream));
+  PSX_HIP(hipStreamSynchronize(s));
+}
+}  // namespace
+
+void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
+                int64_t rows, int64_t cols) {
+  if ((size_t)rows * (size_t)cols * sizeof(double) > kDirectBytes) ensure_stage(c);
+  h2d_matrix_on(c, c->stream, dev, ldd, host, ldh, rows, cols);
+}
+
+void h2d_matrix_x(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
+                  int64_t rows, int64_t cols) {
+  ensure_stage(c);
+  h2d_matrix_on(c, c->xstream, dev, ldd, host, ldh, rows, cols);
 }
 
 }  // namespace psx

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <initializer_list>
 #include <map>
 #include <mutex>
@@ -149,8 +150,11 @@ void release_stage(Ctx* c);
 void release_tile_cache(Ctx* c);
 void release_sht_plans(Ctx* c);
 // spherical-harmonic transforms on device buffers (context stream)
+// `before`, if given, is called with map index q just before map q is first
+// read (its ring transforms are enqueued): a caller can still be copying the
+// later maps in while the first ones are transformed.
 void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
-                 double* dalms);
+                 double* dalms, const std::function<void(int)>* before = nullptr);
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps);
 // (spin0 = false: the spin-2 part only, the I maps' alms left as they are)
 void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
@@ -185,6 +189,12 @@ void d2h_matrices_on(hipStream_t s, void* const stage[2], hipEvent_t const ev[2]
                      const D2HMatrix* m, int nm);
 void h2d_matrix(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
                 int64_t rows, int64_t cols);
+// The same copy through the staging pipeline on the context's copy stream:
+// returns when the data is on the device, without waiting for (or ordering
+// against) work on the context stream.  The caller keeps both streams off the
+// destination meanwhile.
+void h2d_matrix_x(Ctx* c, double* dev, int64_t ldd, const double* host, int64_t ldh,
+                  int64_t rows, int64_t cols);
 
 // Launch helper: records HIP events around the launch when timing is on.
 struct TimedLaunch {

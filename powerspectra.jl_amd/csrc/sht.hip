@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -1480,13 +1481,20 @@ void launch_synthesis(ShtPlan& P, hipStream_t s, const ShtBufs* B, const double2
 
 // alms[q] (+)= A(maps[q]) for a group of nm <= kMaxGroup maps: Legendre passes
 // of up to PSX_AN_GROUP maps (the recurrence shared), ring transforms per map.
+// The maps of each analysis launch are folded just before it; `before`, if
+// given, is called with q before map q is first read.
 void analysis_group(Ctx* c, ShtPlan& P, const ShtBufs* B, int nm, const double* const* maps,
-                    double2* const* alms, bool accumulate) {
+                    double2* const* alms, bool accumulate,
+                    const std::function<void(int)>* before = nullptr) {
   PSX_REQUIRE(nm >= 1 && nm <= kMaxGroup, PSX_ERR_ARGUMENT, "1..4 maps per transform group");
   hipStream_t s = c->stream;
-  for (int q = 0; q < nm; ++q) fold_in(c, P, B[q], maps[q]);
   for (int q0 = 0; q0 < nm; q0 += PSX_AN_GROUP) {
-    switch (std::min(PSX_AN_GROUP, nm - q0)) {
+    const int k = std::min(PSX_AN_GROUP, nm - q0);
+    for (int q = q0; q < q0 + k; ++q) {
+      if (before) (*before)(q);
+      fold_in(c, P, B[q], maps[q]);
+    }
+    switch (k) {
       case 1: launch_analysis<1>(P, s, B + q0); break;
       case 2: launch_analysis<2>(P, s, B + q0); break;
       case 3: launch_analysis<3>(P, s, B + q0); break;
@@ -1656,7 +1664,7 @@ void check_sht_args(int nside, int lmax) {
 // groups of up to kMaxGroup that share their Legendre passes and iterate in
 // lockstep; every result equals the one-map transform bit for bit.
 void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* const* maps,
-                  double* const* alms) {
+                  double* const* alms, const std::function<void(int)>* before = nullptr) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
   const size_t per = (sht_work_bytes(P) + 255) & ~size_t(255);
@@ -1679,7 +1687,12 @@ void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* c
       res[q] = B[q].res;
       syn[q] = B[q].syn;
     }
-    analysis_group(c, P, B, nm, in, alm, false);
+    if (before) {
+      const std::function<void(int)> bq = [&](int q) { (*before)(g0 + q); };
+      analysis_group(c, P, B, nm, in, alm, false, &bq);
+    } else {
+      analysis_group(c, P, B, nm, in, alm, false);
+    }
     for (int it = 0; it < niter; ++it) {
       synthesis_group(c, P, B, nm, calm, syn);
       for (int q = 0; q < nm; ++q) {
@@ -1695,7 +1708,7 @@ void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* c
 
 // map2alm of nmaps consecutive device maps into consecutive alm sets
 void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
-                 double* dalms) {
+                 double* dalms, const std::function<void(int)>* before) {
   check_sht_args(nside, lmax);
   const int64_t npix = 12LL * nside * nside;
   const int64_t nalm = alm_idx(lmax, lmax + 1, lmax + 1);
@@ -1705,7 +1718,7 @@ void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double
     m[i] = dmaps + i * npix;
     a[i] = dalms + 2 * i * nalm;
   }
-  map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data());
+  map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data(), before);
 }
 
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps) {
@@ -1877,8 +1890,16 @@ int psx_mask_spectra(psx_ctx* h, int nside, int lmax, int niter, int nmaps, cons
     double* dm = x.take(mb);
     double* da = x.take(ab);
     double* dc = x.take(cb);
-    h2d_matrix(c, dm, npix, maps, npix, npix, nmaps);
-    map2alm_dev(c, nside, lmax, niter, nmaps, dm, da);
+    // the first analysis launch's maps go in first; the others are copied on
+    // the copy stream while that launch and its ring transforms run
+    const int first = std::min(nmaps, PSX_AN_GROUP);
+    h2d_matrix(c, dm, npix, maps, npix, npix, first);
+    const std::function<void(int)> rest = [&](int q) {
+      if (q == first)
+        h2d_matrix_x(c, dm + first * npix, npix, maps + first * npix, npix, npix,
+                     nmaps - first);
+    };
+    map2alm_dev(c, nside, lmax, niter, nmaps, dm, da, first < nmaps ? &rest : nullptr);
     cross_spectra_dev(c, lmax, nmaps, da, dc);
     d2h_matrix(c, cls, ncl, dc, ncl, ncl, 1);
   });
