@@ -156,9 +156,12 @@ void release_sht_plans(Ctx* c);
 void map2alm_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* dmaps,
                  double* dalms, const std::function<void(int)>* before = nullptr);
 void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, double* dmaps);
-// (spin0 = false: the spin-2 part only, the I maps' alms left as they are)
+// (spin0 = false: the spin-2 part only, the I maps' alms left as they are;
+// `between`, if given, is called once the spin-0 work is enqueued and before
+// Q and U are first read: a caller can still be copying them in)
 void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
-                     double* dteb, bool spin0 = true);
+                     double* dteb, bool spin0 = true,
+                     const std::function<void()>* between = nullptr);
 void alm2map_pol_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dteb, double* diqu);
 
 // Column-major matrix copies between host memory (any pageable buffer,

@@ -1663,8 +1663,10 @@ void check_sht_args(int nside, int lmax) {
 // interleaved), with niter refinements, on the context stream.  Maps go in
 // groups of up to kMaxGroup that share their Legendre passes and iterate in
 // lockstep; every result equals the one-map transform bit for bit.
+// sync = false: return once everything is enqueued on the context stream
 void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* const* maps,
-                  double* const* alms, const std::function<void(int)>* before = nullptr) {
+                  double* const* alms, const std::function<void(int)>* before = nullptr,
+                  bool sync = true) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax);
   const size_t per = (sht_work_bytes(P) + 255) & ~size_t(255);
@@ -1703,7 +1705,7 @@ void map2alm_ptrs(Ctx* c, int nside, int lmax, int niter, int n, const double* c
       analysis_group(c, P, B, nm, res, alm, true);
     }
   }
-  PSX_HIP(hipStreamSynchronize(c->stream));
+  if (sync) PSX_HIP(hipStreamSynchronize(c->stream));
 }
 
 // map2alm of nmaps consecutive device maps into consecutive alm sets
@@ -1746,7 +1748,7 @@ void alm2map_dev(Ctx* c, int nside, int lmax, int nmaps, const double* dalms, do
 // alms, nalm complex each; niter refinements of both the spin-0 and the
 // spin-2 parts.
 void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const double* diqu,
-                     double* dteb, bool spin0) {
+                     double* dteb, bool spin0, const std::function<void()>* between) {
   check_sht_args(nside, lmax);
   ShtPlan& P = *get_plan(c, nside, lmax, true);
   const int64_t npix = P.R.npix, nalm = P.nalm;
@@ -1757,8 +1759,11 @@ void map2alm_pol_dev(Ctx* c, int nside, int lmax, int niter, int nmaps, const do
       m[i] = diqu + 3 * i * npix;
       a[i] = dteb + 6 * i * nalm;
     }
-    map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data());
+    // with a `between` step the spin-0 work is left running meanwhile (the
+    // work area it held is reused below only in stream order)
+    map2alm_ptrs(c, nside, lmax, niter, nmaps, m.data(), a.data(), nullptr, between == nullptr);
   }
+  if (between) (*between)();
   WorkBuf wk(c, pol_work_bytes(P));
   const PolBufs B = carve_pol(P, wk.base());
   for (int i = 0; i < nmaps; ++i) {
@@ -2014,8 +2019,15 @@ int psx_map2alm_pol(psx_ctx* h, int nside, int lmax, int niter, int nmaps, const
     XferBuf x(c, xfer_bytes({mb, ab}));
     double* dm = x.take(mb);
     double* da = x.take(ab);
-    h2d_matrix(c, dm, npix, iqu, npix, npix, 3 * nmaps);
-    map2alm_pol_dev(c, nside, lmax, niter, nmaps, dm, da);
+    // the I maps first; Q and U are copied on the copy stream while I's
+    // transforms run
+    for (int i = 0; i < nmaps; ++i)
+      h2d_matrix(c, dm + 3 * i * npix, npix, iqu + 3 * i * npix, npix, npix, 1);
+    const std::function<void()> qu = [&] {
+      for (int i = 0; i < nmaps; ++i)
+        h2d_matrix_x(c, dm + (3 * i + 1) * npix, npix, iqu + (3 * i + 1) * npix, npix, npix, 2);
+    };
+    map2alm_pol_dev(c, nside, lmax, niter, nmaps, dm, da, true, &qu);
     d2h_matrix(c, teb, 2 * nalm, da, 2 * nalm, 2 * nalm, 3 * nmaps);
   });
 }
