@@ -1963,17 +1963,18 @@ int psx_master_spectra(psx_ctx* h, int nside, int lmax, int niter, const double*
     double* dwa = x.take(wab);
     double* dc = x.take(cb);
     double* dwc = x.take(wcb);
-    for (int i = 0; i < 6; ++i) h2d_matrix(c, dm + i * npix, npix, maps[i], npix, npix, 1);
+    // map m's I by its T mask, Q and U by its P mask
+    auto mask_map = [&](int i) {
+      if (already_masked) return;
+      const int a = 2 * (i / 3) + (i % 3 == 0 ? 0 : 1);
+      hipLaunchKernelGGL(apply_mask, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+                         c->stream, dm + i * npix, (const double*)(dw + slot[a] * npix), npix);
+      PSX_HIP(hipGetLastError());
+    };
+    // the I maps and the masks first: the spin-0 transforms need nothing else
+    for (int i : {0, 3}) h2d_matrix(c, dm + i * npix, npix, maps[i], npix, npix, 1);
     for (int k = 0; k < nu; ++k) h2d_matrix(c, dw + k * npix, npix, uniq[k], npix, npix, 1);
-    if (!already_masked) {
-      // map m's I by its T mask, Q and U by its P mask
-      for (int i = 0; i < 6; ++i) {
-        const int a = 2 * (i / 3) + (i % 3 == 0 ? 0 : 1);
-        hipLaunchKernelGGL(apply_mask, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
-                           c->stream, dm + i * npix, (const double*)(dw + slot[a] * npix), npix);
-        PSX_HIP(hipGetLastError());
-      }
-    }
+    for (int i : {0, 3}) mask_map(i);
     // spin 0: the two I maps and the distinct masks, in transform groups
     {
       const double* m0[6];
@@ -1987,8 +1988,11 @@ int psx_master_spectra(psx_ctx* h, int nside, int lmax, int niter, const double*
         m0[n0] = dw + k * npix;
         a0[n0] = dwa + 2 * k * nalm;
       }
-      map2alm_ptrs(c, nside, lmax, niter, n0, m0, a0);
+      map2alm_ptrs(c, nside, lmax, niter, n0, m0, a0, nullptr, false);
     }
+    // Q and U on the copy stream while the spin-0 transforms run
+    for (int i : {1, 2, 4, 5}) h2d_matrix_x(c, dm + i * npix, npix, maps[i], npix, npix, 1);
+    for (int i : {1, 2, 4, 5}) mask_map(i);
     map2alm_pol_dev(c, nside, lmax, niter, 2, dm, da, false);
     cross_spectra_dev(c, lmax, 6, da, dc);
     cross_spectra_dev(c, lmax, nu, dwa, dwc);
