@@ -165,6 +165,19 @@ def test_map2alm_pol_vs_host(pkg, ctx, nside, lfac, niter):
         assert np.abs(g - w).max() <= 1e-12 * np.abs(w).max()
 
 
+def test_map2alm_pol_several_maps_equal_single_calls(pkg, ctx):
+    """Three (I, Q, U) maps in one call (the I maps copied first, every Q and U
+    copied in while the I maps are transformed) equal one call per map bit for
+    bit."""
+    nside, lmax, niter = 16, 47, 2
+    maps = np.stack([np.stack([_smooth_map(nside, 700 + 3 * k + i) for i in range(3)])
+                     for k in range(3)])
+    both = ctx.map2alm_pol(maps, nside, lmax, niter)
+    for k in range(3):
+        one = ctx.map2alm_pol(maps[k], nside, lmax, niter)[0]
+        assert np.array_equal(both[k], one), k
+
+
 @pytest.mark.parametrize("nside,lmax", [(1, 2), (4, 11), (16, 40), (32, 95)])
 def test_alm2map_pol_vs_host(pkg, ctx, nside, lmax):
     from powerspectra_amd.sht import alm2map_pol_host
