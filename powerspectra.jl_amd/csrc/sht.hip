@@ -148,18 +148,32 @@ struct RingArgs {
 // X[q] = sum_j x_j e^{-2 pi i q j / n}, q = 0 .. n/2.  With the real input
 // folded as s_j = x_j + x_{n-j}, d_j = x_j - x_{n-j} (cos is even, sin odd):
 //   X[q] = x_0 + (-1)^q x_{n/2} + sum_{j=1}^{n/2-1} [s_j cos(2 pi q j/n) - i d_j sin(...)],
-// half the terms of the direct sum.  Each thread takes two frequencies; along
-// j their (cos, sin) advance by the angle-addition recurrence and are reset
-// from the LDS cosine table every kRotAnchor steps, so rounding drift stays
-// at ~kRotAnchor ulp (sin(2 pi t / n) = cos(2 pi (t - n/4) / n)).
+// half the terms of the direct sum.  Along j the (cos, sin) of a frequency
+// advance by the angle-addition recurrence and are reset from the LDS cosine
+// table every kRotAnchor steps, so rounding drift stays at ~kRotAnchor ulp.
 constexpr int kRotAnchor = 32;
+// Frequencies (or output pixel indices) per thread of the polar DFTs: one read
+// of a step's four sub-sequence values from LDS feeds kPolarPer rotations.
+constexpr int kPolarPer = 4;
 
-__device__ __forceinline__ void trig_at(const double* ctab, int n, int t, double& c,
-                                        double& s) {
-  int ts = t - n / 4;
-  ts += ts < 0 ? n : 0;
-  c = ctab[t];
-  s = ctab[ts];
+// cos and sin of 2 pi t / n, n = 4i, 0 <= t < n, from the quarter-wave table
+// cq[r] = cos(2 pi r / n), r = 0 .. i (sin(2 pi r / n) = cq[i - r]).
+__device__ __forceinline__ void trig_q(const double* cq, int i, int t, double& c, double& s) {
+  const bool h2 = t >= 2 * i;
+  int r = h2 ? t - 2 * i : t;
+  const bool h1 = r >= i;
+  r = h1 ? r - i : r;
+  const double a = cq[r], b = cq[i - r];
+  const double c0 = h1 ? -b : a, s0 = h1 ? a : b;
+  c = h2 ? -c0 : c0;
+  s = h2 ? -s0 : s0;
+}
+
+// Polar ring of workgroup b: the largest rings first (ordinal ns - 2 - b/2,
+// north and south alternating), so the longest workgroups start first.
+__device__ __forceinline__ int polar_ring(const RingArgs& R, int ns, int b) {
+  const int ord = ns - 2 - (b >> 1);
+  return (b & 1) ? R.nr - 1 - ord : ord;
 }
 
 // Polar rings have n = 4i pixels, so the forward DFT splits in radix 4: the
@@ -170,26 +184,32 @@ __device__ __forceinline__ void trig_at(const double* ctab, int n, int t, double
 // Y_r[i-p] = conj(Y_r[p]) (real input), one p gives four outputs:
 //   X[p] = sum_r Z_r,                 X[i+p] = sum_r (-i)^r Z_r,
 //   X[i-p] = sum_r (-i)^r conj(Z_r),  X[2i-p] = sum_r (-1)^r conj(Z_r),
-// which cover X[0 .. 2i] for p = 0 .. floor(i/2) (each written once).  One
-// thread per p carries the four sub-sums with one (cos, sin) rotation, reset
-// from the LDS cosine table every kRotAnchor steps; the sub-sequence values
-// are the same for every thread at a step (LDS broadcast).
+// which cover X[0 .. 2i] for p = 0 .. floor(i/2) (each written once).  A
+// thread takes kPolarPer consecutive p, each with its own (cos, sin)
+// rotation; the sub-sequence values of a step are the same for every thread
+// (LDS broadcast, two 16-byte reads each for s and -d).  The workgroup's two
+// halves of NMAX/32 threads (the p blocks of the longest ring, i <= NMAX/4 - 1)
+// sum the first and second half of the anchor blocks of j; the second half's
+// sums then pass through LDS to the first, which writes the outputs.
 template <int NMAX>
-__global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __restrict__ map,
-                                                 double2* __restrict__ X, int ns) {
-  __shared__ double ctab[NMAX];
-  __shared__ double sv[4][NMAX / 8], dv[4][NMAX / 8];   // s_rj, d_rj at [r][j], j >= 1
-  const int pr = blockIdx.x;                        // polar ring ordinal
-  const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
+__global__ void __launch_bounds__(NMAX / 16) polar_dft(RingArgs R, const double* __restrict__ map,
+                                                       double2* __restrict__ X, int ns) {
+  constexpr int kHalf = NMAX / 32;
+  static_assert(2 * (NMAX / 8) * 4 >= kHalf * 8 * kPolarPer, "sum exchange fits in sdv");
+  __shared__ double cq[NMAX / 4 + 1];
+  __shared__ __align__(16) double sdv[2][NMAX / 8][4];
+  double (*const sv)[4] = sdv[0];                   // s_rj at [j][r], j >= 1
+  double (*const dv)[4] = sdv[1];                   // -d_rj
+  const int k = polar_ring(R, ns, blockIdx.x);
   const int n = R.n[k], i = n / 4;
   const int H = (i + 1) / 2 - 1;                    // folded pairs j = 1 .. H
   const double* x = map + R.first[k];
-  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  for (int t = threadIdx.x; t <= i; t += blockDim.x) cq[t] = cospi(2.0 * t / n);
   for (int e = threadIdx.x; e < 4 * H; e += blockDim.x) {
     const int r = e & 3, j = (e >> 2) + 1;
     const double a = x[4 * j + r], b = x[4 * (i - j) + r];
-    sv[r][j] = a + b;
-    dv[r][j] = a - b;
+    sv[j][r] = a + b;
+    dv[j][r] = b - a;
   }
   __syncthreads();
   const bool even = (i & 1) == 0;
@@ -201,59 +221,103 @@ __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __res
   }
   double2* out = X + R.xoff[k];
   const int P = i / 2;
-  for (int p = threadIdx.x; p <= P; p += blockDim.x) {
-    double ar[4], ai[4];
+  const int half = threadIdx.x / kHalf, lt = threadIdx.x % kHalf;
+  const int pb = kPolarPer * lt;                    // p = pb .. pb + kPolarPer - 1
+  const bool live = pb <= P;
+  const int na = (H + kRotAnchor - 1) / kRotAnchor, na0 = (na + 1) / 2;
+  const int jb = 1 + (half ? na0 : 0) * kRotAnchor;
+  const int je = half ? H : min(H, na0 * kRotAnchor);
+  double ar[kPolarPer][4], ai[kPolarPer][4];
+  if (live) {
+    double cd[kPolarPer], sd[kPolarPer], c[kPolarPer], sn[kPolarPer];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      ar[r] = x0[r] + ((p & 1) ? -xm[r] : xm[r]);
-      ai[r] = 0.0;
+    for (int u = 0; u < kPolarPer; ++u) {
+      const int p = pb + u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        ar[u][r] = half ? 0.0 : x0[r] + ((p & 1) ? -xm[r] : xm[r]);
+        ai[u][r] = 0.0;
+      }
+      trig_q(cq, i, (4 * p) % n, cd[u], sd[u]);     // one step: angle 2 pi p / i
     }
-    double cd, sd;                                  // one-step rotation, angle 2 pi p / i
-    trig_at(ctab, n, (4 * p) % n, cd, sd);
-    double c = 1.0, sn = 0.0;
-    int ta = 0;                                     // 4 (p j mod i) at the last anchor
-    for (int j0 = 1; j0 <= H; j0 += kRotAnchor) {
-      ta = (int)((4LL * p * j0) % n);
-      trig_at(ctab, n, ta, c, sn);
-      const int jend = min(H + 1, j0 + kRotAnchor);
-      for (int j = j0; j < jend; ++j) {
+    for (int j0 = jb; j0 <= je; j0 += kRotAnchor) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          ar[r] = fma(sv[r][j], c, ar[r]);
-          ai[r] = fma(-dv[r][j], sn, ai[r]);
+      for (int u = 0; u < kPolarPer; ++u) trig_q(cq, i, (4 * (pb + u) * j0) % n, c[u], sn[u]);
+      const int jend = min(je + 1, j0 + kRotAnchor);
+      for (int j = j0; j < jend; ++j) {
+        const double2 s01 = *(const double2*)&sv[j][0], s23 = *(const double2*)&sv[j][2];
+        const double2 d01 = *(const double2*)&dv[j][0], d23 = *(const double2*)&dv[j][2];
+        const double sj[4] = {s01.x, s01.y, s23.x, s23.y};
+        const double dj[4] = {d01.x, d01.y, d23.x, d23.y};
+#pragma unroll
+        for (int u = 0; u < kPolarPer; ++u) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            ar[u][r] = fma(sj[r], c[u], ar[u][r]);
+            ai[u][r] = fma(dj[r], sn[u], ai[u][r]);
+          }
+          const double c2 = fma(c[u], cd[u], -sn[u] * sd[u]);
+          const double s2 = fma(sn[u], cd[u], c[u] * sd[u]);
+          c[u] = c2;
+          sn[u] = s2;
         }
-        const double c2 = fma(c, cd, -sn * sd), s2 = fma(sn, cd, c * sd);
-        c = c2;
-        sn = s2;
       }
     }
-    // Z_r = e^{-2 pi i r p / n} Y_r
-    double zx[4], zy[4];
-    zx[0] = ar[0];
-    zy[0] = ai[0];
+  }
+  // second half -> first half through sdv (no longer read)
+  __syncthreads();
+  double* xs = &sdv[0][0][0];
+  if (half && live) {
 #pragma unroll
-    for (int r = 1; r < 4; ++r) {
-      double cr, sr;
-      trig_at(ctab, n, (r * p) % n, cr, sr);
-      zx[r] = fma(cr, ar[r], sr * ai[r]);
-      zy[r] = fma(cr, ai[r], -sr * ar[r]);
+    for (int u = 0; u < kPolarPer; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        xs[((2 * u) * 4 + r) * kHalf + lt] = ar[u][r];
+        xs[((2 * u + 1) * 4 + r) * kHalf + lt] = ai[u][r];
+      }
+  }
+  __syncthreads();
+  if (half || !live) return;
+#pragma unroll
+  for (int u = 0; u < kPolarPer; ++u)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ar[u][r] += xs[((2 * u) * 4 + r) * kHalf + lt];
+      ai[u][r] += xs[((2 * u + 1) * 4 + r) * kHalf + lt];
     }
-    const double2 Xp = make_double2(zx[0] + zx[1] + zx[2] + zx[3], zy[0] + zy[1] + zy[2] + zy[3]);
-    const double2 Xip = make_double2(zx[0] + zy[1] - zx[2] - zy[3], zy[0] - zx[1] - zy[2] + zx[3]);
-    const double2 Xim = make_double2(zx[0] - zy[1] - zx[2] + zy[3], -zy[0] - zx[1] + zy[2] + zx[3]);
-    const double2 X2m = make_double2(zx[0] - zx[1] + zx[2] - zx[3], -zy[0] + zy[1] - zy[2] + zy[3]);
-    if (p == 0) {
-      out[0] = Xp;
-      out[i] = Xip;
-      out[2 * i] = X2m;
-    } else if (2 * p < i) {
-      out[p] = Xp;
-      out[i - p] = Xim;
-      out[i + p] = Xip;
-      out[2 * i - p] = X2m;
-    } else {                                        // p = i / 2
-      out[p] = Xp;
-      out[i + p] = Xip;
+  {
+#pragma unroll
+    for (int u = 0; u < kPolarPer; ++u) {
+      const int p = pb + u;
+      if (p > P) break;
+      // Z_r = e^{-2 pi i r p / n} Y_r  (r p < n)
+      double zx[4], zy[4];
+      zx[0] = ar[u][0];
+      zy[0] = ai[u][0];
+#pragma unroll
+      for (int r = 1; r < 4; ++r) {
+        double cr, sr;
+        trig_q(cq, i, r * p, cr, sr);
+        zx[r] = fma(cr, ar[u][r], sr * ai[u][r]);
+        zy[r] = fma(cr, ai[u][r], -sr * ar[u][r]);
+      }
+      const double2 Xp = make_double2(zx[0] + zx[1] + zx[2] + zx[3], zy[0] + zy[1] + zy[2] + zy[3]);
+      const double2 Xip = make_double2(zx[0] + zy[1] - zx[2] - zy[3], zy[0] - zx[1] - zy[2] + zx[3]);
+      const double2 Xim = make_double2(zx[0] - zy[1] - zx[2] + zy[3], -zy[0] - zx[1] + zy[2] + zx[3]);
+      const double2 X2m = make_double2(zx[0] - zx[1] + zx[2] - zx[3], -zy[0] + zy[1] - zy[2] + zy[3]);
+      if (p == 0) {
+        out[0] = Xp;
+        out[i] = Xip;
+        out[2 * i] = X2m;
+      } else if (2 * p < i) {
+        out[p] = Xp;
+        out[i - p] = Xim;
+        out[i + p] = Xip;
+        out[2 * i - p] = X2m;
+      } else {                                      // p = i / 2
+        out[p] = Xp;
+        out[i + p] = Xip;
+      }
     }
   }
 }
@@ -266,18 +330,23 @@ __global__ void __launch_bounds__(256) polar_dft(RingArgs R, const double* __res
 //   f_{4m+r} = sum_{p<i} W_r[p] e^{2 pi i p m/i}
 //            = W_r[0] + (-1)^m W_r[i/2] (i even)
 //              + 2 sum_{p=1}^{ceil(i/2)-1} Re(W_r[p] e^{2 pi i p m/i})
-// (W_r is Hermitian because f is real).  One thread per m carries the four
-// sums with one rotation and writes pixels 4m + r and 4(i - m) + r, whose
-// cosine sums agree and sine sums are opposite.
+// (W_r is Hermitian because f is real).  A thread takes kPolarPer
+// consecutive m, each with its own rotation, and writes pixels 4m + r and
+// 4(i - m) + r, whose cosine sums agree and sine sums are opposite.  As in
+// the forward DFT, the workgroup's two halves sum the two halves of the
+// anchor blocks of p and meet in LDS.
 template <int NMAX>
-__global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __restrict__ X,
-                                                  double* __restrict__ map, int ns) {
-  __shared__ double ctab[NMAX];
-  __shared__ double2 wv[4][NMAX / 8 + 1];           // 2 W_r[p], p = 0 .. i/2
-  const int pr = blockIdx.x;
-  const int k = pr < ns - 1 ? pr : R.nr - 1 - (pr - (ns - 1));
+__global__ void __launch_bounds__(NMAX / 16) polar_idft(RingArgs R, const double2* __restrict__ X,
+                                                        double* __restrict__ map, int ns) {
+  constexpr int kHalf = NMAX / 32;
+  static_assert(2 * (NMAX / 8 + 1) * 4 >= kHalf * 8 * kPolarPer, "sum exchange fits in wxy");
+  __shared__ double cq[NMAX / 4 + 1];
+  __shared__ __align__(16) double wxy[2][NMAX / 8 + 1][4];
+  double (*const wx)[4] = wxy[0];                   // Re 2 W_r[p] at [p][r]
+  double (*const wy)[4] = wxy[1];                   // Im 2 W_r[p]
+  const int k = polar_ring(R, ns, blockIdx.x);
   const int n = R.n[k], h = n / 2, i = n / 4;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) ctab[t] = cospi(2.0 * t / n);
+  for (int t = threadIdx.x; t <= i; t += blockDim.x) cq[t] = cospi(2.0 * t / n);
   __syncthreads();
   const double2* Y = X + R.xoff[k];
   const int P = i / 2;
@@ -304,70 +373,102 @@ __global__ void __launch_bounds__(256) polar_idft(RingArgs R, const double2* __r
     by[2] = y[0].y - y[1].y + y[2].y - y[3].y;
     bx[3] = y[0].x + y[1].y - y[2].x - y[3].y;
     by[3] = y[0].y - y[1].x - y[2].y + y[3].x;
-    wv[0][p] = make_double2(2.0 * bx[0], 2.0 * by[0]);
+    wx[p][0] = 2.0 * bx[0];
+    wy[p][0] = 2.0 * by[0];
 #pragma unroll
     for (int r = 1; r < 4; ++r) {
-      double cr, sr;                                // e^{+2 pi i p r / n}
-      trig_at(ctab, n, (r * p) % n, cr, sr);
-      wv[r][p] = make_double2(2.0 * fma(cr, bx[r], -sr * by[r]), 2.0 * fma(cr, by[r], sr * bx[r]));
+      double cr, sr;                                // e^{+2 pi i p r / n}  (r p < n)
+      trig_q(cq, i, r * p, cr, sr);
+      wx[p][r] = 2.0 * fma(cr, bx[r], -sr * by[r]);
+      wy[p][r] = 2.0 * fma(cr, by[r], sr * bx[r]);
     }
   }
   __syncthreads();
   double* f = map + R.first[k];
   const bool even = (i & 1) == 0;
   const int H = (i + 1) / 2 - 1;                    // p = 1 .. H
-  for (int m = threadIdx.x; m <= P; m += blockDim.x) {
-    double A[4], B[4];
+  double w0[4], wP[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      A[r] = 0.5 * (wv[r][0].x + (even ? ((m & 1) ? -wv[r][P].x : wv[r][P].x) : 0.0));
-      B[r] = 0.0;
+  for (int r = 0; r < 4; ++r) {
+    w0[r] = wx[0][r];
+    wP[r] = even ? wx[P][r] : 0.0;
+  }
+  const int half = threadIdx.x / kHalf, lt = threadIdx.x % kHalf;
+  const int mb = kPolarPer * lt;                    // m = mb .. mb + kPolarPer - 1
+  const bool live = mb <= P;
+  const int na = (H + kRotAnchor - 1) / kRotAnchor, na0 = (na + 1) / 2;
+  const int qb = 1 + (half ? na0 : 0) * kRotAnchor;
+  const int qe = half ? H : min(H, na0 * kRotAnchor);
+  double A[kPolarPer][4], B[kPolarPer][4];
+  if (live) {
+    double cd[kPolarPer], sd[kPolarPer], c[kPolarPer], sn[kPolarPer];
+#pragma unroll
+    for (int u = 0; u < kPolarPer; ++u) {
+      const int m = mb + u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        A[u][r] = half ? 0.0 : 0.5 * (w0[r] + ((m & 1) ? -wP[r] : wP[r]));
+        B[u][r] = 0.0;
+      }
+      trig_q(cq, i, (4 * m) % n, cd[u], sd[u]);     // one step: angle 2 pi m / i
     }
-    double cd, sd;                                  // one-step rotation, angle 2 pi m / i
-    trig_at(ctab, n, (4 * m) % n, cd, sd);
-    double c = 1.0, sn = 0.0;
-    for (int p0 = 1; p0 <= H; p0 += kRotAnchor) {
-      trig_at(ctab, n, (int)((4LL * m * p0) % n), c, sn);
-      const int pend = min(H + 1, p0 + kRotAnchor);
-      for (int p = p0; p < pend; ++p) {
+    for (int p0 = qb; p0 <= qe; p0 += kRotAnchor) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double2 w = wv[r][p];
-          A[r] = fma(w.x, c, A[r]);
-          B[r] = fma(w.y, sn, B[r]);
+      for (int u = 0; u < kPolarPer; ++u) trig_q(cq, i, (4 * (mb + u) * p0) % n, c[u], sn[u]);
+      const int pend = min(qe + 1, p0 + kRotAnchor);
+      for (int p = p0; p < pend; ++p) {
+        const double2 x01 = *(const double2*)&wx[p][0], x23 = *(const double2*)&wx[p][2];
+        const double2 y01 = *(const double2*)&wy[p][0], y23 = *(const double2*)&wy[p][2];
+        const double px[4] = {x01.x, x01.y, x23.x, x23.y};
+        const double py[4] = {y01.x, y01.y, y23.x, y23.y};
+#pragma unroll
+        for (int u = 0; u < kPolarPer; ++u) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            A[u][r] = fma(px[r], c[u], A[u][r]);
+            B[u][r] = fma(py[r], sn[u], B[u][r]);
+          }
+          const double c2 = fma(c[u], cd[u], -sn[u] * sd[u]);
+          const double s2 = fma(sn[u], cd[u], c[u] * sd[u]);
+          c[u] = c2;
+          sn[u] = s2;
         }
-        const double c2 = fma(c, cd, -sn * sd), s2 = fma(sn, cd, c * sd);
-        c = c2;
-        sn = s2;
       }
     }
+  }
+  // second half -> first half through wxy (no longer read)
+  __syncthreads();
+  double* xs = &wxy[0][0][0];
+  if (half && live) {
+#pragma unroll
+    for (int u = 0; u < kPolarPer; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        xs[((2 * u) * 4 + r) * kHalf + lt] = A[u][r];
+        xs[((2 * u + 1) * 4 + r) * kHalf + lt] = B[u][r];
+      }
+  }
+  __syncthreads();
+  if (half || !live) return;
+#pragma unroll
+  for (int u = 0; u < kPolarPer; ++u) {
+    const int m = mb + u;
+    if (m > P) break;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      f[4 * m + r] = A[r] - B[r];
-      if (m > 0 && 2 * m < i) f[4 * (i - m) + r] = A[r] + B[r];
+      const double a = A[u][r] + xs[((2 * u) * 4 + r) * kHalf + lt];
+      const double b = B[u][r] + xs[((2 * u + 1) * 4 + r) * kHalf + lt];
+      f[4 * m + r] = a - b;
+      if (m > 0 && 2 * m < i) f[4 * (i - m) + r] = a + b;
     }
   }
-}
-
-__device__ inline double2 ring_coef(const RingArgs& R, const double2* __restrict__ X, int k,
-                                    int m) {
-  const int n = R.n[k];
-  int q = m % n;
-  double2 v;
-  if (2 * q <= n) {
-    v = X[R.xoff[k] + q];
-  } else {
-    v = X[R.xoff[k] + (n - q)];
-    v.y = -v.y;
-  }
-  double s, c;
-  sincos((double)m * R.phi0[k], &s, &c);
-  // v e^{-i m phi0}
-  return make_double2(v.x * c + v.y * s, v.y * c - v.x * s);
 }
 
 // F+-[m][p] from the ring spectra (north ring p, south ring nr-1-p; the
-// equator pairs with itself and counts once).
+// equator pairs with itself and counts once).  The two rings of a pair have
+// the same n and phi0 (pi/n or 0), so with m = a n + q (0 <= q < n) the
+// shift e^{-i m phi0} = s^a e^{-i q phi0} (s = -1 where phi0 != 0) is one
+// small-angle sincos for both; X[q] = conj(X[n - q]) above n/2.
 __global__ void __launch_bounds__(256) fold_analysis(RingArgs R, const double2* __restrict__ X,
                                                      double2* __restrict__ Fp,
                                                      double2* __restrict__ Fm) {
@@ -375,12 +476,29 @@ __global__ void __launch_bounds__(256) fold_analysis(RingArgs R, const double2* 
   const int m = blockIdx.y;
   if (p >= R.npairs) return;
   const int kn = p, ks = R.nr - 1 - p;
-  const double2 a = ring_coef(R, X, kn, m);
-  double2 b = make_double2(0.0, 0.0);
-  if (ks != kn) b = ring_coef(R, X, ks, m);
+  const int n = R.n[kn];
+  const int a = m / n, q = m - a * n;
+  const double phi0 = R.phi0[kn];
+  double sn, c;
+  sincos((double)q * phi0, &sn, &c);
+  if (phi0 != 0.0 && (a & 1)) {
+    sn = -sn;
+    c = -c;
+  }
+  const bool hi = 2 * q > n;
+  const int qi = hi ? n - q : q;
+  const double cy = hi ? -1.0 : 1.0;
+  // v e^{-i m phi0}
+  auto coef = [&](int k) {
+    const double2 v = X[R.xoff[k] + qi];
+    const double vy = cy * v.y;
+    return make_double2(v.x * c + vy * sn, vy * c - v.x * sn);
+  };
+  const double2 A = coef(kn);
+  const double2 B = ks != kn ? coef(ks) : make_double2(0.0, 0.0);
   const int64_t o = (int64_t)m * R.npairs + p;
-  Fp[o] = make_double2(a.x + b.x, a.y + b.y);
-  Fm[o] = make_double2(a.x - b.x, a.y - b.y);
+  Fp[o] = make_double2(A.x + B.x, A.y + B.y);
+  Fm[o] = make_double2(A.x - B.x, A.y - B.y);
 }
 
 // Recurrence coefficients per (l, m), l >= m + 2, in a_lm order:
@@ -734,17 +852,66 @@ legendre_synthesis(LegArgs A, MapPtrs<NM> M) {
 // reads 8 consecutive rows of one ring group (1 KB contiguous).  Workgroup b
 // runs on XCD b mod 8: the 1-D grid gives all q-blocks of one ring group the
 // same XCD, so the rows of G the group reads are fetched into one L2.
-constexpr int kFoldRings = kGRings, kFoldQ = 32;
+//
+// A polar ring of n pixels folds about 2 (lmax + 1) / n terms into each
+// output: ~1500 dependent loads for n = 4 at lmax 3071, which would make the
+// few smallest rings the critical path of the whole launch.  The rings with
+// n <= kFoldSmallN (the first nsmall rings of each cap) therefore get a
+// workgroup each, launched ahead of the ring groups: each output is summed by a
+// team of S lanes (S a power of two, S (n/2 + 1) <= 256, S <= 64), lane t of
+// the team taking j = t, t + S, ..., and the team's partial sums are combined
+// by a butterfly of fixed order.
+constexpr int kFoldRings = kGRings, kFoldQ = 32, kFoldSmallN = 508;
 
-__global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, int nqb,
+__device__ __forceinline__ void fold_small_ring(const RingArgs& R, const double2* __restrict__ G,
+                                                double2* __restrict__ X, int k) {
+  const int n = R.n[k], nq = n / 2 + 1;
+  int S = 64;
+  while (S * nq > 256) S >>= 1;
+  const int q = threadIdx.x / S, t = threadIdx.x % S;
+  if (q >= nq) return;                              // whole teams: S divides 64
+  const double phi0 = R.phi0[k];
+  const bool neg = phi0 != 0.0;                     // s = -1
+  double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+  for (int j = t, m = q + t * n; m <= R.lmax; j += S, m += S * n) {
+    const double2 g = G[gidx(R.lmax, m, k)];
+    const double sg = (neg && (j & 1)) ? -1.0 : 1.0;
+    ar = fma(sg, g.x, ar);
+    if (m != 0) ai = fma(sg, g.y, ai);
+  }
+  for (int j = 1 + t, m = (1 + t) * n - q; m <= R.lmax; j += S, m += S * n) {
+    const double2 g = G[gidx(R.lmax, m, k)];
+    const double sg = (neg && (j & 1)) ? -1.0 : 1.0;
+    br = fma(sg, g.x, br);
+    bi = fma(sg, g.y, bi);
+  }
+  for (int w = 1; w < S; w <<= 1) {
+    ar += __shfl_xor(ar, w);
+    ai += __shfl_xor(ai, w);
+    br += __shfl_xor(br, w);
+    bi += __shfl_xor(bi, w);
+  }
+  if (t != 0) return;
+  double sn, c;
+  sincos((double)q * phi0, &sn, &c);
+  const double tr = ar + br, ti = ai - bi;
+  X[R.xoff[k] + q] = make_double2(c * tr - sn * ti, sn * tr + c * ti);
+}
+
+__global__ void __launch_bounds__(256) fold_synthesis(RingArgs R, int nqb, int nsmall, int nbs,
                                                       const double2* __restrict__ G,
                                                       double2* __restrict__ X) {
-  const int b = blockIdx.x;
+  if ((int)blockIdx.x < nbs) {                      // one small polar ring per workgroup
+    const int r = blockIdx.x;
+    if (r < 2 * nsmall) fold_small_ring(R, G, X, r < nsmall ? r : R.nr - 1 - (r - nsmall));
+    return;
+  }
+  const int b = blockIdx.x - nbs;                   // nbs is a multiple of 8: XCD b mod 8
   const int u = b >> 3;
   const int grp = (b & 7) + 8 * (u / nqb);
   const int k = grp * kFoldRings + (threadIdx.x & (kFoldRings - 1));
   const int q = (u % nqb) * kFoldQ + (threadIdx.x / kFoldRings);
-  if (k >= R.nr) return;
+  if (k >= R.nr || k < nsmall || k >= R.nr - nsmall) return;
   const int n = R.n[k];
   if (q > n / 2) return;
   const double phi0 = R.phi0[k];
@@ -777,8 +944,12 @@ void launch_fold_synthesis(const Rings& R, hipStream_t s, RingArgs ra, const dou
                            double2* X) {
   const int nqb = (2 * R.nside + 1 + kFoldQ - 1) / kFoldQ;     // n/2 <= 2 nside
   const int ngrp = (R.nr + kFoldRings - 1) / kFoldRings;
-  const int64_t nb = (int64_t)8 * nqb * ((ngrp + 7) / 8);
-  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)nb), dim3(256), 0, s, ra, nqb, G, X);
+  // polar rings k < nside - 1 have n = 4 (k + 1) pixels
+  const int nsmall = std::min(R.nside - 1, kFoldSmallN / 4);
+  const int nbs = (2 * nsmall + 7) / 8 * 8;
+  const int64_t nb = nbs + (int64_t)8 * nqb * ((ngrp + 7) / 8);
+  hipLaunchKernelGGL(fold_synthesis, dim3((unsigned)nb), dim3(256), 0, s, ra, nqb, nsmall, nbs,
+                     G, X);
   PSX_HIP(hipGetLastError());
 }
 
@@ -1185,23 +1356,26 @@ __global__ void __launch_bounds__(256) apply_mask(double* __restrict__ x,
 }
 
 // The polar DFTs with LDS tables sized for the longest ring of the grid
-// (4 nside - 4 pixels): 64 KB for nside <= 1024, 128 KB up to 2048.
+// (4 nside - 4 pixels): 41 KB and 256 threads for nside <= 1024, 82 KB and
+// 512 threads up to 2048.
 void launch_polar_dft(int ns, hipStream_t s, RingArgs ra, const double* map, double2* X) {
-  if (4 * (ns - 1) <= kMaxPolarRing / 2)
-    hipLaunchKernelGGL(polar_dft<kMaxPolarRing / 2>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, map,
-                       X, ns);
-  else
-    hipLaunchKernelGGL(polar_dft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, map, X,
+  constexpr int kSmall = kMaxPolarRing / 2;
+  if (4 * (ns - 1) <= kSmall)
+    hipLaunchKernelGGL(polar_dft<kSmall>, dim3(2 * (ns - 1)), dim3(kSmall / 16), 0, s, ra, map, X,
                        ns);
+  else
+    hipLaunchKernelGGL(polar_dft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(kMaxPolarRing / 16), 0,
+                       s, ra, map, X, ns);
 }
 
 void launch_polar_idft(int ns, hipStream_t s, RingArgs ra, const double2* X, double* map) {
-  if (4 * (ns - 1) <= kMaxPolarRing / 2)
-    hipLaunchKernelGGL(polar_idft<kMaxPolarRing / 2>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, X,
-                       map, ns);
-  else
-    hipLaunchKernelGGL(polar_idft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(256), 0, s, ra, X, map,
+  constexpr int kSmall = kMaxPolarRing / 2;
+  if (4 * (ns - 1) <= kSmall)
+    hipLaunchKernelGGL(polar_idft<kSmall>, dim3(2 * (ns - 1)), dim3(kSmall / 16), 0, s, ra, X, map,
                        ns);
+  else
+    hipLaunchKernelGGL(polar_idft<kMaxPolarRing>, dim3(2 * (ns - 1)), dim3(kMaxPolarRing / 16), 0,
+                       s, ra, X, map, ns);
 }
 
 // ---- plans ----------------------------------------------------------------
@@ -1649,7 +1823,7 @@ void synthesis_pol(Ctx* c, ShtPlan& P, const PolBufs& B, const double2* aE, cons
 
 void check_sht_args(int nside, int lmax) {
   // polar rings have at most 4 nside - 4 pixels: the DFT kernels' LDS
-  // cosine table holds kMaxPolarRing
+  // tables hold rings of up to kMaxPolarRing pixels
   PSX_REQUIRE(nside >= 1 && 4 * (nside - 1) <= kMaxPolarRing, PSX_ERR_ARGUMENT,
               "nside must be 1..2048");
   PSX_REQUIRE((nside & (nside - 1)) == 0, PSX_ERR_ARGUMENT, "nside must be a power of two");
