@@ -245,6 +245,20 @@ int psx_lu_solve_d(psx_ctx* ctx, int n, double* dA, int64_t lda, double* dB, int
  * which is exact.  LU with partial pivoting as psx_lu_solve. */
 int psx_lu_solve_pm(psx_ctx* ctx, int n, int sign, const double* Mpp, const double* Mmm,
                     int64_t ldM, double* B, int64_t ldb, int nrhs);
+/* The decoupling half of maskedalm2spectra (reference
+ * src/modecoupling.jl:348-377): the five blocks TT, TE (= TB), ET (= BT),
+ * ++ and -- from the mask cross-spectra W_TT, W_TE, W_ET, W_PP (length LW,
+ * as psx_mcm_set), then the nine spectra M \ pCl.  pcl holds the nine
+ * pseudo-spectra for l = lmin..lmax, n = lmax - lmin + 1 each, consecutive
+ * in the order TT, TE, TB, ET, BT, EE, BB, EB, BE; out receives the
+ * decoupled spectra in the same layout.  The blocks never leave the device:
+ * TT, TE (for TE and TB), ET (for ET and BT) and the 2n x 2n
+ * [M++ M--; M-- M++] (for EE/BB and, through D = diag(I, -I), EB/BE) are
+ * factored at the same time on separate streams.  LU with partial pivoting
+ * as psx_lu_solve; a zero pivot returns PSX_ERR_SINGULAR. */
+int psx_master_decouple(psx_ctx* ctx, int lmin, int lmax, int64_t LW, const double* W_TT,
+                        const double* W_TE, const double* W_ET, const double* W_PP,
+                        const double* pcl, double* out);
 /* `lu(A)` (reference src/blockspectralmatrix.jl:89-99, Julia's LAPACK getrf
  * with partial pivoting): LU (n x n, leading dimension ldlu) receives the
  * unit-lower L below the diagonal and U on and above it, ipiv[n] the 1-based

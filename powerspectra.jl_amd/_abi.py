@@ -99,6 +99,7 @@ SIGNATURES = {
     "psx_mask_spectra": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
     "psx_alm2cl": (_int, [_vp, _int, _int, ctypes.POINTER(_vp), _dp]),
     "psx_lu_solve_pm": (_int, [_vp, _int, _int, _dp, _dp, _i64, _dp, _i64, _int]),
+    "psx_master_decouple": (_int, [_vp, _int, _int, _i64] + [_dp] * 4 + [_dp, _dp]),
     "psx_mcm_binned": (_int, [_vp, _int, _int, _int, _dp, _i64, _dp, _int, _dp, _int, _dp,
                               _i64]),
     "psx_master_spectra": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp),
@@ -549,6 +550,24 @@ class Context:
         self.check(self.lib.psx_lu_solve(self._h, n, dptr(A), n, dptr(X), n, X.shape[1]),
                    "psx_lu_solve")
         return X[:, 0] if vec else X
+
+    def master_decouple(self, lmin: int, lmax: int, W_TT, W_TE, W_ET, W_PP, pcl):
+        """The nine decoupled spectra (TT, TE, TB, ET, BT, EE, BB, EB, BE) from
+        the four mask cross-spectra and the nine pseudo-spectra (rows of pcl,
+        l = lmin..lmax): the five blocks and the four solves stay on the device
+        (psx_master_decouple).  Returns a 9 x n array."""
+        Ws = [np.ascontiguousarray(w, dtype=np.float64) for w in (W_TT, W_TE, W_ET, W_PP)]
+        LW = Ws[0].size
+        if any(w.size != LW for w in Ws):
+            raise ArgumentError("all W spectra must have the same length")
+        n = _block_size(lmin, lmax)
+        P = np.ascontiguousarray(pcl, dtype=np.float64)
+        if P.shape != (9, n):
+            raise AxisError("pcl: nine pseudo-spectra of lmax - lmin + 1 entries expected")
+        X = np.empty((9, n))
+        self.check(self.lib.psx_master_decouple(self._h, lmin, lmax, LW, *[dptr(w) for w in Ws],
+                                                dptr(P), dptr(X)), "psx_master_decouple")
+        return X
 
     def lu_solve_pm(self, Mpp, Mmm, B, sign: int = 1):
         """[M++ s M--; s M-- M++] \\ B with s = sign (+1: EE/BB, -1: EB/BE), the

@@ -426,6 +426,65 @@ int psx_mcm_set(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
   });
 }
 
+int psx_master_decouple(psx_ctx* h, int lmin, int lmax, int64_t LW, const double* W_TT,
+                        const double* W_TE, const double* W_ET, const double* W_PP,
+                        const double* pcl, double* out) {
+  Ctx* c = C(h);
+  return guarded(c, [&] {
+    check_block(lmin, lmax, LW, lmax - lmin + 1);
+    PSX_REQUIRE(W_TT && W_TE && W_ET && W_PP && pcl && out, PSX_ERR_ARGUMENT, "null pointer");
+    PSX_HIP(hipSetDevice(c->device));
+    const int n = lmax - lmin + 1, N = 2 * n;
+    PSX_REQUIRE((int64_t)N * N <= (int64_t(1) << 31) && N <= INT32_MAX / 2, PSX_ERR_ARGUMENT,
+                "lmax - lmin too large for the EE/BB system");
+    // Every block has leading dimension 2n: TT, TE and ET each in an 2n x n
+    // buffer (top half used), ++ and -- straight into the left half of the
+    // 2n x 2n EE/BB matrix [M++ M--; M-- M++].
+    const size_t wb = sizeof(double) * LW, tb = sizeof(double) * (size_t)N * n,
+                 ab = sizeof(double) * (size_t)N * N, bb = sizeof(double) * 9 * (size_t)n,
+                 pb = sizeof(int) * ((size_t)N + 64);
+    XferBuf x(c, xfer_bytes({wb, wb, wb, wb, tb, tb, tb, ab, bb, pb, pb, pb, pb}));
+    const double* Wh[4] = {W_TT, W_TE, W_ET, W_PP};
+    double* dW[4];
+    for (int k = 0; k < 4; ++k) {
+      dW[k] = x.take(wb);
+      h2d(c, dW[k], Wh[k], wb);
+    }
+    double* dTT = x.take(tb);
+    double* dTE = x.take(tb);
+    double* dET = x.take(tb);
+    double* dA = x.take(ab);
+    double* dB = x.take(bb);
+    int* piv[4];
+    for (int k = 0; k < 4; ++k) piv[k] = reinterpret_cast<int*>(x.take(pb));
+    double* const dM[5] = {dTT, dTE, dET, dA, dA + n};
+    run_contraction(c, set_request(lmin, lmax, LW, dW, dM, N));
+    const int64_t off = (int64_t)n * N + n;   // the bottom-right block
+    PSX_HIP(hipMemcpy2DAsync(dA + off, sizeof(double) * N, dA, sizeof(double) * N,
+                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, c->stream));
+    PSX_HIP(hipMemcpy2DAsync(dA + (int64_t)n * N, sizeof(double) * N, dA + n,
+                             sizeof(double) * N, sizeof(double) * n, n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    // right-hand sides, in the order of pcl (TT, TE, TB, ET, BT, EE, BB, EB,
+    // BE) with BE negated: [EB; -BE] solves [M++ M--; M-- M++] when [EB; BE]
+    // solves [M++ -M--; -M-- M++] = D [M++ M--; M-- M++] D, D = diag(I, -I)
+    std::vector<double> rhs(pcl, pcl + 9 * (size_t)n);
+    for (int i = 0; i < n; ++i) rhs[8 * (size_t)n + i] = -rhs[8 * (size_t)n + i];
+    h2d(c, dB, rhs.data(), bb);
+    const LuSystem sys[4] = {
+        {N, dA, N, dB + 5 * (size_t)n, N, 2, piv[0]},   // EE, BB | EB, -BE
+        {n, dTT, N, dB, n, 1, piv[1]},                  // TT
+        {n, dTE, N, dB + n, n, 2, piv[2]},              // TE | TB
+        {n, dET, N, dB + 3 * (size_t)n, n, 2, piv[3]},  // ET | BT
+    };
+    lu_solve_concurrent(c, sys, 4);
+    PSX_HIP(hipMemcpyAsync(rhs.data(), dB, bb, hipMemcpyDeviceToHost, c->stream));
+    PSX_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) rhs[8 * (size_t)n + i] = -rhs[8 * (size_t)n + i];
+    std::copy(rhs.begin(), rhs.end(), out);
+  });
+}
+
 int psx_cov_d(psx_ctx* h, int kind, int lmin, int lmax, const double* const* dspectra,
               const double* const* dratios, const double* const* dW, int64_t LW,
               int precision, double* dC, int64_t ldC) {

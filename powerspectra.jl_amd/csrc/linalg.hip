@@ -26,7 +26,10 @@ static rocblas_handle blas(Ctx* c) {
   return static_cast<rocblas_handle>(c->blas);
 }
 
+void release_lanes(Ctx* c);
+
 void release_blas(Ctx* c) {
+  release_lanes(c);
   if (c->blas) {
     rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
     c->blas = nullptr;
@@ -443,14 +446,31 @@ __global__ void __launch_bounds__(256) lu_leaf_move(double* A, int64_t lda, int 
     if (k < nm) a[sk[k]] = v[k];
 }
 
-void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinfo) {
-  rocblas_handle h = blas(c);
-  hipStream_t s = c->stream;
+// Where an LU runs: a stream, the rocBLAS handle bound to it and a device work
+// area of lu_work_bytes(n) bytes (pivot candidates, flags, the leaf's row
+// movement).  Factorizations on different lanes may run at the same time.
+struct LuLane {
+  hipStream_t s;
+  rocblas_handle h;
+  void* work;
+};
+
+size_t lu_work_bytes(int n) {
   const int Gmax = (n + kLuRows - 1) / kLuRows;
   const size_t rows_bytes = 2 * (size_t)Gmax * kLuLeaf * sizeof(double);
   const size_t move_off = (rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf + (size_t)n + 255) &
                           ~size_t(255);
-  char* base = (char*)ctx_scratch(c, move_off + sizeof(int) * (4 * kLuLeaf + 1) + 256);
+  return move_off + sizeof(int) * (4 * kLuLeaf + 1) + 256;
+}
+
+void lu_factor_on(const LuLane& L, int n, double* dA, int64_t lda, int* dipiv, int* dinfo) {
+  rocblas_handle h = L.h;
+  hipStream_t s = L.s;
+  const int Gmax = (n + kLuRows - 1) / kLuRows;
+  const size_t rows_bytes = 2 * (size_t)Gmax * kLuLeaf * sizeof(double);
+  const size_t move_off = (rows_bytes + 24 * (size_t)Gmax + 4 * kLuLeaf + (size_t)n + 255) &
+                          ~size_t(255);
+  char* base = (char*)L.work;
   LuWork w;
   w.cstride = Gmax;
   w.candrow = (double*)base;                                  // 2 Gmax kLuLeaf doubles
@@ -523,6 +543,11 @@ void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinf
                             (rocblas_int)lda));
     }
   }
+}
+
+void lu_factor_dev(Ctx* c, int n, double* dA, int64_t lda, int* dipiv, int* dinfo) {
+  lu_factor_on(LuLane{c->stream, blas(c), ctx_scratch(c, lu_work_bytes(n))}, n, dA, lda, dipiv,
+               dinfo);
 }
 
 // ---- triangular solves with the LU factors, one right-hand side ------------
@@ -664,10 +689,9 @@ __global__ void __launch_bounds__(256) trs_step(const double* __restrict__ A, in
   if (t < cn) x[r] = y;
 }
 
-// x = L^{-1} x, then x = U^{-1} x, with the packed LU factors of dA.
-void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
+// x = L^{-1} x, then x = U^{-1} x, with the packed LU factors of dA, on stream s.
+void lu_trs_on(hipStream_t s, int n, const double* dA, int64_t lda, double* dx) {
   const int nb = (n + kTrsRows - 1) / kTrsRows;
-  hipStream_t s = c->stream;
   // (the first step of each sweep has no source block: only the diagonal)
   for (int k = 0; k < nb; ++k)
     hipLaunchKernelGGL(trs_step<false>, dim3(k == 0 ? 1 : nb - k), dim3(256), 0, s, dA, lda, n,
@@ -676,6 +700,10 @@ void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
     hipLaunchKernelGGL(trs_step<true>, dim3(k == nb - 1 ? 1 : k + 1), dim3(256), 0, s, dA, lda,
                        n, k, dx);
   PSX_HIP(hipGetLastError());
+}
+
+void lu_trs_dev(Ctx* c, int n, const double* dA, int64_t lda, double* dx) {
+  lu_trs_on(c->stream, n, dA, lda, dx);
 }
 
 // ---- triangular solves with many right-hand sides ---------------------------
@@ -919,6 +947,82 @@ void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB, int64_t ld
   lu_factor_checked(c, n, dA, lda, pv.p);
   lu_getrs_dev(c, n, dA, lda, pv.p, dB, ldb, nrhs);
   PSX_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Independent systems side by side.  System i runs on lane i: lane 0 is the
+// context stream, lanes 1.. the context's extra streams, each with its own
+// rocBLAS handle and its own part of the scratch.  The lanes start after the
+// work already queued on the context stream, and the context stream waits
+// for all of them, so the call keeps stream order for its caller.  The
+// factorizations are latency-bound (a leaf column runs on one or a few
+// workgroups), so running them at once overlaps their latencies.
+void ensure_lanes(Ctx* c, int nextra) {
+  for (int i = 0; i < nextra; ++i) {
+    if (c->lstream[i]) continue;
+    PSX_HIP(hipStreamCreateWithFlags(&c->lstream[i], hipStreamNonBlocking));
+    rocblas_handle h;
+    PSX_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, PSX_ERR_DEVICE,
+                "rocblas_create_handle failed");
+    rocblas_set_stream(h, c->lstream[i]);
+    c->lblas[i] = h;
+  }
+  for (hipEvent_t& e : c->lane_ev)
+    if (!e) PSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+void lu_solve_concurrent(Ctx* c, const LuSystem* sys, int count) {
+  PSX_REQUIRE(count >= 1 && count <= 1 + kLuExtraLanes, PSX_ERR_ARGUMENT,
+              "too many concurrent systems");
+  ensure_lanes(c, count - 1);
+  std::vector<size_t> off(count + 1, 0);
+  for (int i = 0; i < count; ++i) {
+    PSX_REQUIRE(sys[i].nrhs >= 1 && sys[i].nrhs <= kTrsMaxRhs, PSX_ERR_ARGUMENT,
+                "concurrent systems take 1 to 4 right-hand sides");
+    off[i + 1] = off[i] + ((lu_work_bytes(sys[i].n) + 255) & ~size_t(255));
+  }
+  char* work = (char*)ctx_scratch(c, off[count]);
+  blas(c);
+  PSX_HIP(hipEventRecord(c->lane_ev[0], c->stream));
+  for (int i = 0; i < count; ++i) {
+    const LuSystem& q = sys[i];
+    const LuLane L = i == 0 ? LuLane{c->stream, blas(c), work}
+                            : LuLane{c->lstream[i - 1], (rocblas_handle)c->lblas[i - 1],
+                                     work + off[i]};
+    if (i > 0) PSX_HIP(hipStreamWaitEvent(L.s, c->lane_ev[0], 0));
+    lu_factor_on(L, q.n, q.dA, q.lda, q.ipiv, q.ipiv + q.n + 32);
+    PSX_ROC(rocsolver_dlaswp(L.h, q.nrhs, q.dB, (rocblas_int)q.ldb, 1, q.n,
+                             (const rocblas_int*)q.ipiv, 1));
+    for (int j = 0; j < q.nrhs; ++j) lu_trs_on(L.s, q.n, q.dA, q.lda, q.dB + (int64_t)j * q.ldb);
+    PSX_HIP(hipGetLastError());
+    if (i > 0) {
+      PSX_HIP(hipEventRecord(c->lane_ev[i], L.s));
+      PSX_HIP(hipStreamWaitEvent(c->stream, c->lane_ev[i], 0));
+    }
+  }
+  // SingularException for the first system with a zero pivot
+  std::vector<int> info(count, 0);
+  for (int i = 0; i < count; ++i)
+    PSX_HIP(hipMemcpyAsync(&info[i], sys[i].ipiv + sys[i].n + 32, sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  PSX_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < count; ++i)
+    PSX_REQUIRE(info[i] == 0, PSX_ERR_SINGULAR,
+                "LU: zero pivot at position " + std::to_string(info[i]) + " of system " +
+                    std::to_string(i) + " (Julia SingularException)");
+}
+
+void release_lanes(Ctx* c) {
+  for (int i = 0; i < kLuExtraLanes; ++i) {
+    if (c->lstream[i]) (void)hipStreamSynchronize(c->lstream[i]);
+    if (c->lblas[i]) rocblas_destroy_handle(static_cast<rocblas_handle>(c->lblas[i]));
+    if (c->lstream[i]) (void)hipStreamDestroy(c->lstream[i]);
+    c->lblas[i] = nullptr;
+    c->lstream[i] = nullptr;
+  }
+  for (hipEvent_t& e : c->lane_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
 }
 
 // B[r0 + i, j] = -B[r0 + i, j] for i < nr, j < gridDim.y

@@ -96,6 +96,11 @@ struct Ctx {
   hipStream_t xstream = nullptr;
   // one event per l1 band of a banded contraction
   hipEvent_t band_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // extra streams, rocBLAS handles and events for independent factorizations
+  // run side by side (lu_solve_concurrent), created on first use
+  hipStream_t lstream[3] = {nullptr, nullptr, nullptr};
+  void* lblas[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t lane_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // servers created on this context and not yet destroyed: their worker
   // threads, arenas and resident spectra live on the context, so it is not
   // destroyed while any is alive
@@ -298,6 +303,23 @@ void lu_solve_dev(Ctx* c, int n, double* dA, int64_t lda, double* dB,
                   int64_t ldb, int nrhs);  // dA overwritten by LU
 void decouple_dev(Ctx* c, int n, const double* dY, double* dB1, double* dB2,
                   double* dOut);
+// One of several independent systems A X = B on the device (A overwritten by
+// its LU factors, B by X), with 1 to 4 right-hand sides and ipiv a device
+// array of n + 64 ints (LAPACK interchanges, info slot at n + 32).
+struct LuSystem {
+  int n;
+  double* dA;
+  int64_t lda;
+  double* dB;
+  int64_t ldb;
+  int nrhs;
+  int* ipiv;
+};
+constexpr int kLuExtraLanes = 3;
+// Factor and solve up to 1 + kLuExtraLanes systems at the same time, in
+// stream order on the context stream; returns when all are solved and raises
+// SingularException for a zero pivot.
+void lu_solve_concurrent(Ctx* c, const LuSystem* sys, int count);
 
 // status wrapper used by every extern "C" entry point
 template <class F>

@@ -2,10 +2,10 @@
 
 Mirrors src/modecoupling.jl:343-380 (maskedalm2spectra) and :317-341
 (master), and src/util.jl:3-18 (binning_matrix).  The five mode-coupling
-blocks come from one fused pass (psx_mcm_set: TT, TE (= TB), ET (= BT), ++,
---), the solves from the GPU LU: four factorisations (TT; TE for TE and TB;
-ET for ET and BT; the 2n x 2n EE/BB matrix for EE/BB and EB/BE) where the
-reference factors seven matrices.
+blocks come from one fused pass (TT, TE (= TB), ET (= BT), ++, --), the
+solves from the GPU LU: four factorisations (TT; TE for TE and TB; ET for ET
+and BT; the 2n x 2n EE/BB matrix for EE/BB and EB/BE) where the reference
+factors seven matrices, all in one device call that returns only the spectra.
 """
 from __future__ import annotations
 
@@ -13,9 +13,7 @@ import numpy as np
 
 from ._abi import ArgumentError, Context, default_context
 from .alm import Alm, alm2cl
-from .blockspectralmatrix import solve
-from .modecoupling import mcm_set
-from .spectralarray import SpectralArray, SpectralVector
+from .spectralarray import SpectralArray
 
 
 def _cross_spectra(sets, ctx: Context):
@@ -35,41 +33,32 @@ def _cross_spectra(sets, ctx: Context):
     return lambda i, j: alm2cl(sets[i], sets[j])
 
 
+_SPECTRA = ("TT", "TE", "TB", "ET", "BT", "EE", "BB", "EB", "BE")
+
+
 def _decoupled(W, P, lmin: int, lmax: int, ctx: Context) -> dict:
     """The nine spectra of maskedalm2spectra (src/modecoupling.jl:343-380) from
     the mask cross-spectra W(a, b) (a in "T1", "P1"; b in "T2", "P2") and the
-    pseudo-spectra P(X, Y) of map 1's X with map 2's Y."""
-    MTT, MTE, MET, Mpp, Mmm = mcm_set(W("T1", "T2")[: lmax + 1], W("T1", "P2")[: lmax + 1],
-                                      W("P1", "T2")[: lmax + 1], W("P1", "P2")[: lmax + 1],
-                                      lmin=lmin, lmax=lmax, ctx=ctx)
+    pseudo-spectra P(X, Y) of map 1's X with map 2's Y.
+
+    One device call (psx_master_decouple) builds the five blocks of mcm_set
+    and solves M \\ pCl for each spectrum (src/modecoupling.jl:355-377): TE and
+    TB share mcm(TE) = mcm(TB), ET and BT share mcm(ET), one factorisation
+    each with two right-hand sides; EE/BB and EB/BE are decoupled jointly by
+    [M++ M--; M-- M++] and [M++ -M--; -M-- M++].  The second is D A D for the
+    first, A, with D = diag(I, -I): [EB; BE] solves it exactly when [EB; -BE]
+    solves A against [pEB; -pBE].  Sign flips of rows and columns change
+    neither partial pivoting's choices nor any magnitude, so one factorisation
+    of A serves both systems.  The four factorisations run at the same time
+    on the device."""
+    Ws = [np.asarray(W(a, b), dtype=np.float64)[: lmax + 1]
+          for a, b in (("T1", "T2"), ("T1", "P2"), ("P1", "T2"), ("P1", "P2"))]
     # SpectralVector(alm2cl(a, b))[IdentityRange(lmin:end)] (src/modecoupling.jl:355)
-    pcl = lambda XY: SpectralVector(np.ascontiguousarray(P(XY[0], XY[1])[lmin: lmax + 1]),
-                                    lmin - 1)
-    out = {"TT": solve(MTT, pcl("TT"), ctx=ctx)}
-    # TE and TB share mcm(TE) = mcm(TB), ET and BT share mcm(ET): one
-    # factorisation each, the two pseudo-spectra as two right-hand sides
-    for M, pair in ((MTE, ("TE", "TB")), (MET, ("ET", "BT"))):
-        b = [pcl(XY) for XY in pair]
-        X = solve(M, SpectralArray(np.column_stack([v.parent for v in b]), firsts=(lmin, 1)),
-                  ctx=ctx)
-        for k, XY in enumerate(pair):
-            out[XY] = SpectralArray(np.ascontiguousarray(X.parent[:, k]), firsts=(lmin,))
-    # EE/BB and EB/BE are decoupled jointly (src/modecoupling.jl:362-377) by
-    # [M++ M--; M-- M++] and [M++ -M--; -M-- M++].  The second is D A D for the
-    # first, A, with D = diag(I, -I): [EB; BE] solves it exactly when
-    # [EB; -BE] solves A against [pEB; -pBE].  Sign flips of rows and columns
-    # change neither partial pivoting's choices nor any magnitude, so one
-    # factorisation of A, with two right-hand sides, serves both systems.
-    # The 2n x 2n matrix is assembled on the device (psx_lu_solve_pm).
-    n = lmax - lmin + 1
-    rhs = np.empty((2 * n, 2), order="F")
-    rhs[:n, 0], rhs[n:, 0] = pcl("EE").parent, pcl("BB").parent
-    rhs[:n, 1], rhs[n:, 1] = pcl("EB").parent, -pcl("BE").parent
-    X = ctx.lu_solve_pm(Mpp.parent, Mmm.parent, rhs, 1)
-    vec = lambda v: SpectralArray(np.ascontiguousarray(v), firsts=(lmin,))
-    out["EE"], out["BB"] = vec(X[:n, 0]), vec(X[n:, 0])
-    out["EB"], out["BE"] = vec(X[:n, 1]), vec(-X[n:, 1])
-    return out
+    pcl = np.stack([np.asarray(P(XY[0], XY[1]), dtype=np.float64)[lmin: lmax + 1]
+                    for XY in _SPECTRA])
+    X = ctx.master_decouple(lmin, lmax, *Ws, pcl)
+    return {XY: SpectralArray(np.ascontiguousarray(X[k]), firsts=(lmin,))
+            for k, XY in enumerate(_SPECTRA)}
 
 
 def maskedalm2spectra(maskedmap1, maskT1: Alm, maskP1: Alm, maskedmap2, maskT2: Alm,

@@ -275,3 +275,45 @@ def test_mcm_binned_arguments(pkg, ctx):
         pkg.mcm_binned("TT", V, P[:, :-1], lmax=100, ctx=ctx)
     with pytest.raises(pkg.AxisError):
         ctx.mcm_binned(0, 0, 100, V, P, P[:, :-1].T)
+
+
+@pytest.mark.parametrize("lmin,lmax", [(2, 400), (2, 1601)])
+def test_master_decouple_equals_block_path(ctx, lmin, lmax):
+    """psx_master_decouple (the five blocks and four concurrent solves on the
+    device) against the same blocks through the host path: mcm_set, then
+    lu_solve for TT, TE | TB, ET | BT and lu_solve_pm for EE/BB and EB/BE
+    (src/modecoupling.jl:348-377).  At lmax 1601 the 2n x 2n system (3200
+    rows) takes the multi-workgroup leaves while the n x n ones take the
+    single-workgroup leaves, at the same time.  (lmin >= 2: below l = 2 the
+    spin-2 blocks vanish and the EE/BB system is singular, as in Julia.)"""
+    from powerspectra_amd._abi import SingularException
+    LW = lmax + 1
+    Ws = [mask_like_W(LW, 11), mask_like_W(LW, 12, cross=True),
+          mask_like_W(LW, 13, cross=True), mask_like_W(LW, 14)]
+    n = lmax - lmin + 1
+    rng = np.random.default_rng(lmax)
+    pcl = rng.standard_normal((9, n)) * np.exp(-np.arange(lmin, lmax + 1) / 300.0)
+    got = ctx.master_decouple(lmin, lmax, *Ws, pcl)
+    MTT, MTE, MET, Mpp, Mmm = ctx.mcm_set(lmin, lmax, *Ws)
+    want = np.empty((9, n))
+    want[0] = ctx.lu_solve(MTT, pcl[0])
+    want[1:3] = ctx.lu_solve(MTE, pcl[1:3].T).T
+    want[3:5] = ctx.lu_solve(MET, pcl[3:5].T).T
+    X = ctx.lu_solve_pm(Mpp, Mmm, np.concatenate([pcl[5], pcl[6]]), 1)
+    want[5], want[6] = X[:n], X[n:]
+    X = ctx.lu_solve_pm(Mpp, Mmm, np.concatenate([pcl[7], pcl[8]]), -1)
+    want[7], want[8] = X[:n], X[n:]
+    for k in range(9):
+        assert np.abs(got[k] - want[k]).max() <= RTOL_FP64 * np.abs(want[k]).max(), k
+    # independent check: numpy's LAPACK on the host blocks
+    ref = np.linalg.solve(MTT, pcl[0])
+    assert np.abs(got[0] - ref).max() <= 1e-9 * np.abs(ref).max()
+    A = np.block([[Mpp, -Mmm], [-Mmm, Mpp]])
+    ref = np.linalg.solve(A, np.concatenate([pcl[7], pcl[8]]))
+    assert np.abs(np.concatenate([got[7], got[8]]) - ref).max() <= 1e-9 * np.abs(ref).max()
+    # a second call with all-zero W: every block is singular
+    Z = np.zeros(LW)
+    with pytest.raises(SingularException):
+        ctx.master_decouple(lmin, lmax, Z, Z, Z, Z, pcl)
+    again = ctx.master_decouple(lmin, lmax, *Ws, pcl)
+    assert np.array_equal(again, got)
