@@ -277,15 +277,16 @@ def test_mcm_binned_arguments(pkg, ctx):
         ctx.mcm_binned(0, 0, 100, V, P, P[:, :-1].T)
 
 
-@pytest.mark.parametrize("lmin,lmax", [(2, 400), (2, 1601)])
+@pytest.mark.parametrize("lmin,lmax", [(2, 400), (2, 1601), (2, 3071)])
 def test_master_decouple_equals_block_path(ctx, lmin, lmax):
     """psx_master_decouple (the five blocks and four concurrent solves on the
     device) against the same blocks through the host path: mcm_set, then
     lu_solve for TT, TE | TB, ET | BT and lu_solve_pm for EE/BB and EB/BE
     (src/modecoupling.jl:348-377).  At lmax 1601 the 2n x 2n system (3200
     rows) takes the multi-workgroup leaves while the n x n ones take the
-    single-workgroup leaves, at the same time.  (lmin >= 2: below l = 2 the
-    spin-2 blocks vanish and the EE/BB system is singular, as in Julia.)"""
+    single-workgroup leaves, at the same time; lmax 3071 is master's size
+    (n = 3070, the 2n system 6140 rows).  (lmin >= 2: below l = 2 the spin-2
+    blocks vanish and the EE/BB system is singular, as in Julia.)"""
     from powerspectra_amd._abi import SingularException
     LW = lmax + 1
     Ws = [mask_like_W(LW, 11), mask_like_W(LW, 12, cross=True),
