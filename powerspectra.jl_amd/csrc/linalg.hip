@@ -983,21 +983,29 @@ void lu_solve_concurrent(Ctx* c, const LuSystem* sys, int count) {
   char* work = (char*)ctx_scratch(c, off[count]);
   blas(c);
   PSX_HIP(hipEventRecord(c->lane_ev[0], c->stream));
-  for (int i = 0; i < count; ++i) {
-    const LuSystem& q = sys[i];
-    const LuLane L = i == 0 ? LuLane{c->stream, blas(c), work}
-                            : LuLane{c->lstream[i - 1], (rocblas_handle)c->lblas[i - 1],
-                                     work + off[i]};
-    if (i > 0) PSX_HIP(hipStreamWaitEvent(L.s, c->lane_ev[0], 0));
-    lu_factor_on(L, q.n, q.dA, q.lda, q.ipiv, q.ipiv + q.n + 32);
-    PSX_ROC(rocsolver_dlaswp(L.h, q.nrhs, q.dB, (rocblas_int)q.ldb, 1, q.n,
-                             (const rocblas_int*)q.ipiv, 1));
-    for (int j = 0; j < q.nrhs; ++j) lu_trs_on(L.s, q.n, q.dA, q.lda, q.dB + (int64_t)j * q.ldb);
-    PSX_HIP(hipGetLastError());
-    if (i > 0) {
-      PSX_HIP(hipEventRecord(c->lane_ev[i], L.s));
-      PSX_HIP(hipStreamWaitEvent(c->stream, c->lane_ev[i], 0));
+  try {
+    for (int i = 0; i < count; ++i) {
+      const LuSystem& q = sys[i];
+      const LuLane L = i == 0 ? LuLane{c->stream, blas(c), work}
+                              : LuLane{c->lstream[i - 1], (rocblas_handle)c->lblas[i - 1],
+                                       work + off[i]};
+      if (i > 0) PSX_HIP(hipStreamWaitEvent(L.s, c->lane_ev[0], 0));
+      lu_factor_on(L, q.n, q.dA, q.lda, q.ipiv, q.ipiv + q.n + 32);
+      PSX_ROC(rocsolver_dlaswp(L.h, q.nrhs, q.dB, (rocblas_int)q.ldb, 1, q.n,
+                               (const rocblas_int*)q.ipiv, 1));
+      for (int j = 0; j < q.nrhs; ++j)
+        lu_trs_on(L.s, q.n, q.dA, q.lda, q.dB + (int64_t)j * q.ldb);
+      PSX_HIP(hipGetLastError());
+      if (i > 0) {
+        PSX_HIP(hipEventRecord(c->lane_ev[i], L.s));
+        PSX_HIP(hipStreamWaitEvent(c->stream, c->lane_ev[i], 0));
+      }
     }
+  } catch (...) {
+    // nothing may still run on the caller's buffers once the error is out
+    for (int i = 0; i + 1 < count; ++i) (void)hipStreamSynchronize(c->lstream[i]);
+    (void)hipStreamSynchronize(c->stream);
+    throw;
   }
   // SingularException for the first system with a zero pivot
   std::vector<int> info(count, 0);
