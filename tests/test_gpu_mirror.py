@@ -318,3 +318,23 @@ def test_master_decouple_equals_block_path(ctx, lmin, lmax):
         ctx.master_decouple(lmin, lmax, Z, Z, Z, Z, pcl)
     again = ctx.master_decouple(lmin, lmax, *Ws, pcl)
     assert np.array_equal(again, got)
+
+
+def test_master_decouple_arguments(ctx, pkg):
+    """psx_master_decouple refuses W spectra of different lengths, pseudo-spectra
+    of the wrong shape, lmax below lmin and a null pointer, before any work."""
+    import ctypes
+    lmax = 40
+    W = mask_like_W(lmax + 1, 3)
+    pcl = np.ones((9, lmax - 1))
+    with pytest.raises(pkg.ArgumentError):
+        ctx.master_decouple(2, lmax, W, W, W[:-1], W, pcl)
+    with pytest.raises(pkg.AxisError):
+        ctx.master_decouple(2, lmax, W, W, W, W, pcl[:8])
+    with pytest.raises(pkg.ArgumentError):
+        ctx.master_decouple(5, 4, W, W, W, W, np.ones((9, 0)))
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    out = np.empty((9, lmax - 1))
+    st = ctx.lib.psx_master_decouple(ctx.handle, 2, lmax, W.size, dp(W), dp(W), dp(W), dp(W),
+                                     ctypes.cast(None, ctypes.POINTER(ctypes.c_double)), dp(out))
+    assert st == pkg._abi.PSX_ERR_ARGUMENT
