@@ -469,36 +469,60 @@ __global__ void __launch_bounds__(NMAX / 16) polar_idft(RingArgs R, const double
 // the same n and phi0 (pi/n or 0), so with m = a n + q (0 <= q < n) the
 // shift e^{-i m phi0} = s^a e^{-i q phi0} (s = -1 where phi0 != 0) is one
 // small-angle sincos for both; X[q] = conj(X[n - q]) above n/2.
+// A workgroup takes kFaPairs ring pairs and kFaM consecutive m: each wave
+// reads four pairs' spectra along m (consecutive q: one cache line serves
+// eight lanes), the sums pass through LDS, and F+- leave as runs of
+// kFaPairs pairs per m.
+constexpr int kFaPairs = 16, kFaM = 64;
+
 __global__ void __launch_bounds__(256) fold_analysis(RingArgs R, const double2* __restrict__ X,
                                                      double2* __restrict__ Fp,
                                                      double2* __restrict__ Fm) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
-  if (p >= R.npairs) return;
-  const int kn = p, ks = R.nr - 1 - p;
-  const int n = R.n[kn];
-  const int a = m / n, q = m - a * n;
-  const double phi0 = R.phi0[kn];
-  double sn, c;
-  sincos((double)q * phi0, &sn, &c);
-  if (phi0 != 0.0 && (a & 1)) {
-    sn = -sn;
-    c = -c;
+  __shared__ double2 tp[kFaM][kFaPairs + 1], tm[kFaM][kFaPairs + 1];
+  const int p0 = (int)blockIdx.x * kFaPairs, m0 = (int)blockIdx.y * kFaM;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int m = m0 + lane;
+#pragma unroll
+  for (int k = 0; k < kFaPairs / 4; ++k) {
+    const int pl = wv * (kFaPairs / 4) + k, p = p0 + pl;
+    double2 A = make_double2(0.0, 0.0), B = make_double2(0.0, 0.0);
+    if (p < R.npairs && m <= R.lmax) {
+      const int kn = p, ks = R.nr - 1 - p;
+      const int n = R.n[kn];
+      const int a = m / n, q = m - a * n;
+      const double phi0 = R.phi0[kn];
+      double sn, c;
+      sincos((double)q * phi0, &sn, &c);
+      if (phi0 != 0.0 && (a & 1)) {
+        sn = -sn;
+        c = -c;
+      }
+      const bool hi = 2 * q > n;
+      const int qi = hi ? n - q : q;
+      const double cy = hi ? -1.0 : 1.0;
+      // v e^{-i m phi0}
+      auto coef = [&](int k2) {
+        const double2 v = X[R.xoff[k2] + qi];
+        const double vy = cy * v.y;
+        return make_double2(v.x * c + vy * sn, vy * c - v.x * sn);
+      };
+      A = coef(kn);
+      if (ks != kn) B = coef(ks);
+    }
+    tp[lane][pl] = make_double2(A.x + B.x, A.y + B.y);
+    tm[lane][pl] = make_double2(A.x - B.x, A.y - B.y);
   }
-  const bool hi = 2 * q > n;
-  const int qi = hi ? n - q : q;
-  const double cy = hi ? -1.0 : 1.0;
-  // v e^{-i m phi0}
-  auto coef = [&](int k) {
-    const double2 v = X[R.xoff[k] + qi];
-    const double vy = cy * v.y;
-    return make_double2(v.x * c + vy * sn, vy * c - v.x * sn);
-  };
-  const double2 A = coef(kn);
-  const double2 B = ks != kn ? coef(ks) : make_double2(0.0, 0.0);
-  const int64_t o = (int64_t)m * R.npairs + p;
-  Fp[o] = make_double2(A.x + B.x, A.y + B.y);
-  Fm[o] = make_double2(A.x - B.x, A.y - B.y);
+  __syncthreads();
+  const int pl = (int)threadIdx.x % kFaPairs, p = p0 + pl;
+#pragma unroll
+  for (int jj = 0; jj < kFaM * kFaPairs / 256; ++jj) {
+    const int ml = (int)threadIdx.x / kFaPairs + (256 / kFaPairs) * jj, mm = m0 + ml;
+    if (p < R.npairs && mm <= R.lmax) {
+      const int64_t o = (int64_t)mm * R.npairs + p;
+      Fp[o] = tp[ml][pl];
+      Fm[o] = tm[ml][pl];
+    }
+  }
 }
 
 // Recurrence coefficients per (l, m), l >= m + 2, in a_lm order:
@@ -1614,7 +1638,8 @@ void fold_in(Ctx* c, ShtPlan& P, const ShtBufs& B, const double* dmap) {
     void* out[1] = {(void*)(B.X + R.xoff[ns - 1])};
     PSX_FFT(rocfft_execute(P.fwd, in, out, P.info));
   }
-  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)),
+  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + kFaPairs - 1) / kFaPairs),
+                                         (unsigned)((P.lmax + kFaM) / kFaM)),
                      dim3(256), 0, s, P.ra(), B.X, B.Fp, B.Fm);
   PSX_HIP(hipGetLastError());
 }
@@ -1764,7 +1789,8 @@ void ring_fold(Ctx* c, ShtPlan& P, const double* dmap, double2* X, double2* Fp, 
     void* out[1] = {(void*)(X + R.xoff[ns - 1])};
     PSX_FFT(rocfft_execute(P.fwd, in, out, P.info));
   }
-  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + 255) / 256), (unsigned)(P.lmax + 1)),
+  hipLaunchKernelGGL(fold_analysis, dim3((unsigned)((R.npairs + kFaPairs - 1) / kFaPairs),
+                                         (unsigned)((P.lmax + kFaM) / kFaM)),
                      dim3(256), 0, s, P.ra(), X, Fp, Fm);
   PSX_HIP(hipGetLastError());
 }
