@@ -54,6 +54,9 @@ struct QpArgs {
 // kQpLayer with every odd layer reversed: the SIMD that holds one of the
 // longest families of the first layer also gets one of the shortest of the
 // second.
+#ifndef PSX_QP_PRIO
+#define PSX_QP_PRIO 1
+#endif
 #ifndef PSX_QP_LAYER
 #define PSX_QP_LAYER 1024
 #endif
@@ -163,6 +166,13 @@ __global__ void __launch_bounds__(64) quickpol_xi_kernel(QpArgs A) {
   const int na = max(ad, abs(A.a.M1)), nb = max(ad, abs(A.b.M1));
   const int jc = max(na, nb);
   const int jend = 2 * lpp0 + d;                  // largest l + l'' of the wavefront
+#if PSX_QP_PRIO
+  // issue priority by chain length: where a SIMD holds a long and a short
+  // wavefront, the long one (which sets the kernel's time) issues first
+  if (4 * jend >= 3 * 2 * A.lmax) __builtin_amdgcn_s_setprio(3);
+  else if (2 * jend >= 2 * A.lmax) __builtin_amdgcn_s_setprio(2);
+  else if (4 * jend >= 2 * A.lmax) __builtin_amdgcn_s_setprio(1);
+#endif
   // lanes without a cell run the recurrence of lane 0's cell and discard it
   const int nmax = live ? l + lpp : jend;
   const int ell = live ? l : lpp0 + d, ellpp = live ? lpp : lpp0;
