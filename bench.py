@@ -1119,7 +1119,9 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    # the set kernel's launch time settles over its first ~20 launches (clock
+    # ramp under load): 30 untimed launches put the timed ones at steady state
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--lmax", type=int, default=3071)
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
